@@ -1,0 +1,215 @@
+"""ctypes binding of libdfrocm (include/dfrocm.h).
+
+This is the Python twin of the binding a Mojo provider makes with
+``OwnedDLHandle`` (dataframe/io/native_library.mojo:22-110): the library is
+found next to this file or through ``DATAFRAME_ROCM_LIBRARY``; every call
+returns a status and a malloc'd error string freed with ``dfr_free``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_ENV = "DATAFRAME_ROCM_LIBRARY"
+
+DFR_OK = 0
+DFR_UNSUPPORTED = 1
+DFR_DATA_FAULT = 2
+DFR_CAPACITY = 3
+DFR_DEVICE = 4
+DFR_INVALID = 5
+
+DFR_INT32 = 1
+DFR_INT64 = 2
+DFR_FLOAT64 = 3
+DFR_DICT = 4
+
+EQ, NE, LT, LE, GT, GE, IS_NULL, IS_NOT_NULL = range(8)
+LEAF_MOD = 1
+LEAF_NOT = 2
+MAX_LEAVES = 8
+
+AGG_LEN, AGG_COUNT, AGG_SUM, AGG_MIN, AGG_MAX, AGG_MEAN, AGG_VAR, AGG_STD = range(8)
+
+# every symbol include/dfrocm.h declares
+EXPORTS = [
+    "dfr_abi_version",
+    "dfr_free",
+    "dfr_device_count",
+    "dfr_device_info_get",
+    "dfr_context_create",
+    "dfr_context_destroy",
+    "dfr_context_observe",
+    "dfr_context_tune",
+    "dfr_column_upload",
+    "dfr_column_wrap",
+    "dfr_column_release",
+    "dfr_column_length",
+    "dfr_column_null_count",
+    "dfr_groupby",
+    "dfr_result_groups",
+    "dfr_result_key_dtype",
+    "dfr_result_agg_dtype",
+    "dfr_result_copy_key",
+    "dfr_result_copy_agg",
+    "dfr_result_copy_first_rows",
+    "dfr_result_device_bytes",
+    "dfr_result_device_ms",
+    "dfr_result_phase_ms",
+    "dfr_result_release",
+]
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [
+        ("ordinal", ctypes.c_int32),
+        ("compute_units", ctypes.c_int32),
+        ("memory_bytes", ctypes.c_int64),
+        ("name", ctypes.c_char * 64),
+        ("arch", ctypes.c_char * 32),
+    ]
+
+
+class Observation(ctypes.Structure):
+    _fields_ = [
+        (n, ctypes.c_int64)
+        for n in (
+            "launches",
+            "completed",
+            "upload_bytes",
+            "download_bytes",
+            "reserved_bytes",
+            "peak_reserved_bytes",
+            "waits",
+            "queries",
+        )
+    ]
+
+
+class Leaf(ctypes.Structure):
+    _fields_ = [
+        ("column", ctypes.c_int32),
+        ("op", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("literal", ctypes.c_int64),
+        ("modulus", ctypes.c_int64),
+    ]
+
+
+class Predicate(ctypes.Structure):
+    _fields_ = [
+        ("n_leaves", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("truth", ctypes.c_uint32 * 8),
+        ("leaves", Leaf * MAX_LEAVES),
+    ]
+
+
+class Agg(ctypes.Structure):
+    _fields_ = [
+        ("op", ctypes.c_int32),
+        ("column", ctypes.c_int32),
+        ("ddof", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("min_count", ctypes.c_int64),
+    ]
+
+
+class Query(ctypes.Structure):
+    _fields_ = [
+        ("n_keys", ctypes.c_int32),
+        ("n_columns", ctypes.c_int32),
+        ("keys", ctypes.POINTER(ctypes.c_void_p)),
+        ("columns", ctypes.POINTER(ctypes.c_void_p)),
+        ("predicate", Predicate),
+        ("n_aggs", ctypes.c_int32),
+        ("maintain_order", ctypes.c_int32),
+        ("aggs", ctypes.POINTER(Agg)),
+    ]
+
+
+def library_path() -> str:
+    return os.environ.get(LIBRARY_ENV) or os.path.join(HERE, "libdfrocm.so")
+
+
+_lib = None
+
+
+def load(required: bool = True):
+    """Loads libdfrocm; raises when it is missing and `required`."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        if required:
+            raise RuntimeError(
+                f"libdfrocm is not built ({path} missing); run python -m dataframe_mojo_amd.build"
+            )
+        return None
+    lib = ctypes.CDLL(path)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    err = ctypes.POINTER(ctypes.c_char_p)
+    errp = ctypes.POINTER(ctypes.c_void_p)
+    lib.dfr_abi_version.restype = ctypes.c_int
+    lib.dfr_free.argtypes = [vp]
+    lib.dfr_device_count.argtypes = [ctypes.POINTER(i32), errp]
+    lib.dfr_device_info_get.argtypes = [i32, ctypes.POINTER(DeviceInfo), errp]
+    lib.dfr_context_create.argtypes = [i32, i64, ctypes.POINTER(vp), errp]
+    lib.dfr_context_destroy.argtypes = [vp]
+    lib.dfr_context_observe.argtypes = [vp, ctypes.POINTER(Observation)]
+    lib.dfr_context_tune.argtypes = [vp, ctypes.c_char_p, i64, errp]
+    lib.dfr_column_upload.argtypes = [vp, i32, vp, i64, i64, vp, i64, i64, ctypes.POINTER(vp), errp]
+    lib.dfr_column_wrap.argtypes = [vp, i32, vp, i64, i64, vp, i64, i64, ctypes.POINTER(vp), errp]
+    lib.dfr_column_release.argtypes = [vp]
+    lib.dfr_column_length.argtypes = [vp]
+    lib.dfr_column_length.restype = i64
+    lib.dfr_column_null_count.argtypes = [vp]
+    lib.dfr_column_null_count.restype = i64
+    lib.dfr_groupby.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(vp), errp]
+    lib.dfr_result_groups.argtypes = [vp]
+    lib.dfr_result_groups.restype = i64
+    lib.dfr_result_key_dtype.argtypes = [vp, i32]
+    lib.dfr_result_agg_dtype.argtypes = [vp, i32]
+    lib.dfr_result_copy_key.argtypes = [vp, i32, vp, vp, errp]
+    lib.dfr_result_copy_agg.argtypes = [vp, i32, vp, vp, errp]
+    lib.dfr_result_copy_first_rows.argtypes = [vp, vp, errp]
+    lib.dfr_result_device_bytes.argtypes = [vp]
+    lib.dfr_result_device_bytes.restype = i64
+    lib.dfr_result_device_ms.argtypes = [vp]
+    lib.dfr_result_device_ms.restype = ctypes.c_double
+    lib.dfr_result_phase_ms.argtypes = [vp, i32]
+    lib.dfr_result_phase_ms.restype = ctypes.c_double
+    lib.dfr_result_release.argtypes = [vp]
+    _lib = lib
+    return lib
+
+
+class DfrError(RuntimeError):
+    """A libdfrocm failure: `status` is the DFR_* code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+class DataFault(DfrError):
+    """An aggregate out of its result type's range (the reference's
+    "... expression sum overflow")."""
+
+
+class Unsupported(DfrError):
+    """Work the device path does not take; raised before submission."""
+
+
+def check(status: int, err: ctypes.c_void_p) -> None:
+    if status == DFR_OK:
+        return
+    lib = load()
+    msg = ctypes.string_at(err.value).decode() if err.value else f"libdfrocm status {status}"
+    if err.value:
+        lib.dfr_free(err)
+    cls = {DFR_DATA_FAULT: DataFault, DFR_UNSUPPORTED: Unsupported}.get(status, DfrError)
+    raise cls(status, msg)

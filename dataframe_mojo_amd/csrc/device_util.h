@@ -1,0 +1,219 @@
+// device_util.h — integer, fixed-point and float-ordering helpers shared by
+// the kernels (and, where marked __host__, by the host finish code).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dfr {
+
+#define DFR_HD __host__ __device__ __forceinline__
+
+DFR_HD uint64_t f64_bits(double x) {
+  uint64_t u;
+  __builtin_memcpy(&u, &x, 8);
+  return u;
+}
+DFR_HD double bits_f64(uint64_t u) {
+  double x;
+  __builtin_memcpy(&x, &u, 8);
+  return x;
+}
+
+// Ordered key of a non-NaN double: unsigned order of keys is numeric order,
+// and -0.0 shares +0.0's key (the two compare equal).
+DFR_HD uint64_t ord_key(uint64_t bits) {
+  if ((bits << 1) == 0) bits = 0;  // -0.0 -> +0.0
+  return (bits >> 63) ? ~bits : (bits | (1ull << 63));
+}
+DFR_HD uint64_t ord_key_inverse(uint64_t key) {
+  return (key >> 63) ? (key & ~(1ull << 63)) : ~key;
+}
+
+// 128-bit shifts, shift amounts in [0, 127]; larger right shifts give 0.
+DFR_HD void shl128(uint64_t &lo, uint64_t &hi, int s) {
+  if (s <= 0) return;
+  if (s >= 64) {
+    hi = lo << (s - 64);
+    lo = 0;
+  } else {
+    hi = (hi << s) | (lo >> (64 - s));
+    lo <<= s;
+  }
+}
+DFR_HD void shr128(uint64_t &lo, uint64_t &hi, int s) {
+  if (s <= 0) return;
+  if (s >= 128) {
+    lo = hi = 0;
+  } else if (s >= 64) {
+    lo = hi >> (s - 64);
+    hi = 0;
+  } else {
+    lo = (lo >> s) | (hi << (64 - s));
+    hi >>= s;
+  }
+}
+
+// Fixed-point image of a finite double: trunc(|x| * 2^S) with x's sign, as
+// a two's-complement 128-bit integer. The caller's S keeps the magnitude
+// below 2^127.
+DFR_HD void fix_of(uint64_t bits, int S, uint64_t &lo, uint64_t &hi) {
+  int e = (int)((bits >> 52) & 0x7ff);
+  uint64_t m = bits & ((1ull << 52) - 1);
+  if (e == 0)
+    e = 1;
+  else
+    m |= 1ull << 52;
+  int sh = e - 1075 + S;  // value = m * 2^(e - 1075)
+  uint64_t l = m, h = 0;
+  if (sh >= 0)
+    shl128(l, h, sh);
+  else
+    shr128(l, h, -sh);
+  if (bits >> 63) {
+    l = ~l + 1;
+    h = ~h + (l == 0 ? 1 : 0);
+  }
+  lo = l;
+  hi = h;
+}
+
+// Fixed-point image of x*x: trunc(x^2 * 2^S2), non-negative.
+DFR_HD void fixsq_of(uint64_t bits, int S2, uint64_t &lo, uint64_t &hi) {
+  int e = (int)((bits >> 52) & 0x7ff);
+  uint64_t m = bits & ((1ull << 52) - 1);
+  if (e == 0)
+    e = 1;
+  else
+    m |= 1ull << 52;
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t ph = __umul64hi(m, m);
+#else
+  uint64_t ph = (uint64_t)(((unsigned __int128)m * m) >> 64);
+#endif
+  uint64_t pl = m * m;
+  int sh = 2 * (e - 1075) + S2;
+  if (sh >= 0)
+    shl128(pl, ph, sh);
+  else
+    shr128(pl, ph, -sh);
+  lo = pl;
+  hi = ph;
+}
+
+// Correctly rounded (nearest-even) conversion of an unsigned 64-bit value.
+DFR_HD double u64_to_f64(uint64_t u) {
+  double h = (double)(uint32_t)(u >> 32);
+  double l = (double)(uint32_t)u;
+  return h * 4294967296.0 + l;  // both operands exact: one rounding
+}
+
+// Correctly rounded conversion of a 128-bit two's-complement integer,
+// times 2^-scale.
+DFR_HD double i128_to_f64(uint64_t lo, uint64_t hi, int scale) {
+  bool neg = (int64_t)hi < 0;
+  if (neg) {
+    lo = ~lo + 1;
+    hi = ~hi + (lo == 0 ? 1 : 0);
+  }
+  double d;
+  if (hi == 0) {
+    d = u64_to_f64(lo);
+    d = ldexp(d, -scale);
+  } else {
+    int lz = __builtin_clzll(hi);
+    int s = 64 - lz;  // right shift that leaves 64 significant bits
+    uint64_t top = (lz == 0) ? hi : ((hi << lz) | (lo >> s));
+    uint64_t dropped = (s == 64) ? lo : (lo << (64 - s));
+    top |= (dropped != 0) ? 1 : 0;  // round to odd keeps the final rounding exact
+    d = ldexp(u64_to_f64(top), s - scale);
+  }
+  return neg ? -d : d;
+}
+
+// 256-bit unsigned arithmetic for the variance finish.
+struct U256 {
+  uint64_t w[4];
+};
+
+DFR_HD void mul64(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  hi = __umul64hi(a, b);
+  lo = a * b;
+#else
+  unsigned __int128 p = (unsigned __int128)a * b;
+  lo = (uint64_t)p;
+  hi = (uint64_t)(p >> 64);
+#endif
+}
+
+// (a_lo, a_hi) * (b_lo, b_hi) as 256 bits.
+DFR_HD U256 mul128(uint64_t alo, uint64_t ahi, uint64_t blo, uint64_t bhi) {
+  U256 r = {{0, 0, 0, 0}};
+  uint64_t a[2] = {alo, ahi}, b[2] = {blo, bhi};
+  for (int i = 0; i < 2; ++i) {
+    uint64_t carry = 0;
+    for (int j = 0; j < 2; ++j) {
+      uint64_t plo, phi;
+      mul64(a[i], b[j], plo, phi);
+      uint64_t t = r.w[i + j] + plo;
+      uint64_t c1 = t < plo ? 1 : 0;
+      uint64_t t2 = t + carry;
+      uint64_t c2 = t2 < t ? 1 : 0;
+      r.w[i + j] = t2;
+      carry = phi + c1 + c2;
+    }
+    r.w[i + 2] += carry;
+  }
+  return r;
+}
+
+DFR_HD U256 shl256(U256 x, int s) {
+  U256 r = {{0, 0, 0, 0}};
+  int q = s >> 6, b = s & 63;
+  for (int i = 3; i >= 0; --i) {
+    int src = i - q;
+    if (src < 0) continue;
+    uint64_t v = x.w[src] << b;
+    if (b && src - 1 >= 0) v |= x.w[src - 1] >> (64 - b);
+    r.w[i] = v;
+  }
+  return r;
+}
+
+// a - b; returns false when the difference is negative (then r = 0).
+DFR_HD bool sub256(U256 a, U256 b, U256 &r) {
+  uint64_t borrow = 0;
+  for (int i = 0; i < 4; ++i) {
+    uint64_t d = a.w[i] - b.w[i];
+    uint64_t b1 = a.w[i] < b.w[i] ? 1 : 0;
+    uint64_t d2 = d - borrow;
+    uint64_t b2 = d < borrow ? 1 : 0;
+    r.w[i] = d2;
+    borrow = b1 | b2;
+  }
+  if (borrow) {
+    r.w[0] = r.w[1] = r.w[2] = r.w[3] = 0;
+    return false;
+  }
+  return true;
+}
+
+// Correctly rounded value of x * 2^-scale.
+DFR_HD double u256_to_f64(U256 x, int scale) {
+  int top = 3;
+  while (top > 0 && x.w[top] == 0) --top;
+  if (top == 0) return ldexp(u64_to_f64(x.w[0]), -scale);
+  int lz = __builtin_clzll(x.w[top]);
+  // 64 significant bits starting at the leading one
+  uint64_t hi = x.w[top], lo = x.w[top - 1];
+  uint64_t t = lz ? ((hi << lz) | (lo >> (64 - lz))) : hi;
+  uint64_t sticky = lz ? (lo << lz) : lo;
+  for (int i = top - 2; i >= 0; --i) sticky |= x.w[i];
+  t |= sticky ? 1 : 0;
+  // the leading one is bit 64*top + 63 - lz, so t's lowest bit weighs
+  // 2^(64*top - lz)
+  int exp = 64 * top - lz;
+  return ldexp(u64_to_f64(t), exp - scale);
+}
+
+}  // namespace dfr

@@ -1,0 +1,23 @@
+// groupby.h — the compiled form of a query on the host side.
+#pragma once
+#include <vector>
+
+#include "plan.h"
+#include "runtime.h"
+
+struct Compiled {
+  dfr::DevPlan plan;
+  std::vector<dfr_column *> cols;  // key columns first
+  std::vector<int32_t> key_dtypes, agg_dtypes;
+  int64_t rows = 0;
+  bool maintain_order = false;
+  // launch shape
+  int block = 512;
+  int grid = 1;
+  int64_t chunk = 0;
+  size_t lds_bytes = 0;
+};
+
+int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **error);
+void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots);
+int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error);

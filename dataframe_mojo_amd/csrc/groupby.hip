@@ -1,0 +1,673 @@
+// groupby.hip — query compiler (dfr_query -> DevPlan) and the launch
+// sequence of one filtered group-by, plus the column statistics pass.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/dfrocm.h"
+#include "groupby.h"
+#include "kernels.h"
+#include "runtime.h"
+
+using namespace dfr;
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return dfr_fail(error, DFR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+__global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t value) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+int bitlen_u64(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
+uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+int log2_exact(uint64_t p) { return 63 - __builtin_clzll(p); }
+
+int env_int(const char *name, int dflt) {
+  const char *v = getenv(name);
+  return v && *v ? atoi(v) : dflt;
+}
+
+}  // namespace
+
+Tuning default_tuning(dfr_context *ctx) {
+  Tuning t;
+  t.block = env_int("DFR_BLOCK", 512);
+  t.wg_per_cu = env_int("DFR_WG_PER_CU", 2);
+  t.lds_bytes = std::min(env_int("DFR_LDS_BYTES", 65536), ctx->lds_per_block);
+  t.lds_probe = env_int("DFR_LDS_PROBE", 4);
+  t.lds_mode = env_int("DFR_LDS_MODE", -1);
+  return t;
+}
+
+int column_stats(dfr_context *ctx, dfr_column *col, char **error) {
+  ColStats init;
+  init.imin = INT64_MAX;
+  init.imax = INT64_MIN;
+  init.emin = 0x7ff;
+  init.emax = 0;
+  init.flags = 0;
+  init.pad = 0;
+  init.valid = 0;
+  ColStats *d = nullptr;
+  int st = ctx->pool.alloc(sizeof(ColStats), (void **)&d, error);
+  if (st) return st;
+  ColStats *h = (ColStats *)ctx->staging;
+  *h = init;
+  HIP_TRY(hipMemcpyAsync(d, h, sizeof init, hipMemcpyHostToDevice, ctx->stream));
+  DevCol C = col->dev();
+  int64_t n = col->length;
+  int blocks = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, (int64_t)ctx->cus * 8);
+  if (blocks > 0) {
+    hipLaunchKernelGGL(k_col_stats, dim3(blocks), dim3(kBlock), 0, ctx->stream, C, n, d);
+    ctx->obs.launches++;
+  }
+  HIP_TRY(hipMemcpyAsync(h, d, sizeof(ColStats), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->obs.waits++;
+  ctx->obs.completed = ctx->obs.launches;
+  col->stats = *h;
+  col->nulls = n - (int64_t)h->valid;
+  ctx->pool.release(d);
+  return DFR_OK;
+}
+
+// ------------------------------------------------------------ compiler
+
+int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **error) {
+  if (!q) return dfr_fail(error, DFR_INVALID, "query is null");
+  if (q->n_keys < 1 || q->n_keys > kMaxKeys || !q->keys)
+    return dfr_fail(error, DFR_UNSUPPORTED, "the device path groups by 1 to %d keys", kMaxKeys);
+  if (q->n_columns < 0 || (q->n_columns > 0 && !q->columns))
+    return dfr_fail(error, DFR_INVALID, "bad column list");
+  if (q->n_aggs < 0 || q->n_aggs > kMaxAggs || (q->n_aggs > 0 && !q->aggs))
+    return dfr_fail(error, DFR_UNSUPPORTED, "at most %d aggregates", kMaxAggs);
+  if (q->predicate.n_leaves < 0 || q->predicate.n_leaves > kMaxLeaves)
+    return dfr_fail(error, DFR_UNSUPPORTED, "at most %d predicate leaves", kMaxLeaves);
+
+  DevPlan &P = C.plan;
+  memset(&P, 0, sizeof P);
+  C.cols.clear();
+  for (int k = 0; k < q->n_keys; ++k) {
+    if (!q->keys[k]) return dfr_fail(error, DFR_INVALID, "key %d is null", k);
+    C.cols.push_back(q->keys[k]);
+  }
+  std::vector<int> colmap(q->n_columns);
+  for (int i = 0; i < q->n_columns; ++i) {
+    dfr_column *c = q->columns[i];
+    if (!c) return dfr_fail(error, DFR_INVALID, "column %d is null", i);
+    int at = -1;
+    for (size_t j = q->n_keys; j < C.cols.size(); ++j)
+      if (C.cols[j] == c) at = (int)j;
+    if (at < 0) {
+      // a key column used as an input is loaded once, as the key
+      for (int j = 0; j < q->n_keys; ++j)
+        if (C.cols[j] == c) at = j;
+    }
+    if (at < 0) {
+      if ((int)C.cols.size() >= kMaxCols)
+        return dfr_fail(error, DFR_UNSUPPORTED, "at most %d distinct input columns", kMaxCols);
+      at = (int)C.cols.size();
+      C.cols.push_back(c);
+    }
+    colmap[i] = at;
+  }
+  const int64_t rows = C.cols[0]->length;
+  for (dfr_column *c : C.cols) {
+    if (c->length != rows) return dfr_fail(error, DFR_INVALID, "columns differ in length");
+    if (c->ctx != ctx) return dfr_fail(error, DFR_INVALID, "a column belongs to another context");
+  }
+  C.rows = rows;
+  P.rows = rows;
+  P.n_cols = (int32_t)C.cols.size();
+  P.n_keys = q->n_keys;
+  P.vec_ok = 1;
+  for (int c = 0; c < P.n_cols; ++c) {
+    P.cols[c] = C.cols[c]->dev();
+    uintptr_t a = (uintptr_t)P.cols[c].values;
+    int need = (P.cols[c].dtype == DFR_INT64 || P.cols[c].dtype == DFR_FLOAT64) ? 16 : 8;
+    if (a % need) P.vec_ok = 0;
+  }
+
+  // keys: mixed-radix packing from the column statistics
+  unsigned __int128 domain = 1;
+  C.key_dtypes.clear();
+  for (int k = 0; k < q->n_keys; ++k) {
+    dfr_column *c = C.cols[k];
+    if (c->dtype == DFR_FLOAT64)
+      return dfr_fail(error, DFR_UNSUPPORTED, "Float64 group keys are not on the device path");
+    DevKey &K = P.keys[k];
+    bool has_values = c->stats.valid > 0;
+    K.minv = has_values ? c->stats.imin : 0;
+    unsigned __int128 span =
+        has_values ? (unsigned __int128)((uint64_t)c->stats.imax - (uint64_t)c->stats.imin) + 1 : 0;
+    K.nullable = c->nulls > 0 ? 1 : 0;
+    unsigned __int128 radix = span + (unsigned __int128)K.nullable;
+    if (radix == 0) radix = 1;
+    if (radix > ((unsigned __int128)1 << 62))
+      return dfr_fail(error, DFR_UNSUPPORTED, "key %d spans more than 2^62 values", k);
+    K.radix = (uint64_t)radix;
+    K.stride = (uint64_t)domain;
+    domain *= radix;
+    if (domain > ((unsigned __int128)1 << 62))
+      return dfr_fail(error, DFR_UNSUPPORTED, "the combined key domain exceeds 2^62");
+    C.key_dtypes.push_back(c->dtype);
+  }
+  P.domain = (uint64_t)domain;
+
+  // predicate leaves, grouped by column; leaf i keeps truth bit i
+  const dfr_predicate &pr = q->predicate;
+  P.n_leaves = pr.n_leaves;
+  for (int i = 0; i < 8; ++i) P.truth[i] = pr.truth[i];
+  {
+    std::vector<std::pair<int, int>> order;
+    for (int i = 0; i < pr.n_leaves; ++i) {
+      const dfr_leaf &L = pr.leaves[i];
+      if (L.column < 0 || L.column >= q->n_columns)
+        return dfr_fail(error, DFR_INVALID, "leaf %d names column %d", i, L.column);
+      if (L.op < DFR_EQ || L.op > DFR_IS_NOT_NULL)
+        return dfr_fail(error, DFR_UNSUPPORTED, "leaf %d: unknown comparison %d", i, L.op);
+      int pc = colmap[L.column];
+      if ((L.flags & DFR_LEAF_MOD) && P.cols[pc].dtype == DFR_FLOAT64)
+        return dfr_fail(error, DFR_UNSUPPORTED, "leaf %d: Float64 modulo is not on the device path", i);
+      order.push_back({pc, i});
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
+    int at = 0;
+    for (int c = 0; c <= kMaxCols; ++c) {
+      P.leaf_begin[c] = (int8_t)at;
+      while (at < (int)order.size() && order[at].first == c) {
+        const dfr_leaf &L = pr.leaves[order[at].second];
+        DevLeaf &D = P.leaves[at];
+        D.op = L.op;
+        D.flags = L.flags;
+        D.bit = order[at].second;
+        D.lit = L.literal;
+        D.mod = L.modulus;
+        P.leaf_cols |= 1 << c;
+        ++at;
+      }
+    }
+  }
+
+  // aggregates -> per-column blocks
+  uint32_t mask[kMaxCols] = {0};
+  int int_as_float[kMaxCols] = {0};
+  P.n_aggs = q->n_aggs;
+  C.agg_dtypes.clear();
+  for (int a = 0; a < q->n_aggs; ++a) {
+    const dfr_agg &G = q->aggs[a];
+    DevAgg &D = P.aggs[a];
+    D.op = G.op;
+    D.ddof = G.ddof;
+    D.min_count = G.min_count;
+    if (G.op == DFR_AGG_LEN) {
+      D.col = 0;
+      D.out_dtype = T_INT64;
+      C.agg_dtypes.push_back(DFR_INT64);
+      continue;
+    }
+    if (G.op < DFR_AGG_LEN || G.op > DFR_AGG_STD)
+      return dfr_fail(error, DFR_UNSUPPORTED, "aggregate %d: unknown op %d", a, G.op);
+    if (G.column < 0 || G.column >= q->n_columns)
+      return dfr_fail(error, DFR_INVALID, "aggregate %d names column %d", a, G.column);
+    int c = colmap[G.column];
+    D.col = c;
+    dfr_column *col = C.cols[c];
+    const bool nulls = col->nulls > 0;
+    const bool fl = col->dtype == DFR_FLOAT64;
+    const uint32_t fstats = col->stats.flags;
+    const bool nonfinite = (fstats & 7u) != 0;
+    if (col->dtype == DFR_DICT && G.op != DFR_AGG_COUNT)
+      return dfr_fail(error, DFR_UNSUPPORTED, "aggregate %d: only count() of a dictionary column", a);
+    if (nulls && G.op != DFR_AGG_LEN) mask[c] |= M_VCOUNT;
+    int out = T_FLOAT64;
+    // an integer total fits 64 bits when |value| * rows < 2^63
+    unsigned __int128 maxabs = 0;
+    if (!fl && col->stats.valid > 0) {
+      int64_t lo = col->stats.imin, hi = col->stats.imax;
+      unsigned __int128 alo = lo < 0 ? (unsigned __int128)(-(__int128)lo) : (unsigned __int128)lo;
+      unsigned __int128 ahi = hi < 0 ? (unsigned __int128)(-(__int128)hi) : (unsigned __int128)hi;
+      maxabs = std::max(alo, ahi);
+    }
+    const bool narrow = maxabs * (unsigned __int128)(rows + 1) < ((unsigned __int128)1 << 63);
+    switch (G.op) {
+      case DFR_AGG_COUNT:
+        out = T_INT64;
+        break;
+      case DFR_AGG_SUM:
+        if (fl) {
+          mask[c] |= M_FSUM | (nonfinite ? M_FLAGS : 0);
+          out = T_FLOAT64;
+        } else {
+          mask[c] |= narrow ? M_ISUM64 : M_ISUM128;
+          out = col->dtype == DFR_INT32 ? T_INT32 : T_INT64;
+        }
+        break;
+      case DFR_AGG_MIN:
+      case DFR_AGG_MAX: {
+        const bool mn = G.op == DFR_AGG_MIN;
+        if (fl) {
+          mask[c] |= (mn ? M_FMIN : M_FMAX) | ((fstats & 1u) ? M_FLAGS : 0) |
+                     ((fstats & 8u) ? M_ZFIRST : 0);
+          out = T_FLOAT64;
+        } else {
+          mask[c] |= mn ? M_IMIN : M_IMAX;
+          out = col->dtype == DFR_INT32 ? T_INT32 : T_INT64;
+        }
+        break;
+      }
+      case DFR_AGG_MEAN:
+        if (fl)
+          mask[c] |= M_FSUM | (nonfinite ? M_FLAGS : 0);
+        else
+          mask[c] |= narrow ? M_ISUM64 : M_ISUM128;
+        out = T_FLOAT64;
+        break;
+      default:  // VAR, STD
+        if (G.ddof < 0) return dfr_fail(error, DFR_INVALID, "aggregate %d: negative ddof", a);
+        mask[c] |= M_FSUM | M_FSUMSQ | (nonfinite ? M_FLAGS : 0);
+        if (!fl) int_as_float[c] = 1;
+        out = T_FLOAT64;
+        break;
+    }
+    D.out_dtype = out;
+    C.agg_dtypes.push_back(out);
+  }
+
+  // slot layout
+  int w = 1;  // ROWS
+  P.has_first = q->maintain_order ? 1 : 0;
+  if (P.has_first) w = 2;
+  int lw = w;
+  int l2g[kMaxWords];
+  for (int i = 0; i < w; ++i) l2g[i] = i;
+  uint64_t ident[kMaxWords];
+  ident[0] = 0;
+  if (P.has_first) ident[1] = kEmpty;
+  for (int c = 0; c < P.n_cols; ++c) {
+    DevColAgg &A = P.cagg[c];
+    memset(&A, 0, sizeof A);
+    A.o_count = A.o_isum = A.o_fsum = A.o_sq = A.o_min = A.o_max = A.o_flags = A.o_zfirst = -1;
+    A.mask = mask[c];
+    if (!mask[c]) continue;
+    if (w + 14 > kMaxWords) return dfr_fail(error, DFR_UNSUPPORTED, "too many aggregate words");
+    A.gbase = (int16_t)w;
+    A.lbase = (int16_t)lw;
+    int o = 0;
+    auto put = [&](int8_t &field, int n, uint64_t id, bool lds) {
+      field = (int8_t)o;
+      for (int i = 0; i < n; ++i) {
+        ident[w] = id;
+        if (lds) l2g[lw++] = w;
+        ++w;
+        ++o;
+      }
+    };
+    if (mask[c] & M_VCOUNT) put(A.o_count, 1, 0, true);
+    if (mask[c] & M_ISUM64) put(A.o_isum, 1, 0, true);
+    if (mask[c] & M_ISUM128) put(A.o_isum, 2, 0, true);
+    if (mask[c] & M_FSUM) put(A.o_fsum, 2, 0, true);
+    if (mask[c] & M_FSUMSQ) put(A.o_sq, 2, 0, true);
+    if (mask[c] & M_IMIN) put(A.o_min, 1, (uint64_t)INT64_MAX, true);
+    if (mask[c] & M_IMAX) put(A.o_max, 1, (uint64_t)INT64_MIN, true);
+    if (mask[c] & M_FMIN) put(A.o_min, 1, kEmpty, true);
+    if (mask[c] & M_FMAX) put(A.o_max, 1, 0, true);
+    if (mask[c] & M_FLAGS) put(A.o_flags, 1, 0, false);
+    if (mask[c] & M_ZFIRST) put(A.o_zfirst, 1, kEmpty, false);
+    // fixed-point scales: N values below 2^B each sum below 2^126
+    if (mask[c] & (M_FSUM | M_FSUMSQ)) {
+      dfr_column *col = C.cols[c];
+      int L = std::max(1, bitlen_u64((uint64_t)rows));
+      int B;
+      if (col->dtype == DFR_FLOAT64) {
+        B = (col->stats.flags & 32u) ? (int)col->stats.emax - 1023 + 1 : 0;
+      } else {
+        int64_t lo = col->stats.imin, hi = col->stats.imax;
+        uint64_t a = std::max(lo < 0 ? (uint64_t)(-(lo + 1)) + 1 : (uint64_t)lo,
+                              hi < 0 ? (uint64_t)(-(hi + 1)) + 1 : (uint64_t)hi);
+        B = col->stats.valid ? bitlen_u64(a) + 1 : 0;
+      }
+      A.fshift = 126 - B - L;
+      A.fshift2 = 126 - 2 * B - L;
+      A.int_as_float = int_as_float[c];
+    }
+  }
+  P.words = w;
+  P.lds_words = lw;
+  for (int i = 0; i < w; ++i) P.ident[i] = ident[i];
+  for (int i = 0; i < lw; ++i) P.lds_to_g[i] = (int8_t)l2g[i];
+  C.maintain_order = q->maintain_order != 0;
+  return DFR_OK;
+}
+
+// Table mode and launch shape from the tuning and the plan.
+void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots) {
+  DevPlan &P = C.plan;
+  const Tuning &T = ctx->tune;
+  const int64_t rows = C.rows;
+  const uint64_t D = P.domain;
+  const size_t slot_bytes = (size_t)P.words * 8;
+  const bool direct_fits = D <= (uint64_t)T.direct_max_slots &&
+                           (int64_t)(D * slot_bytes) <= std::max<int64_t>(ctx->pool.budget / 4, 1 << 20);
+  if (direct_fits) {
+    P.mode = MODE_DIRECT;
+    P.cap = std::max<uint64_t>(D, 1);
+    P.cap_log2 = 1;
+  } else {
+    P.mode = MODE_HASH;
+    uint64_t want = hash_slots > 0 ? (uint64_t)hash_slots
+                                   : next_pow2(std::max<uint64_t>(64, 2 * std::min<uint64_t>(D, (uint64_t)rows)));
+    P.cap = next_pow2(want);
+    P.cap_log2 = log2_exact(P.cap);
+  }
+  P.max_probe = (int32_t)std::min<uint64_t>((uint64_t)T.hash_probe, P.cap);
+  // LDS table
+  const int lds_bytes = std::min(T.lds_bytes, ctx->lds_per_block);
+  const int LW = P.lds_words;
+  int mode = T.lds_mode;
+  const uint64_t direct_entries = (uint64_t)lds_bytes / (8ull * LW);
+  if (mode < 0) mode = (D <= direct_entries) ? LDS_DIRECT : LDS_HASHED;
+  if (mode == LDS_DIRECT && D > direct_entries) mode = LDS_HASHED;
+  if (mode == LDS_DIRECT) {
+    P.lds_mode = LDS_DIRECT;
+    P.lds_entries = (int32_t)std::max<uint64_t>(D, 1);
+    P.lds_log2 = 1;
+  } else if (mode == LDS_HASHED) {
+    uint64_t e = 1;
+    while (e * 2 * 8ull * (LW + 1) <= (uint64_t)lds_bytes && e < 8192) e *= 2;
+    if (e < 64) {
+      P.lds_mode = LDS_NONE;
+      P.lds_entries = 0;
+    } else {
+      P.lds_mode = LDS_HASHED;
+      P.lds_entries = (int32_t)e;
+      P.lds_log2 = log2_exact(e);
+    }
+  } else {
+    P.lds_mode = LDS_NONE;
+    P.lds_entries = 0;
+  }
+  P.lds_probe = T.lds_probe;
+  // launch shape: persistent workgroups over contiguous row chunks
+  C.block = std::max(64, std::min(1024, T.block));
+  int64_t step = 2 * (int64_t)C.block;
+  int64_t blocks = (int64_t)ctx->cus * std::max(1, T.wg_per_cu);
+  int64_t chunk = (rows + blocks - 1) / std::max<int64_t>(blocks, 1);
+  chunk = std::max<int64_t>(step, (chunk + step - 1) / step * step);
+  C.chunk = chunk;
+  C.grid = (int)std::max<int64_t>(1, (rows + chunk - 1) / chunk);
+  C.lds_bytes = P.lds_mode == LDS_NONE
+                    ? 0
+                    : (size_t)P.lds_entries * 8 * (LW + (P.lds_mode == LDS_HASHED ? 1 : 0));
+}
+
+// ------------------------------------------------------------ launch
+
+namespace {
+
+struct Scratch {
+  dfr_context *ctx;
+  std::vector<void *> held;
+  explicit Scratch(dfr_context *c) : ctx(c) {}
+  ~Scratch() {
+    for (void *p : held) ctx->pool.release(p);
+  }
+  template <class T>
+  int get(size_t count, T **out, char **error) {
+    void *p = nullptr;
+    int st = ctx->pool.alloc(std::max<size_t>(count * sizeof(T), 8), &p, error);
+    if (st) return st;
+    held.push_back(p);
+    *out = (T *)p;
+    return DFR_OK;
+  }
+};
+
+int fill(dfr_context *ctx, uint64_t *p, uint64_t n, uint64_t value) {
+  if (n == 0) return DFR_OK;
+  int blocks = (int)std::min<uint64_t>((n + kBlock - 1) / kBlock, (uint64_t)ctx->cus * 16);
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(kBlock), 0, ctx->stream, p, n, value);
+  ctx->obs.launches++;
+  return DFR_OK;
+}
+
+// Exclusive prefix of popcounts of `n` bitmap words; the total lands at
+// sums[nb] where nb = number of scan blocks.
+int scan_bits(dfr_context *ctx, Scratch &S, const uint64_t *bits, uint64_t n, uint64_t *prefix,
+              uint64_t **total, char **error) {
+  uint64_t nb = std::max<uint64_t>(1, (n + kScanChunk - 1) / kScanChunk);
+  uint64_t *sums;
+  int st = S.get(nb + 1, &sums, error);
+  if (st) return st;
+  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, bits, n, sums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, ctx->stream, sums, nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, bits, n, sums,
+                     prefix);
+  ctx->obs.launches += 3;
+  *total = sums + nb;
+  return DFR_OK;
+}
+
+int launch_aggregate(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
+  const DevPlan &P = C.plan;
+  dim3 grid(C.grid), block(C.block);
+  size_t lds = C.lds_bytes;
+  switch (P.lds_mode) {
+    case LDS_DIRECT:
+      if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_aggregate<LDS_DIRECT>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(k_aggregate<LDS_DIRECT>, grid, block, lds, ctx->stream, dplan, (int64_t)0,
+                         C.rows, C.chunk);
+      break;
+    case LDS_HASHED:
+      if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_aggregate<LDS_HASHED>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(k_aggregate<LDS_HASHED>, grid, block, lds, ctx->stream, dplan, (int64_t)0,
+                         C.rows, C.chunk);
+      break;
+    default:
+      hipLaunchKernelGGL(k_aggregate<LDS_NONE>, grid, block, 0, ctx->stream, dplan, (int64_t)0, C.rows,
+                         C.chunk);
+  }
+  ctx->obs.launches++;
+  HIP_TRY(hipGetLastError());
+  return DFR_OK;
+}
+
+}  // namespace
+
+int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
+  DevPlan &P = C.plan;
+  int64_t hash_slots = ctx->tune.hash_slots;
+  for (int attempt = 0;; ++attempt) {
+    choose_layout(ctx, C, hash_slots);
+    Scratch S(ctx);
+    int st;
+    DevPlan *dplan;
+    uint32_t *fault;
+    if ((st = S.get(1, &dplan, error)) || (st = S.get(4, &fault, error)) ||
+        (st = S.get(P.cap * (uint64_t)P.words, &P.state, error)))
+      return st;
+    if (P.mode == MODE_HASH && (st = S.get(P.cap, &P.keys_tab, error))) return st;
+    P.keys_tab = P.mode == MODE_HASH ? P.keys_tab : nullptr;
+    P.fault = fault;
+    const uint64_t occ_words = (P.cap + 63) / 64;
+    uint64_t *occ, *occ_prefix, *order;
+    const uint64_t max_groups = std::min<uint64_t>(P.cap, (uint64_t)std::max<int64_t>(C.rows, 1));
+    if ((st = S.get(occ_words, &occ, error)) || (st = S.get(occ_words, &occ_prefix, error)) ||
+        (st = S.get(max_groups + 1, &order, error)))
+      return st;
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    memcpy(ctx->staging, &P, sizeof P);
+    HIP_TRY(hipMemcpyAsync(dplan, ctx->staging, sizeof P, hipMemcpyHostToDevice, ctx->stream));
+    fill(ctx, (uint64_t *)fault, 2, 0);
+    {
+      int blocks = (int)std::min<uint64_t>((P.cap + kBlock - 1) / kBlock, (uint64_t)ctx->cus * 8);
+      hipLaunchKernelGGL(k_init_state, dim3(std::max(blocks, 1)), dim3(kBlock), 0, ctx->stream, dplan);
+      ctx->obs.launches++;
+    }
+    HIP_TRY(hipEventRecord(ctx->evp[0], ctx->stream));
+    if (C.rows > 0 && (st = launch_aggregate(ctx, C, dplan, error))) return st;
+    HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
+    {
+      uint64_t threads = occ_words * 64;
+      hipLaunchKernelGGL(k_occupancy, dim3((unsigned)((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                         ctx->stream, dplan, occ, occ_words);
+      ctx->obs.launches++;
+    }
+    uint64_t *total;
+    if ((st = scan_bits(ctx, S, occ, occ_words, occ_prefix, &total, error))) return st;
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)((P.cap + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, dplan, occ, occ_prefix, order);
+    ctx->obs.launches++;
+    uint64_t *hs = (uint64_t *)ctx->staging;
+    HIP_TRY(hipMemcpyAsync(hs, total, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(hs + 1, fault, 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->obs.waits++;
+    ctx->obs.completed = ctx->obs.launches;
+    const uint64_t G = hs[0];
+    const uint32_t *hf = (const uint32_t *)(hs + 1);
+    if (hf[0]) {
+      if (P.mode != MODE_HASH || attempt >= 3)
+        return dfr_fail(error, DFR_CAPACITY, "group table capacity exceeded");
+      hash_slots = (int64_t)P.cap * 4;
+      continue;  // retry with a larger table
+    }
+    P.groups = total;
+    P.order = order;
+    if (C.maintain_order && G > 0) {
+      const uint64_t rwords = ((uint64_t)C.rows + 63) / 64;
+      uint64_t *rowbits, *rowprefix, *order2;
+      if ((st = S.get(rwords, &rowbits, error)) || (st = S.get(rwords, &rowprefix, error)) ||
+          (st = S.get(G + 1, &order2, error)))
+        return st;
+      fill(ctx, rowbits, rwords, 0);
+      unsigned gb = (unsigned)((G + kBlock - 1) / kBlock);
+      hipLaunchKernelGGL(k_mark_first, dim3(gb), dim3(kBlock), 0, ctx->stream, dplan, order, G, rowbits);
+      uint64_t *rtotal;
+      if ((st = scan_bits(ctx, S, rowbits, rwords, rowprefix, &rtotal, error))) return st;
+      hipLaunchKernelGGL(k_rank_first, dim3(gb), dim3(kBlock), 0, ctx->stream, dplan, order, G, rowbits,
+                         rowprefix, order2);
+      ctx->obs.launches += 2;
+      P.order = order2;
+    }
+    HIP_TRY(hipEventRecord(ctx->evp[2], ctx->stream));
+    // outputs: one pool block
+    dfr_result *r = new dfr_result();
+    r->ctx = ctx;
+    r->groups = (int64_t)G;
+    r->key_dtypes = C.key_dtypes;
+    r->agg_dtypes = C.agg_dtypes;
+    const uint64_t vwords = (G + 63) / 64 + 1;
+    size_t bytes = 0;
+    auto col_bytes = [&](int dtype) {
+      size_t b = (size_t)G * ((dtype == DFR_INT32 || dtype == DFR_DICT) ? 4 : 8);
+      return (b + 255) / 256 * 256 + vwords * 8;
+    };
+    for (int32_t d : C.key_dtypes) bytes += col_bytes(d);
+    for (int32_t d : C.agg_dtypes) bytes += col_bytes(d);
+    if (C.maintain_order) bytes += (size_t)G * 8 + 256;
+    if ((st = ctx->pool.alloc(bytes, &r->buffer, error))) {
+      delete r;
+      return st;
+    }
+    r->bytes = bytes;
+    char *cur = (char *)r->buffer;
+    auto take = [&](int dtype, void **vals, uint64_t **valid) {
+      size_t b = (size_t)G * ((dtype == DFR_INT32 || dtype == DFR_DICT) ? 4 : 8);
+      *vals = cur;
+      cur += (b + 255) / 256 * 256;
+      *valid = (uint64_t *)cur;
+      cur += vwords * 8;
+    };
+    for (size_t k = 0; k < C.key_dtypes.size(); ++k) {
+      void *v;
+      uint64_t *m;
+      take(C.key_dtypes[k], &v, &m);
+      r->key_values.push_back(v);
+      r->key_valid.push_back(m);
+      P.out_keys[k] = v;
+      P.out_key_valid[k] = m;
+    }
+    for (size_t a = 0; a < C.agg_dtypes.size(); ++a) {
+      void *v;
+      uint64_t *m;
+      take(C.agg_dtypes[a], &v, &m);
+      r->agg_values.push_back(v);
+      r->agg_valid.push_back(m);
+      P.aggs[a].out_values = v;
+      P.aggs[a].out_valid = m;
+    }
+    if (C.maintain_order) {
+      r->first_rows = (uint64_t *)cur;
+      P.out_first = r->first_rows;
+    } else {
+      P.out_first = nullptr;
+    }
+    memcpy(ctx->staging, &P, sizeof P);
+    if (G > 0) {
+      HIP_TRY(hipMemcpyAsync(dplan, ctx->staging, sizeof P, hipMemcpyHostToDevice, ctx->stream));
+      hipLaunchKernelGGL(k_finalize, dim3((unsigned)((G + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                         ctx->stream, dplan);
+      ctx->obs.launches++;
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    uint32_t *hf2 = (uint32_t *)(ctx->staging + sizeof(DevPlan) + 64);
+    HIP_TRY(hipMemcpyAsync(hf2, fault, 16, hipMemcpyDeviceToHost, ctx->stream));
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    ctx->obs.waits++;
+    if (e != hipSuccess) {
+      dfr_result_release(r);
+      return dfr_fail(error, DFR_DEVICE, "group-by kernels: %s", hipGetErrorString(e));
+    }
+    ctx->obs.completed = ctx->obs.launches;
+    ctx->obs.queries++;
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    r->device_ms = ms;
+    hipEvent_t marks[5] = {ctx->ev0, ctx->evp[0], ctx->evp[1], ctx->evp[2], ctx->ev1};
+    for (int ph = 0; ph < 4; ++ph) {
+      float t = 0;
+      hipEventElapsedTime(&t, marks[ph], marks[ph + 1]);
+      r->phase_ms[ph] = t;
+    }
+    if (hf2[1]) {
+      dfr_result_release(r);
+      return dfr_fail(error, DFR_DATA_FAULT, "Int64 expression sum overflow");
+    }
+    if (hf2[2]) {
+      dfr_result_release(r);
+      return dfr_fail(error, DFR_DATA_FAULT, "int32 expression sum overflow");
+    }
+    *out = r;
+    return DFR_OK;
+  }
+}
+
+extern "C" int dfr_groupby(dfr_context *ctx, const dfr_query *query, dfr_result **out, char **error) {
+  if (!ctx || !out) return dfr_fail(error, DFR_INVALID, "null argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  Compiled C;
+  int st = compile_query(ctx, query, C, error);
+  if (st) return st;
+  return run_query(ctx, C, out, error);
+}

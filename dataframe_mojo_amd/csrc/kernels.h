@@ -1,0 +1,710 @@
+// kernels.h — CDNA4 (gfx950) kernels of the filtered group-by path.
+//
+//   k_col_stats      one pass over a column at hand-off: min/max, exponent
+//                    range, NaN/inf/signed-zero flags, null count
+//   k_init_state     fills the HBM group table with each word's identity
+//   k_aggregate      the fused scan: loads column values and validity words,
+//                    runs the predicate program, packs the key, and folds
+//                    each kept row into an LDS-private group table (direct
+//                    or first-come hashed) or, when that misses, into the
+//                    HBM table (direct-addressed or open-addressed with
+//                    64-bit CAS); LDS entries are merged into HBM at the end
+//   k_occupancy      slot occupancy -> bitmap (one ballot per 64 slots)
+//   k_scan_*         exclusive prefix of bitmap popcounts
+//   k_compact        slot order -> dense group order
+//   k_mark_first,
+//   k_rank_first     maintain_order: group order by first kept row, by
+//                    ranking first rows in a row bitmap (no sort)
+//   k_finalize       keys and aggregate results, validity as packed bits
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_util.h"
+#include "plan.h"
+#include "runtime.h"
+
+namespace dfr {
+
+constexpr int kBlock = 256;
+
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ uint64_t hash_index(uint64_t key, int log2) {
+  return (key * 0x9E3779B97F4A7C15ull) >> (64 - log2);
+}
+
+__device__ __forceinline__ uint64_t atomic_load_u64(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Two rows' values of one column (rows r and r+1; `two` is false for the
+// last odd row of a range).
+__device__ __forceinline__ void load_pair(const DevCol &C, int64_t r, bool two,
+                                          bool vec, uint64_t &a, uint64_t &b) {
+  if (C.dtype == T_INT64 || C.dtype == T_FLOAT64) {
+    const uint64_t *p = (const uint64_t *)C.values + r;
+    if (vec && two) {
+      u64x2 q = __builtin_nontemporal_load((const u64x2 *)p);
+      a = q.x;
+      b = q.y;
+    } else {
+      a = p[0];
+      b = two ? p[1] : 0;
+    }
+  } else {
+    const uint32_t *p = (const uint32_t *)C.values + r;
+    uint32_t x, y;
+    if (vec && two) {
+      u32x2 q = __builtin_nontemporal_load((const u32x2 *)p);
+      x = q.x;
+      y = q.y;
+    } else {
+      x = p[0];
+      y = two ? p[1] : 0;
+    }
+    if (C.dtype == T_INT32) {
+      a = (uint64_t)(int64_t)(int32_t)x;
+      b = (uint64_t)(int64_t)(int32_t)y;
+    } else {
+      a = x;
+      b = y;
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t load_valid_pair(const DevCol &C, int64_t r,
+                                                    bool two) {
+  if (!C.valid) return 3u;
+  int64_t bit = C.vbit + r;
+  int s = (int)(bit & 63);
+  uint64_t w = C.valid[bit >> 6];
+  uint32_t v0 = (uint32_t)(w >> s) & 1u;
+  uint32_t v1 = 0;
+  if (two) v1 = (s < 63) ? ((uint32_t)(w >> (s + 1)) & 1u) : ((uint32_t)C.valid[(bit >> 6) + 1] & 1u);
+  return v0 | (v1 << 1);
+}
+
+__device__ __forceinline__ bool leaf_true(const DevLeaf &L, bool is_float,
+                                          uint64_t v, bool valid) {
+  if (L.op == 6) return !valid;  // DFR_IS_NULL
+  if (L.op == 7) return valid;   // DFR_IS_NOT_NULL
+  if (!valid) return false;
+  bool t;
+  if (is_float) {
+    double x = bits_f64(v), y = bits_f64((uint64_t)L.lit);
+    switch (L.op) {
+      case 0: t = x == y; break;
+      case 1: t = x != y; break;
+      case 2: t = x < y; break;
+      case 3: t = x <= y; break;
+      case 4: t = x > y; break;
+      default: t = x >= y; break;
+    }
+  } else {
+    int64_t x = (int64_t)v;
+    if (L.flags & 1) {
+      int64_t m = L.mod;
+      if (m == 0) return false;  // modulo zero is null
+      int64_t q = (m == -1) ? 0 : x % m;
+      if (q != 0 && ((q < 0) != (m < 0))) q += m;
+      x = q;
+    }
+    int64_t y = L.lit;
+    switch (L.op) {
+      case 0: t = x == y; break;
+      case 1: t = x != y; break;
+      case 2: t = x < y; break;
+      case 3: t = x <= y; break;
+      case 4: t = x > y; break;
+      default: t = x >= y; break;
+    }
+  }
+  return (L.flags & 2) ? !t : t;
+}
+
+__device__ __forceinline__ uint32_t truth_bit(const DevPlan &P, uint32_t m) {
+  return (P.truth[m >> 5] >> (m & 31)) & 1u;
+}
+
+// HBM slot of a packed key (inserting it in the hash table when new).
+__device__ __noinline__ uint64_t global_slot_hash(const DevPlan &P, uint64_t key) {
+  uint64_t mask = P.cap - 1;
+  uint64_t i = hash_index(key, P.cap_log2);
+  for (int p = 0; p < P.max_probe; ++p) {
+    uint64_t k = atomic_load_u64(&P.keys_tab[i]);
+    if (k == key) return i;
+    if (k == kEmpty) {
+      unsigned long long old =
+          atomicCAS((unsigned long long *)&P.keys_tab[i], (unsigned long long)kEmpty,
+                    (unsigned long long)key);
+      if (old == kEmpty || old == key) return i;
+    }
+    i = (i + 1) & mask;
+  }
+  atomicOr(&P.fault[0], 1u);
+  return kEmpty;
+}
+
+__device__ __forceinline__ uint64_t global_slot(const DevPlan &P, uint64_t key) {
+  if (P.mode == MODE_DIRECT) return key;
+  return global_slot_hash(P, key);
+}
+
+__device__ __forceinline__ void add128(uint64_t *p, uint64_t lo, uint64_t hi) {
+  unsigned long long old = atomicAdd((unsigned long long *)p, (unsigned long long)lo);
+  uint64_t carry = ((uint64_t)old + lo) < (uint64_t)old ? 1 : 0;
+  uint64_t h = hi + carry;
+  if (h) atomicAdd((unsigned long long *)(p + 1), (unsigned long long)h);
+}
+
+// The rare-event words (FLAGS, ZFIRST) live in HBM only.
+__device__ __noinline__ void rare_update(const DevPlan &P, uint64_t packed, int gword,
+                                         uint64_t value, bool is_min) {
+  uint64_t g = global_slot(P, packed);
+  if (g == kEmpty) return;
+  uint64_t *p = P.state + g * (uint64_t)P.words + gword;
+  if (is_min)
+    atomicMin((unsigned long long *)p, (unsigned long long)value);
+  else
+    atomicOr((unsigned long long *)p, (unsigned long long)value);
+}
+
+// Folds one kept row into a group's words. `LDS` selects the ds_* form.
+template <bool LDS>
+__device__ __forceinline__ void fold_row(const DevPlan &P, uint64_t *st, int64_t row,
+                                         const uint64_t *vals, uint32_t validbits,
+                                         uint64_t packed) {
+  atomicAdd((unsigned long long *)&st[0], 1ull);
+  if (P.has_first) {
+    if ((uint64_t)row < st[1]) atomicMin((unsigned long long *)&st[1], (unsigned long long)row);
+  }
+#pragma unroll
+  for (int c = 0; c < kMaxCols; ++c) {
+    if (c >= P.n_cols) continue;
+    const DevColAgg &A = P.cagg[c];
+    uint32_t m = A.mask;
+    if (!m || !((validbits >> c) & 1u)) continue;
+    uint64_t *b = st + (LDS ? A.lbase : A.gbase);
+    uint64_t v = vals[c];
+    if (m & M_VCOUNT) atomicAdd((unsigned long long *)&b[A.o_count], 1ull);
+    if (m & M_ISUM64) atomicAdd((unsigned long long *)&b[A.o_isum], (unsigned long long)v);
+    if (m & M_ISUM128) add128(&b[A.o_isum], v, ((int64_t)v < 0) ? ~0ull : 0ull);
+    if (m & M_IMIN) {
+      if ((int64_t)v < (int64_t)b[A.o_min]) atomicMin((long long *)&b[A.o_min], (long long)v);
+    }
+    if (m & M_IMAX) {
+      if ((int64_t)v > (int64_t)b[A.o_max]) atomicMax((long long *)&b[A.o_max], (long long)v);
+    }
+    if (m & (M_FSUM | M_FSUMSQ | M_FMIN | M_FMAX | M_FLAGS | M_ZFIRST)) {
+      uint64_t bits = A.int_as_float ? f64_bits((double)(int64_t)v) : v;
+      uint32_t ex = (uint32_t)(bits >> 52) & 0x7ffu;
+      bool nan = false;
+      if (ex == 0x7ffu) {
+        nan = (bits << 12) != 0;
+        uint64_t fl = nan ? FL_NAN : ((bits >> 63) ? FL_NINF : FL_PINF);
+        if (m & M_FLAGS) rare_update(P, packed, A.gbase + A.o_flags, fl, false);
+      } else {
+        if (m & M_FSUM) {
+          uint64_t lo, hi;
+          fix_of(bits, A.fshift, lo, hi);
+          add128(&b[A.o_fsum], lo, hi);
+        }
+        if (m & M_FSUMSQ) {
+          uint64_t lo, hi;
+          fixsq_of(bits, A.fshift2, lo, hi);
+          add128(&b[A.o_sq], lo, hi);
+        }
+        if ((m & M_ZFIRST) && (bits << 1) == 0)
+          rare_update(P, packed, A.gbase + A.o_zfirst, ((uint64_t)row << 1) | (bits >> 63), true);
+      }
+      if (!nan) {
+        uint64_t k = ord_key(bits);
+        if (m & M_FMIN) {
+          if (k < b[A.o_min]) atomicMin((unsigned long long *)&b[A.o_min], (unsigned long long)k);
+        }
+        if (m & M_FMAX) {
+          if (k > b[A.o_max]) atomicMax((unsigned long long *)&b[A.o_max], (unsigned long long)k);
+        }
+      }
+    }
+  }
+}
+
+template <int LDSM>
+__device__ __forceinline__ int lds_find(const DevPlan &P, uint64_t *lkeys, uint64_t key) {
+  if (LDSM == LDS_DIRECT) return (int)key;
+  const int E = P.lds_entries;
+  int h = (int)hash_index(key, P.lds_log2);
+  for (int p = 0; p < P.lds_probe; ++p) {
+    uint64_t k = lkeys[h];
+    if (k == key) return h;
+    if (k == kEmpty) {
+      unsigned long long old = atomicCAS((unsigned long long *)&lkeys[h],
+                                         (unsigned long long)kEmpty, (unsigned long long)key);
+      if (old == kEmpty || old == key) return h;
+    }
+    h = (h + 1) & (E - 1);
+  }
+  return -1;
+}
+
+template <int LDSM>
+__device__ __forceinline__ void fold_kept(const DevPlan &P, uint64_t *lkeys, uint64_t *lst,
+                                          int64_t row, const uint64_t *vals,
+                                          uint32_t validbits) {
+  uint64_t packed = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxKeys; ++k) {
+    if (k >= P.n_keys) continue;
+    const DevKey &K = P.keys[k];
+    uint64_t digit = ((validbits >> k) & 1u) ? (vals[k] - (uint64_t)K.minv) : (K.radix - 1);
+    packed += digit * K.stride;
+  }
+  if (LDSM != LDS_NONE) {
+    int e = lds_find<LDSM>(P, lkeys, packed);
+    if (e >= 0) {
+      fold_row<true>(P, lst + (int64_t)e * P.lds_words, row, vals, validbits, packed);
+      return;
+    }
+  }
+  uint64_t g = global_slot(P, packed);
+  if (g == kEmpty) return;
+  fold_row<false>(P, P.state + g * (uint64_t)P.words, row, vals, validbits, packed);
+}
+
+template <int LDSM>
+__device__ __forceinline__ void process_pair(const DevPlan &P, uint64_t *lkeys, uint64_t *lst,
+                                             int64_t r, int64_t end) {
+  if (r >= end) return;
+  const bool two = r + 1 < end;
+  uint64_t v0[kMaxCols], v1[kMaxCols];
+  uint32_t valid = 0;
+#pragma unroll
+  for (int c = 0; c < kMaxCols; ++c) {
+    if (c >= P.n_cols) continue;
+    load_pair(P.cols[c], r, two, P.vec_ok, v0[c], v1[c]);
+    valid |= load_valid_pair(P.cols[c], r, two) << (2 * c);
+  }
+  uint32_t va = 0, vb = 0;  // validity of row r / r+1, bit c
+#pragma unroll
+  for (int c = 0; c < kMaxCols; ++c) {
+    va |= ((valid >> (2 * c)) & 1u) << c;
+    vb |= ((valid >> (2 * c + 1)) & 1u) << c;
+  }
+  uint32_t keep = two ? 3u : 1u;
+  if (P.n_leaves) {
+    uint32_t m0 = 0, m1 = 0;
+#pragma unroll
+    for (int c = 0; c < kMaxCols; ++c) {
+      if (c >= P.n_cols || !((P.leaf_cols >> c) & 1)) continue;
+      const bool fl = P.cols[c].is_float;
+      for (int l = P.leaf_begin[c]; l < P.leaf_begin[c + 1]; ++l) {
+        const DevLeaf &L = P.leaves[l];
+        m0 |= (uint32_t)leaf_true(L, fl, v0[c], (va >> c) & 1u) << L.bit;
+        m1 |= (uint32_t)leaf_true(L, fl, v1[c], (vb >> c) & 1u) << L.bit;
+      }
+    }
+    keep &= truth_bit(P, m0) | (truth_bit(P, m1) << 1);
+  }
+  if (keep & 1u) fold_kept<LDSM>(P, lkeys, lst, r, v0, va);
+  if (keep & 2u) fold_kept<LDSM>(P, lkeys, lst, r + 1, v1, vb);
+}
+
+// Merges one LDS entry into its HBM slot.
+__device__ __forceinline__ void merge_entry(const DevPlan &P, const uint64_t *s, uint64_t key) {
+  uint64_t g = global_slot(P, key);
+  if (g == kEmpty) return;
+  uint64_t *d = P.state + g * (uint64_t)P.words;
+  atomicAdd((unsigned long long *)&d[0], (unsigned long long)s[0]);
+  if (P.has_first) atomicMin((unsigned long long *)&d[1], (unsigned long long)s[1]);
+  for (int c = 0; c < P.n_cols; ++c) {
+    const DevColAgg &A = P.cagg[c];
+    uint32_t m = A.mask;
+    if (!m) continue;
+    const uint64_t *a = s + A.lbase;
+    uint64_t *b = d + A.gbase;
+    if ((m & M_VCOUNT) && a[A.o_count]) atomicAdd((unsigned long long *)&b[A.o_count], (unsigned long long)a[A.o_count]);
+    if (m & M_ISUM64) atomicAdd((unsigned long long *)&b[A.o_isum], (unsigned long long)a[A.o_isum]);
+    if (m & M_ISUM128) add128(&b[A.o_isum], a[A.o_isum], a[A.o_isum + 1]);
+    if (m & M_FSUM) add128(&b[A.o_fsum], a[A.o_fsum], a[A.o_fsum + 1]);
+    if (m & M_FSUMSQ) add128(&b[A.o_sq], a[A.o_sq], a[A.o_sq + 1]);
+    if (m & M_IMIN) atomicMin((long long *)&b[A.o_min], (long long)a[A.o_min]);
+    if (m & M_IMAX) atomicMax((long long *)&b[A.o_max], (long long)a[A.o_max]);
+    if ((m & M_FMIN) && a[A.o_min] != kEmpty) atomicMin((unsigned long long *)&b[A.o_min], (unsigned long long)a[A.o_min]);
+    if ((m & M_FMAX) && a[A.o_max] != 0) atomicMax((unsigned long long *)&b[A.o_max], (unsigned long long)a[A.o_max]);
+  }
+}
+
+// ------------------------------------------------------------- the scan
+
+// Each workgroup folds rows [row_begin + blockIdx.x * chunk, +chunk). chunk
+// is a multiple of 2 * blockDim.x.
+template <int LDSM>
+__global__ void __launch_bounds__(1024) k_aggregate(const DevPlan *__restrict__ plan,
+                                                    int64_t row_begin, int64_t row_end,
+                                                    int64_t chunk) {
+  extern __shared__ uint64_t smem[];
+  const DevPlan &P = *plan;
+  uint64_t *lkeys = smem;
+  uint64_t *lst = smem + (LDSM == LDS_HASHED ? P.lds_entries : 0);
+  if (LDSM != LDS_NONE) {
+    const int E = P.lds_entries, LW = P.lds_words;
+    for (int i = threadIdx.x; i < E; i += blockDim.x) {
+      if (LDSM == LDS_HASHED) lkeys[i] = kEmpty;
+      for (int w = 0; w < LW; ++w) lst[i * LW + w] = P.ident[P.lds_to_g[w]];
+    }
+    __syncthreads();
+  }
+  const int64_t b0 = row_begin + (int64_t)blockIdx.x * chunk;
+  const int64_t b1 = min(b0 + chunk, row_end);
+  const int64_t step = 2 * (int64_t)blockDim.x;
+  for (int64_t base = b0; base < b1; base += step)
+    process_pair<LDSM>(P, lkeys, lst, base + 2 * (int64_t)threadIdx.x, b1);
+  if (LDSM != LDS_NONE) {
+    __syncthreads();
+    const int E = P.lds_entries, LW = P.lds_words;
+    for (int e = threadIdx.x; e < E; e += blockDim.x) {
+      const uint64_t *s = lst + (int64_t)e * LW;
+      if (s[0] == 0) continue;
+      merge_entry(P, s, LDSM == LDS_DIRECT ? (uint64_t)e : lkeys[e]);
+    }
+  }
+}
+
+// ------------------------------------------------------ table set-up
+
+__global__ void k_init_state(const DevPlan *__restrict__ plan) {
+  const DevPlan &P = *plan;
+  const uint32_t W = (uint32_t)P.words;
+  // each block covers kBlock slots, coalesced over their words
+  for (uint64_t s0 = (uint64_t)blockIdx.x * kBlock; s0 < P.cap; s0 += (uint64_t)gridDim.x * kBlock) {
+    uint64_t nslots = min((uint64_t)kBlock, P.cap - s0);
+    uint32_t total = (uint32_t)nslots * W;
+    uint64_t *base = P.state + s0 * W;
+    for (uint32_t j = threadIdx.x; j < total; j += kBlock) base[j] = P.ident[j % W];
+    if (P.mode == MODE_HASH && threadIdx.x < nslots) P.keys_tab[s0 + threadIdx.x] = kEmpty;
+  }
+}
+
+// -------------------------------------------------- compaction / order
+
+__global__ void k_occupancy(const DevPlan *__restrict__ plan, uint64_t *bits, uint64_t nwords) {
+  const DevPlan &P = *plan;
+  uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool occ = s < P.cap && P.state[s * (uint64_t)P.words] != 0;
+  uint64_t ball = __ballot(occ);
+  if ((threadIdx.x & 63) == 0 && (s >> 6) < nwords) bits[s >> 6] = ball;
+}
+
+constexpr int kScanItems = 8;  // words per thread
+constexpr int kScanChunk = kBlock * kScanItems;
+
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t x, uint64_t *total) {
+  __shared__ uint64_t sh[kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t inc = x;
+  for (int d = 1; d < 64; d <<= 1) {
+    uint64_t y = __shfl_up((unsigned long long)inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) sh[wave] = inc;
+  __syncthreads();
+  uint64_t off = 0, tot = 0;
+  for (int w = 0; w < kBlock / 64; ++w) {
+    if (w < wave) off += sh[w];
+    tot += sh[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - x;
+}
+
+__global__ void k_scan_blocks(const uint64_t *bits, uint64_t n, uint64_t *block_sums) {
+  uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
+  uint64_t c = 0;
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) c += __popcll(bits[base + i]);
+  uint64_t tot;
+  block_exclusive_scan(c, &tot);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+// One workgroup: exclusive scan of block sums in place; total at [nb].
+__global__ void k_scan_top(uint64_t *block_sums, uint64_t nb) {
+  uint64_t carry = 0;
+  for (uint64_t b0 = 0; b0 < nb; b0 += kBlock) {
+    uint64_t i = b0 + threadIdx.x;
+    uint64_t x = i < nb ? block_sums[i] : 0;
+    uint64_t tot;
+    uint64_t ex = block_exclusive_scan(x, &tot);
+    if (i < nb) block_sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) block_sums[nb] = carry;
+}
+
+__global__ void k_scan_apply(const uint64_t *bits, uint64_t n, const uint64_t *block_sums,
+                             uint64_t *prefix) {
+  uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
+  uint64_t c = 0;
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) c += __popcll(bits[base + i]);
+  uint64_t tot;
+  uint64_t run = block_sums[blockIdx.x] + block_exclusive_scan(c, &tot);
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) {
+      prefix[base + i] = run;
+      run += __popcll(bits[base + i]);
+    }
+}
+
+__device__ __forceinline__ uint64_t rank_of(const uint64_t *bits, const uint64_t *prefix, uint64_t i) {
+  uint64_t w = bits[i >> 6];
+  uint64_t below = (i & 63) ? (w & ((1ull << (i & 63)) - 1)) : 0;
+  return prefix[i >> 6] + __popcll(below);
+}
+
+__global__ void k_compact(const DevPlan *__restrict__ plan, const uint64_t *bits,
+                          const uint64_t *prefix, uint64_t *order) {
+  const DevPlan &P = *plan;
+  uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.cap) return;
+  if ((bits[s >> 6] >> (s & 63)) & 1) order[rank_of(bits, prefix, s)] = s;
+}
+
+__global__ void k_mark_first(const DevPlan *__restrict__ plan, const uint64_t *order,
+                             uint64_t groups, uint64_t *rowbits) {
+  const DevPlan &P = *plan;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= groups) return;
+  uint64_t r = P.state[order[i] * (uint64_t)P.words + 1];
+  atomicOr((unsigned long long *)&rowbits[r >> 6], (unsigned long long)(1ull << (r & 63)));
+}
+
+__global__ void k_rank_first(const DevPlan *__restrict__ plan, const uint64_t *order_in,
+                             uint64_t groups, const uint64_t *rowbits, const uint64_t *rowprefix,
+                             uint64_t *order_out) {
+  const DevPlan &P = *plan;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= groups) return;
+  uint64_t s = order_in[i];
+  uint64_t r = P.state[s * (uint64_t)P.words + 1];
+  order_out[rank_of(rowbits, rowprefix, r)] = s;
+}
+
+// ------------------------------------------------------------ finalize
+
+struct AggOut {
+  uint64_t bits;  // int64 value, or double bits
+  bool valid;
+};
+
+__device__ __forceinline__ double finish_fsum(const DevColAgg &A, const uint64_t *b, uint64_t flags) {
+  if ((flags & FL_NAN) || ((flags & FL_PINF) && (flags & FL_NINF))) return __builtin_nan("");
+  if (flags & FL_PINF) return __builtin_inf();
+  if (flags & FL_NINF) return -__builtin_inf();
+  return i128_to_f64(b[A.o_fsum], b[A.o_fsum + 1], A.fshift);
+}
+
+__device__ __forceinline__ double finish_var(const DevColAgg &A, const uint64_t *b, uint64_t n,
+                                             int ddof, uint64_t flags) {
+  if (flags & (FL_NAN | FL_PINF | FL_NINF)) return __builtin_nan("");
+  // n * sumsq * 2^(2S - S2) - sum^2, exactly, in units of 2^-2S
+  uint64_t qlo = b[A.o_sq], qhi = b[A.o_sq + 1];
+  uint64_t tlo = b[A.o_fsum], thi = b[A.o_fsum + 1];
+  if ((int64_t)thi < 0) {
+    tlo = ~tlo + 1;
+    thi = ~thi + (tlo == 0 ? 1 : 0);
+  }
+  U256 nq = mul128(qlo, qhi, n, 0);
+  U256 a = shl256(nq, 2 * A.fshift - A.fshift2);
+  U256 t2 = mul128(tlo, thi, tlo, thi);
+  U256 r;
+  sub256(a, t2, r);
+  double num = u256_to_f64(r, 2 * A.fshift);
+  double den = (double)n * (double)(n - (uint64_t)ddof);
+  return num / den;
+}
+
+__device__ AggOut finish_agg(const DevPlan &P, const DevAgg &G, const uint64_t *slot) {
+  AggOut o = {0, true};
+  uint64_t rows = slot[0];
+  if (G.op == 0) {  // LEN
+    o.bits = rows;
+    return o;
+  }
+  const DevColAgg &A = P.cagg[G.col];
+  const uint64_t *b = slot + A.gbase;
+  uint64_t n = (A.mask & M_VCOUNT) ? b[A.o_count] : rows;
+  uint64_t flags = (A.mask & M_FLAGS) ? b[A.o_flags] : 0;
+  const bool is_float = P.cols[G.col].is_float;
+  switch (G.op) {
+    case 1:  // COUNT
+      o.bits = n;
+      return o;
+    case 2: {  // SUM
+      o.valid = (int64_t)n >= G.min_count;
+      if (is_float) {
+        o.bits = o.valid ? f64_bits(finish_fsum(A, b, flags)) : 0;
+      } else if (A.mask & M_ISUM64) {
+        int64_t t = (int64_t)b[A.o_isum];
+        if (!o.valid) t = 0;
+        if (G.out_dtype == T_INT32 && (t > 2147483647ll || t < -2147483648ll)) atomicOr(&P.fault[2], 1u);
+        o.bits = (uint64_t)t;
+      } else {
+        uint64_t lo = b[A.o_isum], hi = b[A.o_isum + 1];
+        bool fits = (hi == 0 && (int64_t)lo >= 0) || (hi == ~0ull && (int64_t)lo < 0);
+        if (!o.valid) lo = 0;
+        else if (!fits) atomicOr(&P.fault[G.out_dtype == T_INT32 ? 2 : 1], 1u);
+        int64_t t = (int64_t)lo;
+        if (o.valid && G.out_dtype == T_INT32 && (t > 2147483647ll || t < -2147483648ll))
+          atomicOr(&P.fault[2], 1u);
+        o.bits = (uint64_t)t;
+      }
+      return o;
+    }
+    case 3:
+    case 4: {  // MIN / MAX
+      const bool is_min = G.op == 3;
+      if (is_float) {
+        uint64_t k = b[is_min ? A.o_min : A.o_max];
+        bool seen = is_min ? (k != kEmpty) : (k != 0);
+        if (!seen) {
+          o.valid = (flags & FL_NAN) != 0;  // only NaN values: the result is NaN
+          o.bits = o.valid ? f64_bits(__builtin_nan("")) : 0;
+          return o;
+        }
+        uint64_t v = ord_key_inverse(k);
+        if ((v << 1) == 0 && (A.mask & M_ZFIRST)) {
+          uint64_t z = b[A.o_zfirst];
+          if (z != kEmpty) v = (z & 1) << 63;  // the first zero's sign
+        }
+        o.bits = v;
+      } else {
+        o.valid = n > 0;
+        o.bits = o.valid ? b[is_min ? A.o_min : A.o_max] : 0;
+      }
+      return o;
+    }
+    case 5: {  // MEAN
+      o.valid = n > 0;
+      if (!o.valid) return o;
+      double total;
+      if (A.mask & M_ISUM64)
+        total = (double)(int64_t)b[A.o_isum];
+      else if (A.mask & M_ISUM128)
+        total = i128_to_f64(b[A.o_isum], b[A.o_isum + 1], 0);
+      else
+        total = finish_fsum(A, b, flags);
+      o.bits = f64_bits(total / (double)n);
+      return o;
+    }
+    default: {  // VAR / STD
+      o.valid = (int64_t)n - (int64_t)G.ddof > 0;
+      if (!o.valid) return o;
+      double v = finish_var(A, b, n, G.ddof, flags);
+      if (G.op == 7) v = sqrt(v);
+      o.bits = f64_bits(v);
+      return o;
+    }
+  }
+}
+
+__device__ __forceinline__ void store_value(void *dst, int dtype, uint64_t i, uint64_t bits) {
+  if (dtype == T_INT32 || dtype == T_DICT)
+    ((uint32_t *)dst)[i] = (uint32_t)bits;
+  else
+    ((uint64_t *)dst)[i] = bits;
+}
+
+__global__ void k_finalize(const DevPlan *__restrict__ plan) {
+  const DevPlan &P = *plan;
+  const uint64_t G = *P.groups;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < G;
+  const uint64_t nwords = (G + 63) >> 6;
+  const bool writer = (threadIdx.x & 63) == 0 && (i >> 6) < nwords;
+  uint64_t s = active ? P.order[i] : 0;
+  const uint64_t *slot = P.state + s * (uint64_t)P.words;
+  uint64_t packed = 0;
+  if (active) packed = (P.mode == MODE_DIRECT) ? s : P.keys_tab[s];
+  if (active && P.out_first) P.out_first[i] = slot[1];
+  for (int k = 0; k < P.n_keys; ++k) {
+    const DevKey &K = P.keys[k];
+    uint64_t digit = (packed / K.stride) % K.radix;
+    bool valid = !(K.nullable && digit == K.radix - 1);
+    uint64_t v = valid ? (uint64_t)K.minv + digit : 0;
+    if (active) store_value(P.out_keys[k], P.cols[k].dtype, i, v);
+    uint64_t ball = __ballot(active && valid);
+    if (writer) P.out_key_valid[k][i >> 6] = ball;
+  }
+  for (int a = 0; a < P.n_aggs; ++a) {
+    const DevAgg &Ga = P.aggs[a];
+    AggOut o = {0, false};
+    if (active) o = finish_agg(P, Ga, slot);
+    if (active) store_value(Ga.out_values, Ga.out_dtype, i, o.bits);
+    uint64_t ball = __ballot(active && o.valid);
+    if (writer) Ga.out_valid[i >> 6] = ball;
+  }
+}
+
+// ------------------------------------------------------- column stats
+
+__global__ void k_col_stats(DevCol C, int64_t n, ColStats *out) {
+  int64_t lo = INT64_MAX, hi = INT64_MIN;
+  uint32_t emin = 0x7ff, emax = 0, flags = 0;
+  uint64_t cnt = 0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    if (C.valid) {
+      int64_t bit = C.vbit + r;
+      if (!((C.valid[bit >> 6] >> (bit & 63)) & 1)) continue;
+    }
+    ++cnt;
+    if (C.dtype == T_FLOAT64) {
+      uint64_t b = ((const uint64_t *)C.values)[r];
+      uint32_t e = (uint32_t)(b >> 52) & 0x7ff;
+      if (e == 0x7ff) {
+        flags |= (b << 12) ? 1u : ((b >> 63) ? 4u : 2u);
+      } else if ((b << 1) == 0) {
+        flags |= (b >> 63) ? 8u : 16u;
+      } else {
+        flags |= 32u;
+        uint32_t eb = e ? e : 1;
+        emin = min(emin, eb);
+        emax = max(emax, e ? e : 1u);
+      }
+    } else {
+      int64_t v;
+      if (C.dtype == T_INT64)
+        v = ((const int64_t *)C.values)[r];
+      else if (C.dtype == T_INT32)
+        v = ((const int32_t *)C.values)[r];
+      else
+        v = ((const uint32_t *)C.values)[r];
+      lo = min(lo, v);
+      hi = max(hi, v);
+    }
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    lo = min(lo, (int64_t)__shfl_xor((long long)lo, d, 64));
+    hi = max(hi, (int64_t)__shfl_xor((long long)hi, d, 64));
+    emin = min(emin, (uint32_t)__shfl_xor((unsigned int)emin, d, 64));
+    emax = max(emax, (uint32_t)__shfl_xor((unsigned int)emax, d, 64));
+    flags |= (uint32_t)__shfl_xor((unsigned int)flags, d, 64);
+    cnt += (uint64_t)__shfl_xor((unsigned long long)cnt, d, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin((long long *)&out->imin, (long long)lo);
+    atomicMax((long long *)&out->imax, (long long)hi);
+    atomicMin(&out->emin, emin);
+    atomicMax(&out->emax, emax);
+    atomicOr(&out->flags, flags);
+    atomicAdd((unsigned long long *)&out->valid, (unsigned long long)cnt);
+  }
+}
+
+}  // namespace dfr

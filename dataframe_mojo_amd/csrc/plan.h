@@ -1,0 +1,149 @@
+// plan.h — the device-side form of one filtered group-by query.
+//
+// The host compiles a dfr_query into a DevPlan (one per query, kept in device
+// memory). Everything the kernels need is here: the input columns (key
+// columns first), the predicate program, the packed-key layout, and the
+// per-group state layout in HBM and in LDS.
+//
+// Per-group state is a row of 64-bit words ("slot"), stored array-of-structs
+// so that one group's words share cache lines:
+//   word 0            ROWS   rows of the group that passed the filter
+//   word 1 (optional) FIRST  smallest row index of the group (maintain_order)
+//   then, for each input column with aggregates, a block:
+//     VCOUNT  non-null values                        (1 word)
+//     SUM     int64 total (narrow) or 128-bit total  (1 or 2 words)
+//     SUMSQ   128-bit total of squares (fixed point) (2 words)
+//     MIN     int64 or ordered float key             (1 word)
+//     MAX                                            (1 word)
+//     FLAGS   NaN / +inf / -inf seen                 (1 word, HBM only)
+//     ZFIRST  first zero's (row << 1 | sign)         (1 word, HBM only)
+// An LDS entry holds the same words except FLAGS and ZFIRST, which are
+// rare-event words updated in HBM directly. Float sums are exact 128-bit
+// fixed-point integers (scale 2^shift chosen per column from its exponent
+// range), so every state is order independent and results are bitwise
+// reproducible whatever the launch geometry or batching.
+#pragma once
+#include <stdint.h>
+
+namespace dfr {
+
+constexpr int kMaxCols = 12;
+constexpr int kMaxKeys = 4;
+constexpr int kMaxLeaves = 8;
+constexpr int kMaxWords = 64;     // HBM words per group slot
+constexpr int kMaxAggs = 32;
+constexpr uint64_t kEmpty = ~0ull;  // empty hash-table key
+
+// column block masks
+enum : uint32_t {
+  M_VCOUNT = 1u << 0,
+  M_ISUM64 = 1u << 1,
+  M_ISUM128 = 1u << 2,
+  M_FSUM = 1u << 3,
+  M_FSUMSQ = 1u << 4,
+  M_IMIN = 1u << 5,
+  M_IMAX = 1u << 6,
+  M_FMIN = 1u << 7,
+  M_FMAX = 1u << 8,
+  M_FLAGS = 1u << 9,
+  M_ZFIRST = 1u << 10,
+};
+
+// flag bits of the FLAGS word
+enum : uint64_t { FL_NAN = 1, FL_PINF = 2, FL_NINF = 4 };
+
+enum : int32_t { MODE_DIRECT = 0, MODE_HASH = 1 };
+enum : int32_t { LDS_NONE = 0, LDS_DIRECT = 1, LDS_HASHED = 2 };
+
+// dtypes (same values as dfrocm.h)
+enum : int32_t { T_INT32 = 1, T_INT64 = 2, T_FLOAT64 = 3, T_DICT = 4 };
+
+struct DevCol {
+  const void *values;     // first element of the window
+  const uint64_t *valid;  // LSB-first validity words, or nullptr
+  int64_t vbit;           // bit of row 0 in `valid`
+  int32_t dtype;
+  int32_t is_float;       // 1 for FLOAT64
+};
+
+// A key column's digit in the mixed-radix packed key:
+// digit = null ? radix - 1 : value - minv; packed = sum(digit * stride).
+struct DevKey {
+  int64_t minv;
+  uint64_t radix;
+  uint64_t stride;
+  int32_t nullable;
+  int32_t pad;
+};
+
+struct DevLeaf {
+  int32_t op;
+  int32_t flags;
+  int32_t bit;  // position of this leaf's is-true bit in the truth index
+  int32_t pad;
+  int64_t lit;
+  int64_t mod;
+};
+
+// Aggregate blocks of one column.
+struct DevColAgg {
+  uint32_t mask;
+  int16_t gbase;   // first HBM word of the block
+  int16_t lbase;   // first LDS word of the block (LDS entry without its key)
+  // word offsets inside the block (-1 when absent)
+  int8_t o_count, o_isum, o_fsum, o_sq, o_min, o_max, o_flags, o_zfirst;
+  int32_t fshift;   // scale of the 128-bit fixed-point sum
+  int32_t fshift2;  // scale of the 128-bit fixed-point sum of squares
+  int32_t int_as_float;  // an integer column whose VAR goes through doubles
+  int32_t pad2;
+};
+
+// Output aggregate (finalize).
+struct DevAgg {
+  int32_t op;         // DFR_AGG_*
+  int32_t col;        // index into cols
+  int32_t ddof;
+  int32_t out_dtype;  // T_INT32 / T_INT64 / T_FLOAT64
+  int64_t min_count;
+  void *out_values;
+  uint64_t *out_valid;
+};
+
+struct DevPlan {
+  int64_t rows;
+  int32_t n_cols, n_keys, n_leaves, n_aggs;
+  int32_t words;       // HBM words per slot
+  int32_t lds_words;   // LDS words per entry (not counting the key word)
+  int32_t mode;        // MODE_DIRECT / MODE_HASH
+  int32_t lds_mode;    // LDS_NONE / LDS_DIRECT / LDS_HASHED
+  int32_t lds_entries; // power of two (LDS_HASHED) or >= domain (LDS_DIRECT)
+  int32_t lds_log2;
+  int32_t lds_probe;   // probes before a row goes to HBM
+  int32_t has_first;   // FIRST word present (maintain_order)
+  int32_t vec_ok;      // every column window is 16-byte aligned
+  int32_t leaf_cols;   // bit c set: column c carries predicate leaves
+  int32_t max_probe;   // HBM hash table probes before a capacity fault
+  int32_t cap_log2;
+  uint32_t truth[8];
+  uint64_t cap;         // slots
+  uint64_t domain;      // product of key radices
+  uint64_t *keys_tab;   // MODE_HASH: packed key per slot
+  uint64_t *state;      // cap * words
+  uint32_t *fault;      // [0] capacity, [1] Int64 sum overflow, [2] int32 sum overflow
+  uint64_t ident[kMaxWords];  // identity value of each HBM word
+  int8_t lds_to_g[kMaxWords];  // HBM word of each LDS word
+  DevCol cols[kMaxCols];
+  DevColAgg cagg[kMaxCols];
+  DevKey keys[kMaxKeys];
+  int8_t leaf_begin[kMaxCols + 1];  // leaves sorted by column
+  DevLeaf leaves[kMaxLeaves];
+  DevAgg aggs[kMaxAggs];
+  // outputs of finalize
+  void *out_keys[kMaxKeys];
+  uint64_t *out_key_valid[kMaxKeys];
+  uint64_t *order;      // output index -> slot
+  uint64_t *out_first;  // first kept row per output group (when has_first)
+  uint64_t *groups;     // device scalar: number of groups
+};
+
+}  // namespace dfr

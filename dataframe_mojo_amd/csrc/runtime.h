@@ -1,0 +1,103 @@
+// runtime.h — host-side objects behind the opaque handles of dfrocm.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <vector>
+
+#include "../../include/dfrocm.h"
+#include "plan.h"
+
+namespace dfr {
+
+// Column statistics gathered on the device when a column is handed off.
+struct ColStats {
+  int64_t imin, imax;   // integer columns: min/max of valid values
+  uint32_t emin, emax;  // FLOAT64: biased exponent range of finite non-zero values
+  uint32_t flags;       // FLOAT64: 1 NaN, 2 +inf, 4 -inf, 8 -0.0, 16 +0.0, 32 finite non-zero
+  uint32_t pad;
+  uint64_t valid;       // non-null values
+};
+
+// Launch and table parameters. Defaults come from the device; the autotuner
+// overrides them per context (dfr_context_tune).
+struct Tuning {
+  int block = 512;             // threads per workgroup of the scan
+  int wg_per_cu = 2;           // persistent workgroups per CU
+  int lds_bytes = 65536;       // LDS table bytes per workgroup
+  int lds_probe = 4;           // LDS probes before a row goes to HBM
+  int lds_mode = -1;           // -1 automatic, else LDS_NONE/LDS_DIRECT/LDS_HASHED
+  int64_t direct_max_slots = int64_t(1) << 26;  // largest direct-addressed HBM table
+  int hash_probe = 1 << 20;    // HBM probes before a capacity fault
+  int64_t hash_slots = 0;      // 0: sized from the key domain and the rows
+};
+
+}  // namespace dfr
+
+struct Pool {
+  int64_t budget = 0;
+  size_t reserved = 0;  // bytes held from hipMalloc (live + cached)
+  size_t cached = 0;
+  size_t peak = 0;
+  std::map<void *, size_t> live;
+  std::map<size_t, std::vector<void *>> free_;
+  int alloc(size_t bytes, void **out, char **error);
+  void release(void *p);
+  void trim();
+  static size_t round_size(size_t bytes);
+};
+
+constexpr size_t kStagingBytes = 1 << 20;
+
+struct dfr_context {
+  int device = 0;
+  int cus = 0;
+  int lds_per_block = 65536;
+  hipStream_t stream = nullptr;
+  Pool pool;
+  char *staging = nullptr;  // pinned host bytes for plans and small read-backs
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evp[4] = {nullptr, nullptr, nullptr, nullptr};  // phase boundaries
+  dfr_observation obs = {};
+  dfr::Tuning tune;
+};
+
+struct dfr_column {
+  dfr_context *ctx = nullptr;
+  int32_t dtype = 0;
+  int64_t length = 0;
+  int64_t dict_size = 0;
+  int64_t nulls = 0;
+  const void *values = nullptr;
+  const uint64_t *valid = nullptr;
+  int64_t vbit = 0;
+  void *owned_values = nullptr;
+  void *owned_valid = nullptr;
+  dfr::ColStats stats = {};
+  dfr::DevCol dev() const {
+    dfr::DevCol c;
+    c.values = values;
+    c.valid = valid;
+    c.vbit = vbit;
+    c.dtype = dtype;
+    c.is_float = dtype == DFR_FLOAT64 ? 1 : 0;
+    return c;
+  }
+};
+
+struct dfr_result {
+  dfr_context *ctx = nullptr;
+  int64_t groups = 0;
+  std::vector<int32_t> key_dtypes, agg_dtypes;
+  std::vector<void *> key_values, agg_values;
+  std::vector<uint64_t *> key_valid, agg_valid;
+  uint64_t *first_rows = nullptr;
+  void *buffer = nullptr;  // one pool block holding every output
+  size_t bytes = 0;
+  double device_ms = 0.0;
+  double phase_ms[4] = {0, 0, 0, 0};
+};
+
+int dfr_fail(char **error, int status, const char *fmt, ...);
+dfr::Tuning default_tuning(dfr_context *ctx);
+int column_stats(dfr_context *ctx, dfr_column *col, char **error);

@@ -1,0 +1,154 @@
+"""Expressions for the filter -> group_by -> agg path (expr.mojo:2869-3006).
+
+Nodes: col, lit, comparisons (== != < <= > >=), %, & | ~, is_null,
+is_not_null, and the reductions sum(min_count), count, len, min, max, mean,
+var(ddof), std(ddof), with alias.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+CMP_OPS = ("eq", "ne", "lt", "le", "gt", "ge")
+AGG_OPS = ("len", "count", "sum", "min", "max", "mean", "var", "std")
+
+
+@dataclass(frozen=True)
+class Expr:
+    kind: str
+    args: tuple = ()
+    name: str | None = None
+
+    # naming ------------------------------------------------------------
+    def alias(self, name: str) -> "Expr":
+        return Expr(self.kind, self.args, name)
+
+    def output_name(self) -> str:
+        if self.name is not None:
+            return self.name
+        if self.kind == "col":
+            return self.args[0]
+        if self.kind == "agg" and self.args[0] == "len" and self.args[1] is None:
+            return "len"
+        for a in self.args:
+            if isinstance(a, Expr):
+                return a.output_name()
+        return "literal"
+
+    # comparisons ---------------------------------------------------------
+    def _cmp(self, op: str, other) -> "Expr":
+        return Expr("cmp", (op, self, _wrap(other)))
+
+    def __eq__(self, other):  # type: ignore[override]
+        return self._cmp("eq", other)
+
+    def __ne__(self, other):  # type: ignore[override]
+        return self._cmp("ne", other)
+
+    def __lt__(self, other):
+        return self._cmp("lt", other)
+
+    def __le__(self, other):
+        return self._cmp("le", other)
+
+    def __gt__(self, other):
+        return self._cmp("gt", other)
+
+    def __ge__(self, other):
+        return self._cmp("ge", other)
+
+    def eq(self, other):
+        return self._cmp("eq", other)
+
+    def ne(self, other):
+        return self._cmp("ne", other)
+
+    __hash__ = object.__hash__
+
+    # arithmetic / logic --------------------------------------------------
+    def __mod__(self, other) -> "Expr":
+        return Expr("mod", (self, _wrap(other)))
+
+    def __and__(self, other) -> "Expr":
+        return Expr("and", (self, _wrap(other)))
+
+    def __or__(self, other) -> "Expr":
+        return Expr("or", (self, _wrap(other)))
+
+    def __invert__(self) -> "Expr":
+        return Expr("not", (self,))
+
+    def is_null(self) -> "Expr":
+        return Expr("is_null", (self,))
+
+    def is_not_null(self) -> "Expr":
+        return Expr("is_not_null", (self,))
+
+    # reductions ----------------------------------------------------------
+    def _agg(self, op: str, **kw) -> "Expr":
+        return Expr("agg", (op, self, tuple(sorted(kw.items()))))
+
+    def sum(self, min_count: int = 0) -> "Expr":
+        """Skip nulls; zero when empty unless fewer than min_count are valid."""
+        return self._agg("sum", min_count=min_count)
+
+    def count(self) -> "Expr":
+        """Number of non-null values, as Int64."""
+        return self._agg("count")
+
+    def len(self) -> "Expr":
+        """Number of rows including nulls, as Int64."""
+        return self._agg("len")
+
+    def min(self) -> "Expr":
+        """Smallest non-null value; NaN only when every valid value is NaN."""
+        return self._agg("min")
+
+    def max(self) -> "Expr":
+        """Largest non-null value; NaN only when every valid value is NaN."""
+        return self._agg("max")
+
+    def mean(self) -> "Expr":
+        """Arithmetic mean of non-null values as Float64; null when empty."""
+        return self._agg("mean")
+
+    def var(self, ddof: int = 1) -> "Expr":
+        """Variance; null when fewer than ddof + 1 values."""
+        return self._agg("var", ddof=ddof)
+
+    def std(self, ddof: int = 1) -> "Expr":
+        """Standard deviation; null when fewer than ddof + 1 values."""
+        return self._agg("std", ddof=ddof)
+
+    def __repr__(self) -> str:
+        if self.kind == "col":
+            s = f"col({self.args[0]!r})"
+        elif self.kind == "lit":
+            s = f"lit({self.args[0]!r})"
+        elif self.kind == "cmp":
+            s = f"({self.args[1]!r} {self.args[0]} {self.args[2]!r})"
+        elif self.kind == "agg":
+            s = f"{self.args[1]!r}.{self.args[0]}({dict(self.args[2])})" if self.args[1] is not None else "len()"
+        else:
+            s = f"{self.kind}({', '.join(repr(a) for a in self.args)})"
+        return s if self.name is None else f"{s}.alias({self.name!r})"
+
+
+def _wrap(v) -> Expr:
+    return v if isinstance(v, Expr) else lit(v)
+
+
+def col(name: str) -> Expr:
+    return Expr("col", (name,))
+
+
+def lit(value) -> Expr:
+    return Expr("lit", (value,))
+
+
+def len_() -> Expr:
+    """len(): rows per group (the reference's `len()` expression)."""
+    return Expr("agg", ("len", None, ()))
+
+
+def sum_(name: str) -> Expr:
+    return col(name).sum()

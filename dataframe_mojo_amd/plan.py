@@ -1,0 +1,235 @@
+"""Compiles a LazyFrame of the form source -> filter* -> group_by -> agg into
+the device query: the predicate program (leaves + truth table, as
+dataframe_metal/predicate_program.mojo:1-71 compiles positive AND/OR
+predicates), the key list and the aggregate list.
+
+NOT is pushed into the leaves with Kleene semantics: NOT(a AND b) is
+NOT a OR NOT b, and a negated comparison stays "not true" on a null value.
+After that the formula is monotone in the leaves' is-true bits, so a 256-bit
+truth table over those bits is the SQL filter (false and null dropped,
+docs/semantics.md:207-209).
+
+Anything outside the device path raises `Unsupported` before any device
+work, as the accelerator contract asks (docs/design/accelerator-contract.md).
+"""
+from __future__ import annotations
+
+import math
+import struct
+from dataclasses import dataclass, field
+
+from . import _lib
+from .expr import Expr
+from .frame import FLOAT64, INT32, INT64, STRING, LazyFrame
+
+_CMP = {"eq": _lib.EQ, "ne": _lib.NE, "lt": _lib.LT, "le": _lib.LE, "gt": _lib.GT, "ge": _lib.GE}
+_FLIP = {"eq": "eq", "ne": "ne", "lt": "gt", "le": "ge", "gt": "lt", "ge": "le"}
+_AGG = {
+    "len": _lib.AGG_LEN,
+    "count": _lib.AGG_COUNT,
+    "sum": _lib.AGG_SUM,
+    "min": _lib.AGG_MIN,
+    "max": _lib.AGG_MAX,
+    "mean": _lib.AGG_MEAN,
+    "var": _lib.AGG_VAR,
+    "std": _lib.AGG_STD,
+}
+DTYPE_CODE = {INT32: _lib.DFR_INT32, INT64: _lib.DFR_INT64, FLOAT64: _lib.DFR_FLOAT64, STRING: _lib.DFR_DICT}
+
+
+class Unsupported(_lib.Unsupported):
+    def __init__(self, message: str):
+        super().__init__(_lib.DFR_UNSUPPORTED, message)
+
+
+@dataclass
+class Leaf:
+    column: str
+    op: int
+    flags: int = 0
+    literal: int = 0  # int64 value or double bits
+    modulus: int = 0
+
+
+@dataclass
+class Program:
+    leaves: list = field(default_factory=list)
+    truth: list = field(default_factory=lambda: [0] * 8)
+
+
+@dataclass
+class DevicePlan:
+    keys: list
+    columns: list  # names of the query's input columns
+    program: Program
+    aggs: list  # (op, column index or -1, ddof, min_count)
+    agg_names: list
+    maintain_order: bool
+
+    @property
+    def output_names(self) -> list:
+        return list(self.keys) + list(self.agg_names)
+
+
+def _f64_bits(x: float) -> int:
+    return struct.unpack("<q", struct.pack("<d", float(x)))[0]
+
+
+def _is_int(v) -> bool:
+    return isinstance(v, int) and not isinstance(v, bool)
+
+
+class _Compiler:
+    def __init__(self, frame):
+        self.frame = frame
+        self.leaves: list[Leaf] = []
+
+    def dtype(self, name: str) -> str:
+        try:
+            return self.frame.column(name).dtype
+        except KeyError:
+            raise Unsupported(f"no column named {name!r}") from None
+
+    # returns a formula: ("leaf", i) | ("and", a, b) | ("or", a, b) | ("const", bool)
+    def formula(self, e: Expr, neg: bool):
+        k = e.kind
+        if k == "not":
+            return self.formula(e.args[0], not neg)
+        if k in ("and", "or"):
+            a = self.formula(e.args[0], neg)
+            b = self.formula(e.args[1], neg)
+            op = k if not neg else ("or" if k == "and" else "and")
+            return (op, a, b)
+        if k == "lit" and isinstance(e.args[0], bool):
+            return ("const", e.args[0] != neg)
+        if k in ("is_null", "is_not_null"):
+            target = e.args[0]
+            if target.kind != "col":
+                raise Unsupported("is_null of an expression")
+            op = _lib.IS_NULL if (k == "is_null") != neg else _lib.IS_NOT_NULL
+            return self.leaf(Leaf(target.args[0], op))
+        if k == "cmp":
+            return self.comparison(e, neg)
+        if k == "col" and self.dtype(e.args[0]) not in (INT32, INT64, FLOAT64, STRING):
+            raise Unsupported("boolean columns are not on the device path")
+        raise Unsupported(f"predicate {e!r} is not on the device path")
+
+    def leaf(self, leaf: Leaf):
+        self.leaves.append(leaf)
+        if len(self.leaves) > _lib.MAX_LEAVES:
+            raise Unsupported(f"more than {_lib.MAX_LEAVES} predicate leaves")
+        return ("leaf", len(self.leaves) - 1)
+
+    def comparison(self, e: Expr, neg: bool):
+        op, a, b = e.args
+        if a.kind == "lit" and b.kind != "lit":
+            a, b, op = b, a, _FLIP[op]
+        if b.kind != "lit":
+            raise Unsupported("comparisons between two columns are not on the device path")
+        value = b.args[0]
+        flags = _lib.LEAF_NOT if neg else 0
+        modulus = 0
+        if a.kind == "mod":
+            base, m = a.args
+            if base.kind != "col" or m.kind != "lit" or not _is_int(m.args[0]):
+                raise Unsupported("only column % integer literal")
+            a = base
+            modulus = int(m.args[0])
+            flags |= _lib.LEAF_MOD
+        if a.kind != "col":
+            raise Unsupported(f"comparison of {a!r}")
+        name = a.args[0]
+        dt = self.dtype(name)
+        if value is None:
+            # a comparison with null is null: never true, also when negated
+            return ("const", False)
+        if dt == STRING:
+            if modulus or op not in ("eq", "ne") or not isinstance(value, str):
+                raise Unsupported("dictionary columns compare by (in)equality with a string")
+            labels = self.frame.column(name).column.labels
+            code = labels.index(value) if value in labels else -1
+            return self.leaf(Leaf(name, _CMP[op], flags, code))
+        if dt == FLOAT64:
+            if modulus:
+                raise Unsupported("Float64 modulo is not on the device path")
+            if not isinstance(value, (int, float)) or isinstance(value, bool):
+                raise Unsupported(f"Float64 column compared with {value!r}")
+            return self.leaf(Leaf(name, _CMP[op], flags, _f64_bits(float(value))))
+        # integer column
+        if isinstance(value, float):
+            if not math.isfinite(value) or value != int(value):
+                raise Unsupported("integer column compared with a fractional Float literal")
+            value = int(value)
+        if not _is_int(value):
+            raise Unsupported(f"integer column compared with {value!r}")
+        if not (-(1 << 63) <= value < (1 << 63)):
+            raise Unsupported("literal outside Int64")
+        return self.leaf(Leaf(name, _CMP[op], flags, int(value), modulus))
+
+
+def _eval(f, bits: int) -> bool:
+    t = f[0]
+    if t == "leaf":
+        return bool((bits >> f[1]) & 1)
+    if t == "const":
+        return f[1]
+    if t == "and":
+        return _eval(f[1], bits) and _eval(f[2], bits)
+    return _eval(f[1], bits) or _eval(f[2], bits)
+
+
+def compile_predicate(frame, filters) -> Program:
+    comp = _Compiler(frame)
+    formula = ("const", True)
+    for f in filters:
+        formula = ("and", formula, comp.formula(f, False))
+    prog = Program(leaves=comp.leaves)
+    n = len(comp.leaves)
+    for m in range(1 << n):
+        if _eval(formula, m):
+            prog.truth[m >> 5] |= 1 << (m & 31)
+    return prog
+
+
+def compile_plan(plan: LazyFrame) -> DevicePlan:
+    if not plan.grouped:
+        raise Unsupported("the device path runs filter -> group_by -> agg plans")
+    frame = plan.source
+    if not plan.keys:
+        raise Unsupported("group_by needs at least one key")
+    for k in plan.keys:
+        dt = frame.column(k).dtype
+        if dt == FLOAT64:
+            raise Unsupported("Float64 group keys are not on the device path")
+    program = compile_predicate(frame, plan.filters)
+    columns: list[str] = []
+
+    def colindex(name: str) -> int:
+        frame.column(name)
+        if name not in columns:
+            columns.append(name)
+        return columns.index(name)
+
+    for leaf in program.leaves:
+        colindex(leaf.column)
+    aggs, names = [], []
+    for e in plan.aggs:
+        if not isinstance(e, Expr) or e.kind != "agg":
+            raise Unsupported(f"{e!r} is not a reduction")
+        op, target, kw = e.args
+        kw = dict(kw)
+        if target is None:
+            aggs.append((_lib.AGG_LEN, -1, 0, 0))
+        else:
+            if target.kind != "col":
+                raise Unsupported("reductions of expressions are not on the device path")
+            name = target.args[0]
+            dt = frame.column(name).dtype
+            if dt == STRING and op not in ("count", "len"):
+                raise Unsupported(f"{op} of a String column is not on the device path")
+            aggs.append((_AGG[op], colindex(name), int(kw.get("ddof", 1)), int(kw.get("min_count", 0))))
+        names.append(e.output_name())
+    out = list(plan.keys) + names
+    if len(set(out)) != len(out):
+        raise ValueError(f"duplicate output names {out}")
+    return DevicePlan(list(plan.keys), columns, program, aggs, names, plan.maintain_order)

@@ -1,0 +1,301 @@
+"""Device contexts, device columns and query execution over libdfrocm.
+
+A `Context` owns one device, one HIP stream and one HBM budget
+(DATAFRAME_ACCEL_MEMORY_LIMIT, as dataframe_metal/metal.mojo:36-46 reads it).
+Columns reach the device by upload (host arrays) or by wrapping device
+memory that already holds them (zero copy; e.g. torch tensors), which is the
+device column hand-off of storage/residence.mojo.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+from . import _lib
+from .frame import FLOAT64, INT32, INT64, STRING, Column, DataFrame, Series
+from .plan import DTYPE_CODE, DevicePlan
+
+MEMORY_LIMIT_ENV = "DATAFRAME_ACCEL_MEMORY_LIMIT"
+DEVICE_ENV = "DATAFRAME_ACCEL_DEVICE"
+
+_NP_OF = {_lib.DFR_INT32: np.int32, _lib.DFR_INT64: np.int64, _lib.DFR_FLOAT64: np.float64, _lib.DFR_DICT: np.uint32}
+_NAME_OF = {_lib.DFR_INT32: INT32, _lib.DFR_INT64: INT64, _lib.DFR_FLOAT64: FLOAT64, _lib.DFR_DICT: STRING}
+
+
+def memory_limit() -> int:
+    """DATAFRAME_ACCEL_MEMORY_LIMIT in bytes (0 = the device's free memory);
+    a bad value raises, as the reference's runtime configuration does."""
+    raw = os.environ.get(MEMORY_LIMIT_ENV, "").strip()
+    if not raw:
+        return 0
+    try:
+        v = int(raw)
+    except ValueError:
+        raise ValueError(f"{MEMORY_LIMIT_ENV}={raw!r} is not a byte count") from None
+    if v <= 0:
+        raise ValueError(f"{MEMORY_LIMIT_ENV} must be positive")
+    return v
+
+
+def device_ordinal() -> int:
+    raw = os.environ.get(DEVICE_ENV, "").strip()
+    if not raw:
+        return 0
+    try:
+        v = int(raw)
+    except ValueError:
+        raise ValueError(f"{DEVICE_ENV}={raw!r} is not a device ordinal") from None
+    if v < 0:
+        raise ValueError(f"{DEVICE_ENV} must be non-negative")
+    return v
+
+
+def _err():
+    return ctypes.c_void_p()
+
+
+class DeviceColumn:
+    """A column on the device: a libdfrocm handle plus what keeps its memory
+    alive (for wrapped device memory) and its labels (dictionary Strings)."""
+
+    def __init__(self, ctx: "Context", handle: ctypes.c_void_p, dtype: int, length: int, labels=None, keepalive=None):
+        self.ctx = ctx
+        self.handle = handle
+        self.dtype = dtype
+        self.length = length
+        self.labels = labels
+        self.keepalive = keepalive
+
+    def null_count(self) -> int:
+        return int(self.ctx.lib.dfr_column_null_count(self.handle))
+
+    def release(self):
+        if self.handle:
+            self.ctx.lib.dfr_column_release(self.handle)
+            self.handle = None
+            self.keepalive = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Result:
+    """A group-by result resident on the device until copied out."""
+
+    def __init__(self, ctx: "Context", handle: ctypes.c_void_p, n_keys: int, n_aggs: int):
+        self.ctx = ctx
+        self.handle = handle
+        self.n_keys = n_keys
+        self.n_aggs = n_aggs
+        self.groups = int(ctx.lib.dfr_result_groups(handle))
+        self.device_ms = float(ctx.lib.dfr_result_device_ms(handle))
+        self.device_bytes = int(ctx.lib.dfr_result_device_bytes(handle))
+        self.phase_ms = [float(ctx.lib.dfr_result_phase_ms(handle, p)) for p in range(4)]
+
+    def _copy(self, fn, i: int, dtype: int):
+        n = self.groups
+        vals = np.empty(n, dtype=_NP_OF[dtype])
+        bits = np.empty((n + 7) // 8 + 1, dtype=np.uint8)
+        e = _err()
+        _lib.check(fn(self.handle, i, vals.ctypes.data_as(ctypes.c_void_p), bits.ctypes.data_as(ctypes.c_void_p), ctypes.byref(e)), e)
+        valid = np.unpackbits(bits, bitorder="little")[:n].astype(bool)
+        return vals, valid
+
+    def key(self, k: int):
+        dt = self.ctx.lib.dfr_result_key_dtype(self.handle, k)
+        return dt, *self._copy(self.ctx.lib.dfr_result_copy_key, k, dt)
+
+    def agg(self, a: int):
+        dt = self.ctx.lib.dfr_result_agg_dtype(self.handle, a)
+        return dt, *self._copy(self.ctx.lib.dfr_result_copy_agg, a, dt)
+
+    def first_rows(self) -> np.ndarray:
+        out = np.empty(self.groups, dtype=np.int64)
+        e = _err()
+        _lib.check(self.ctx.lib.dfr_result_copy_first_rows(self.handle, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(e)), e)
+        return out
+
+    def release(self):
+        if self.handle:
+            self.ctx.lib.dfr_result_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def build_query(plan: DevicePlan, keys: list, columns: list):
+    """The dfr_query struct (and the arrays it points at) of a compiled plan
+    over device columns."""
+    q = _lib.Query()
+    key_arr = (ctypes.c_void_p * max(1, len(keys)))(*[k.handle for k in keys])
+    col_arr = (ctypes.c_void_p * max(1, len(columns)))(*[c.handle for c in columns])
+    q.n_keys = len(keys)
+    q.n_columns = len(columns)
+    q.keys = ctypes.cast(key_arr, ctypes.POINTER(ctypes.c_void_p))
+    q.columns = ctypes.cast(col_arr, ctypes.POINTER(ctypes.c_void_p))
+    prog = plan.program
+    q.predicate.n_leaves = len(prog.leaves)
+    for i in range(8):
+        q.predicate.truth[i] = prog.truth[i] & 0xFFFFFFFF
+    for i, leaf in enumerate(prog.leaves):
+        L = q.predicate.leaves[i]
+        L.column = plan.columns.index(leaf.column)
+        L.op = leaf.op
+        L.flags = leaf.flags
+        L.literal = leaf.literal
+        L.modulus = leaf.modulus
+    aggs = (_lib.Agg * max(1, len(plan.aggs)))()
+    for i, (op, c, ddof, min_count) in enumerate(plan.aggs):
+        aggs[i].op = op
+        aggs[i].column = max(c, 0)
+        aggs[i].ddof = ddof
+        aggs[i].min_count = min_count
+    q.n_aggs = len(plan.aggs)
+    q.aggs = ctypes.cast(aggs, ctypes.POINTER(_lib.Agg))
+    q.maintain_order = 1 if plan.maintain_order else 0
+    return q, (key_arr, col_arr, aggs)
+
+
+class Context:
+    def __init__(self, device: int = 0, budget: int = 0):
+        self.lib = _lib.load()
+        self.device = device
+        h = ctypes.c_void_p()
+        e = _err()
+        _lib.check(self.lib.dfr_context_create(device, budget, ctypes.byref(h), ctypes.byref(e)), e)
+        self.handle = h
+        self.lock = threading.Lock()
+
+    def close(self):
+        if self.handle:
+            self.lib.dfr_context_destroy(self.handle)
+            self.handle = None
+
+    def tune(self, key: str, value: int):
+        e = _err()
+        _lib.check(self.lib.dfr_context_tune(self.handle, key.encode(), int(value), ctypes.byref(e)), e)
+
+    def observe(self) -> dict:
+        o = _lib.Observation()
+        self.lib.dfr_context_observe(self.handle, ctypes.byref(o))
+        return {f: getattr(o, f) for f, _ in o._fields_}
+
+    # columns -------------------------------------------------------------
+    def upload(self, column: Column) -> DeviceColumn:
+        dt = DTYPE_CODE[column.dtype]
+        vals = np.ascontiguousarray(column.values)
+        bits = column.validity_bytes()
+        h = ctypes.c_void_p()
+        e = _err()
+        _lib.check(
+            self.lib.dfr_column_upload(
+                self.handle,
+                dt,
+                vals.ctypes.data_as(ctypes.c_void_p),
+                0,
+                len(vals),
+                None if bits is None else bits.ctypes.data_as(ctypes.c_void_p),
+                0,
+                len(column.labels or []),
+                ctypes.byref(h),
+                ctypes.byref(e),
+            ),
+            e,
+        )
+        return DeviceColumn(self, h, dt, len(vals), column.labels)
+
+    def wrap(self, dtype: int, values_ptr: int, length: int, valid_ptr: int = 0, bit_offset: int = 0,
+             offset: int = 0, labels=None, keepalive=None) -> DeviceColumn:
+        h = ctypes.c_void_p()
+        e = _err()
+        _lib.check(
+            self.lib.dfr_column_wrap(
+                self.handle, dtype, ctypes.c_void_p(values_ptr), offset, length,
+                ctypes.c_void_p(valid_ptr) if valid_ptr else None, bit_offset,
+                len(labels or []), ctypes.byref(h), ctypes.byref(e),
+            ),
+            e,
+        )
+        return DeviceColumn(self, h, dtype, length, labels, keepalive)
+
+    def wrap_tensor(self, values, valid_words=None, labels=None, dict_codes=False) -> DeviceColumn:
+        """Wraps torch device tensors (values; optional int64 validity words)."""
+        import torch
+
+        dt = {torch.int32: _lib.DFR_DICT if dict_codes else _lib.DFR_INT32, torch.int64: _lib.DFR_INT64,
+              torch.float64: _lib.DFR_FLOAT64}[values.dtype]
+        return self.wrap(dt, values.data_ptr(), values.numel(), valid_words.data_ptr() if valid_words is not None else 0,
+                         labels=labels, keepalive=(values, valid_words))
+
+    # queries ---------------------------------------------------------------
+    def groupby(self, plan: DevicePlan, keys: list, columns: list) -> Result:
+        q, keep = build_query(plan, keys, columns)
+        h = ctypes.c_void_p()
+        e = _err()
+        with self.lock:
+            _lib.check(self.lib.dfr_groupby(self.handle, ctypes.byref(q), ctypes.byref(h), ctypes.byref(e)), e)
+        del keep
+        return Result(self, h, len(keys), len(plan.aggs))
+
+
+_contexts: dict = {}
+_ctx_lock = threading.Lock()
+
+
+def context(device: int | None = None) -> Context:
+    """The process's context for a device (created on first use)."""
+    if device is None:
+        device = device_ordinal()
+    with _ctx_lock:
+        ctx = _contexts.get(device)
+        if ctx is None:
+            ctx = Context(device, memory_limit())
+            _contexts[device] = ctx
+        return ctx
+
+
+def result_frame(res: Result, plan: DevicePlan, key_cols: list) -> DataFrame:
+    """Copies a device result into a host DataFrame: keys (dictionary keys
+    decoded through their labels), then the aggregates."""
+    series = []
+    for k, name in enumerate(plan.keys):
+        dt, vals, valid = res.key(k)
+        if dt == _lib.DFR_DICT:
+            col = Column(STRING, vals, valid, key_cols[k].labels)
+        else:
+            col = Column(_NAME_OF[dt], vals, valid)
+        series.append(Series(name, col))
+    for a, name in enumerate(plan.agg_names):
+        dt, vals, valid = res.agg(a)
+        series.append(Series(name, Column(_NAME_OF[dt], vals, valid)))
+    return DataFrame(series)
+
+
+def run_plan(ctx: Context, frame: DataFrame, plan: DevicePlan) -> tuple[DataFrame, Result]:
+    """Uploads the columns a plan reads, runs it, and returns the host frame."""
+    uploaded: dict = {}
+
+    def dev(name):
+        if name not in uploaded:
+            uploaded[name] = ctx.upload(frame.column(name).column)
+        return uploaded[name]
+
+    keys = [dev(k) for k in plan.keys]
+    cols = [dev(c) for c in plan.columns]
+    try:
+        res = ctx.groupby(plan, keys, cols)
+        out = result_frame(res, plan, keys)
+    finally:
+        for d in uploaded.values():
+            d.release()
+    return out, res

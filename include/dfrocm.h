@@ -1,0 +1,263 @@
+/*
+ * dfrocm.h — C ABI of libdfrocm, the MI355X filtered group-by provider for
+ * dataframe_mojo.
+ *
+ * The reference's accelerator boundary is thirteen Mojo functions
+ * (providers/cpu/dataframe_provider/__init__.mojo:15-70, imported only by
+ * dataframe/_accel_provider.mojo:10-24). A provider package written in the
+ * host language implements those functions and reaches the device through
+ * this library. Every entry point below is plain C: pointers, sizes and
+ * integer status codes, no C++ or torch types.
+ *
+ * Conventions follow the reference's one native library
+ * (native/dfparquet/dfparquet.cc:51-56, dataframe/io/native_library.mojo):
+ *   - every fallible call returns an int status (DFR_OK on success);
+ *   - on failure, when `error` is non-null, *error receives a malloc'd,
+ *     NUL-terminated message that the caller frees with dfr_free();
+ *   - only dfr_* symbols are exported (version script libdfrocm.map).
+ *
+ * Threading: one dfr_context serves one caller thread at a time. Concurrent
+ * callers use distinct contexts (each owns its own HIP stream).
+ */
+#ifndef DFROCM_H
+#define DFROCM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFR_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define DFR_OK 0
+/* The query is outside what the device path supports; reported before any
+ * device work is submitted (accelerator contract: unsupported work raises
+ * before submission, docs/design/accelerator-contract.md:95-103). */
+#define DFR_UNSUPPORTED 1
+/* A data fault: an aggregate's final value is out of its result type's
+ * range. The message text is the reference's own error text
+ * ("Int64 expression sum overflow", function.mojo:2786-2798;
+ * "int32 expression sum overflow", function.mojo:2906-2914). */
+#define DFR_DATA_FAULT 2
+/* The query does not fit the context's HBM budget. */
+#define DFR_CAPACITY 3
+/* A HIP runtime error. */
+#define DFR_DEVICE 4
+/* Invalid arguments (null pointers, mismatched lengths, bad indexes). */
+#define DFR_INVALID 5
+
+/* ---- column types ------------------------------------------------------ */
+#define DFR_INT32 1
+#define DFR_INT64 2
+#define DFR_FLOAT64 3
+/* A dictionary-encoded String column: uint32 codes into `dict_size`
+ * canonical labels (storage/string.mojo:35-89 StringCodes; equal labels must
+ * share one code, kernels/canonical_labels.mojo:13-47). Labels stay on the
+ * host; results carry codes. */
+#define DFR_DICT 4
+
+/* ---- devices (no HIP context is created) -------------------------------- */
+typedef struct dfr_device_info {
+    int32_t ordinal;
+    int32_t compute_units;
+    int64_t memory_bytes;  /* HBM size of the device */
+    char name[64];         /* marketing name, e.g. "AMD Instinct MI355X" */
+    char arch[32];         /* e.g. "gfx950" */
+} dfr_device_info;
+
+/* ABI version of the loaded library (DFR_ABI_VERSION it was built with). */
+int dfr_abi_version(void);
+
+/* Frees an error string or any other buffer this library malloc'd. */
+void dfr_free(void *p);
+
+/* Number of GPU agents, read from the KFD topology in sysfs. Opens no HIP
+ * runtime and creates no context, as the provider's installed()/devices()
+ * may not (accelerator-contract.md:27; devices.mojo:24-40). */
+int dfr_device_count(int32_t *count, char **error);
+int dfr_device_info_get(int32_t ordinal, dfr_device_info *info, char **error);
+
+/* ---- context: one device, one stream, one HBM budget -------------------- */
+typedef struct dfr_context dfr_context;
+
+/* budget_bytes <= 0 means "the device's free memory". The budget bounds
+ * what the context allocates (DATAFRAME_ACCEL_MEMORY_LIMIT semantics,
+ * dataframe_metal/metal.mojo:36-46; query_scope.mojo:17-29). */
+int dfr_context_create(int32_t device, int64_t budget_bytes, dfr_context **out,
+                       char **error);
+void dfr_context_destroy(dfr_context *ctx);
+
+/* Execution observation, as dataframe/execution_observation.mojo:23-94
+ * records it for a device route. */
+typedef struct dfr_observation {
+    int64_t launches;          /* kernels launched */
+    int64_t completed;         /* launches known complete (after a sync) */
+    int64_t upload_bytes;      /* host -> device */
+    int64_t download_bytes;    /* device -> host */
+    int64_t reserved_bytes;    /* device bytes currently held */
+    int64_t peak_reserved_bytes;
+    int64_t waits;             /* host waits on the stream */
+    int64_t queries;           /* group-by queries executed */
+} dfr_observation;
+int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
+
+/* Sets one launch or table parameter of the context (no counterpart in the
+ * reference: the autotuner's knob). Keys: "block", "wg_per_cu",
+ * "lds_bytes", "lds_probe", "lds_mode" (-1 auto, 0 none, 1 direct,
+ * 2 hashed), "direct_max_slots", "hash_probe", "hash_slots". */
+int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
+
+/* ---- device columns (the column hand-off) ------------------------------ */
+typedef struct dfr_column dfr_column;
+
+/* Copies `length` values starting at element `offset` of `values` to the
+ * device. `validity` is an LSB-first bitmap (Arrow order, as
+ * storage/column.mojo:1-34 and storage/bitmap.mojo:6-48), where bit
+ * `bit_offset + i` is row i; NULL means every row is valid. `dict_size` is
+ * the number of labels of a DFR_DICT column (ignored otherwise). Column
+ * statistics (min/max, float exponent range, null count) are computed on the
+ * device at upload. */
+int dfr_column_upload(dfr_context *ctx, int32_t dtype, const void *values,
+                      int64_t offset, int64_t length, const uint8_t *validity,
+                      int64_t bit_offset, int64_t dict_size, dfr_column **out,
+                      char **error);
+
+/* Wraps device memory the caller owns (zero copy): `device_values` and
+ * `device_validity` (64-bit LSB-first words, may be NULL) must stay valid
+ * until dfr_column_release. Statistics are computed on the device once. */
+int dfr_column_wrap(dfr_context *ctx, int32_t dtype, const void *device_values,
+                    int64_t offset, int64_t length,
+                    const uint64_t *device_validity, int64_t bit_offset,
+                    int64_t dict_size, dfr_column **out, char **error);
+
+void dfr_column_release(dfr_column *column);
+int64_t dfr_column_length(const dfr_column *column);
+int64_t dfr_column_null_count(const dfr_column *column);
+
+/* ---- the query ---------------------------------------------------------- */
+/* Predicate leaves: `column <op> literal`, or `floor_mod(column, modulus)
+ * <op> literal` when DFR_LEAF_MOD is set, or a null test. A comparison with
+ * a null value is not true (a null is neither true nor false, so it stays
+ * not true under DFR_LEAF_NOT as well). Float comparisons are IEEE (NaN
+ * compares false, NaN != NaN is true; docs/expressions.md:93-94). The
+ * modulo takes the divisor's sign and modulo zero is null
+ * (docs/expressions.md:91). `literal` holds an int64 value, or the bits of
+ * a double when the column is DFR_FLOAT64. */
+#define DFR_EQ 0
+#define DFR_NE 1
+#define DFR_LT 2
+#define DFR_LE 3
+#define DFR_GT 4
+#define DFR_GE 5
+#define DFR_IS_NULL 6
+#define DFR_IS_NOT_NULL 7
+#define DFR_LEAF_MOD 1 /* leaf flag: compare floor_mod(column, modulus) */
+#define DFR_LEAF_NOT 2 /* leaf flag: Kleene NOT of the comparison */
+#define DFR_MAX_LEAVES 8
+
+typedef struct dfr_leaf {
+    int32_t column;  /* index into dfr_query.columns */
+    int32_t op;      /* DFR_EQ .. DFR_IS_NOT_NULL */
+    int32_t flags;   /* DFR_LEAF_MOD */
+    int32_t pad;
+    int64_t literal; /* int64 value or double bits */
+    int64_t modulus; /* for DFR_LEAF_MOD; modulo by zero is null */
+} dfr_leaf;
+
+/* A row is kept when bit m of the truth table is set, m being the leaves'
+ * is-true bits (leaf i is bit i). With no leaves every row is kept. NOT is
+ * pushed into the leaves by the host compiler, so AND/OR of is-true bits
+ * is SQL filter semantics even with nulls
+ * (dataframe_metal/predicate_program.mojo:1-71). */
+typedef struct dfr_predicate {
+    int32_t n_leaves;
+    int32_t pad;
+    uint32_t truth[8]; /* 256 bits */
+    dfr_leaf leaves[DFR_MAX_LEAVES];
+} dfr_predicate;
+
+/* Aggregates (expr.mojo:2869-3006). */
+#define DFR_AGG_LEN 0   /* rows in the group, Int64 */
+#define DFR_AGG_COUNT 1 /* non-null values, Int64 */
+#define DFR_AGG_SUM 2   /* Int64 for Int64, Int32 for Int32, Float64 */
+#define DFR_AGG_MIN 3   /* input type */
+#define DFR_AGG_MAX 4   /* input type */
+#define DFR_AGG_MEAN 5  /* Float64 */
+#define DFR_AGG_VAR 6   /* Float64, ddof */
+#define DFR_AGG_STD 7   /* Float64, ddof */
+
+typedef struct dfr_agg {
+    int32_t op;
+    int32_t column;    /* index into dfr_query.columns (ignored for LEN) */
+    int32_t ddof;      /* VAR/STD; the reference's default is 1 */
+    int32_t pad;
+    int64_t min_count; /* SUM: null when fewer valid values (default 0) */
+} dfr_agg;
+
+#define DFR_MAX_KEYS 4
+#define DFR_MAX_COLUMNS 12
+
+/* filter(predicate).group_by(keys, maintain_order).agg(aggs) over columns of
+ * one frame (lazy_api.mojo:418-470,3223). Keys and columns all have the same
+ * length. A key may also appear among the columns. */
+typedef struct dfr_query {
+    int32_t n_keys;
+    int32_t n_columns;
+    dfr_column *const *keys;
+    dfr_column *const *columns;
+    dfr_predicate predicate;
+    int32_t n_aggs;
+    int32_t maintain_order; /* groups in order of their first kept row */
+    const dfr_agg *aggs;
+} dfr_query;
+
+/* ---- results (device resident until copied) ---------------------------- */
+typedef struct dfr_result dfr_result;
+
+/* Runs one query. On DFR_OK *out holds the result, whose buffers stay on
+ * the device (storage/residence.mojo:33-80) until copied out. */
+int dfr_groupby(dfr_context *ctx, const dfr_query *query, dfr_result **out,
+                char **error);
+
+int64_t dfr_result_groups(const dfr_result *result);
+/* DFR_INT32 / DFR_INT64 / DFR_FLOAT64 / DFR_DICT of key k or aggregate a. */
+int32_t dfr_result_key_dtype(const dfr_result *result, int32_t k);
+int32_t dfr_result_agg_dtype(const dfr_result *result, int32_t a);
+
+/* Copies key k (or aggregate a) to host memory: `values` receives groups
+ * values of the column's width (4 bytes for DFR_INT32/DFR_DICT, else 8);
+ * `validity` (may be NULL) receives ceil(groups/8) bytes of LSB-first bits.
+ * Row order is the result's group order. */
+int dfr_result_copy_key(const dfr_result *result, int32_t k, void *values,
+                        uint8_t *validity, char **error);
+int dfr_result_copy_agg(const dfr_result *result, int32_t a, void *values,
+                        uint8_t *validity, char **error);
+/* Each group's first kept row (its ordering key under maintain_order). */
+int dfr_result_copy_first_rows(const dfr_result *result, int64_t *rows,
+                               char **error);
+/* Device bytes the result holds, and release of them
+ * (the _CopyToHost/_ReleaseDevice pair of storage/residence.mojo:21-22). */
+int64_t dfr_result_device_bytes(const dfr_result *result);
+void dfr_result_release(dfr_result *result);
+
+/* Milliseconds the device spent on the query's kernels (HIP events on the
+ * context's stream). */
+double dfr_result_device_ms(const dfr_result *result);
+
+/* Milliseconds of one phase of the query, from HIP events recorded around
+ * it on the context's stream: DFR_PHASE_INIT (group-table set-up),
+ * DFR_PHASE_SCAN (the fused filter + aggregate scan), DFR_PHASE_ORDER
+ * (compaction and ordering), DFR_PHASE_FINISH (keys and results). */
+#define DFR_PHASE_INIT 0
+#define DFR_PHASE_SCAN 1
+#define DFR_PHASE_ORDER 2
+#define DFR_PHASE_FINISH 3
+double dfr_result_phase_ms(const dfr_result *result, int32_t phase);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DFROCM_H */

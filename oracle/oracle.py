@@ -1,0 +1,256 @@
+"""TEST INFRASTRUCTURE ONLY — the CPU oracle for the filtered group-by path.
+
+Loaded only by tests/, __graft_entry__.smoke() and bench.py's CPU baseline,
+as the checker; the product (dataframe_mojo_amd) never imports it.
+
+Two independent halves:
+  * `evaluate` restates the reference's filter semantics in numpy: Kleene
+    AND/OR/NOT, comparisons with null are null, IEEE float comparisons,
+    `%` with the divisor's sign and modulo-by-zero null
+    (docs/semantics.md:207-209; docs/expressions.md:91-94). It walks the
+    expression tree directly and shares nothing with the product's
+    predicate compiler.
+  * `dfr_oracle.c` (libdforacle.so) restates the CPU sink's grouping,
+    per-row state updates and finish (file:line citations in its header).
+
+`collect(lazyframe)` returns the same DataFrame shape the provider returns.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from dataframe_mojo_amd.frame import FLOAT64, INT32, INT64, STRING, Column, DataFrame, Series
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdforacle.so")
+
+_CODES = {INT32: 1, INT64: 2, FLOAT64: 3, STRING: 4}
+_AGG = {"len": 0, "count": 1, "sum": 2, "min": 3, "max": 4, "mean": 5, "var": 6, "std": 7}
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class _Col(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int32), ("pad", ctypes.c_int32), ("values", ctypes.c_void_p), ("valid", ctypes.c_void_p)]
+
+
+class _Agg(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int32), ("col", ctypes.c_int32), ("ddof", ctypes.c_int32), ("pad", ctypes.c_int32), ("min_count", ctypes.c_int64)]
+
+
+_lib = None
+
+
+def build() -> str:
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64 = ctypes.c_void_p, ctypes.c_int64
+        L.dfo_groupby.argtypes = [i64, ctypes.c_int, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]
+        L.dfo_groupby.restype = ctypes.c_int
+        L.dfo_dd_moments.argtypes = [i64, vp, vp, vp, i64, ctypes.c_int, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+# ------------------------------------------------------------ filter semantics
+
+
+def _column_eval(frame, name):
+    c = frame.column(name).column
+    if c.dtype == STRING:
+        labels = np.array(c.labels + [""], dtype=object)
+        vals = labels[np.minimum(c.values.astype(np.int64), len(c.labels))]
+        return vals, c.is_valid().copy(), STRING
+    return c.values, c.is_valid().copy(), c.dtype
+
+
+def evaluate(frame, e):
+    """(values, valid, dtype) of an expression over every row of `frame`."""
+    n = frame.height()
+    k = e.kind
+    if k == "col":
+        return _column_eval(frame, e.args[0])
+    if k == "lit":
+        v = e.args[0]
+        if v is None:
+            return np.zeros(n, dtype=np.int64), np.zeros(n, dtype=bool), INT64
+        if isinstance(v, bool):
+            return np.full(n, v), np.ones(n, dtype=bool), "bool"
+        if isinstance(v, str):
+            return np.full(n, v, dtype=object), np.ones(n, dtype=bool), STRING
+        if isinstance(v, float):
+            return np.full(n, v, dtype=np.float64), np.ones(n, dtype=bool), FLOAT64
+        return np.full(n, v, dtype=np.int64), np.ones(n, dtype=bool), INT64
+    if k == "cmp":
+        op, a, b = e.args
+        av, am, at = evaluate(frame, a)
+        bv, bm, bt = evaluate(frame, b)
+        if FLOAT64 in (at, bt):
+            av, bv = av.astype(np.float64), bv.astype(np.float64)
+        with np.errstate(invalid="ignore"):
+            f = {"eq": np.equal, "ne": np.not_equal, "lt": np.less, "le": np.less_equal, "gt": np.greater, "ge": np.greater_equal}[op]
+            v = np.asarray(f(av, bv), dtype=bool)
+        return v, am & bm, "bool"
+    if k == "mod":
+        av, am, _ = evaluate(frame, e.args[0])
+        bv, bm, _ = evaluate(frame, e.args[1])
+        av = av.astype(np.int64)
+        bv = bv.astype(np.int64)
+        zero = bv == 0
+        safe = np.where(zero, 1, bv)
+        with np.errstate(all="ignore"):
+            r = np.remainder(av, safe)  # takes the divisor's sign
+        return r, am & bm & ~zero, INT64
+    if k in ("and", "or"):
+        av, am, _ = evaluate(frame, e.args[0])
+        bv, bm, _ = evaluate(frame, e.args[1])
+        a_t, a_f = am & av.astype(bool), am & ~av.astype(bool)
+        b_t, b_f = bm & bv.astype(bool), bm & ~bv.astype(bool)
+        if k == "and":
+            t, f = a_t & b_t, a_f | b_f
+        else:
+            t, f = a_t | b_t, a_f & b_f
+        return t, t | f, "bool"
+    if k == "not":
+        av, am, _ = evaluate(frame, e.args[0])
+        return ~av.astype(bool), am, "bool"
+    if k == "is_null":
+        _, am, _ = evaluate(frame, e.args[0])
+        return ~am, np.ones(n, dtype=bool), "bool"
+    if k == "is_not_null":
+        _, am, _ = evaluate(frame, e.args[0])
+        return am.copy(), np.ones(n, dtype=bool), "bool"
+    raise OracleError(f"oracle cannot evaluate {e!r}")
+
+
+def kept_rows(frame, filters) -> np.ndarray:
+    keep = np.ones(frame.height(), dtype=bool)
+    for f in filters:
+        v, m, _ = evaluate(frame, f)
+        keep &= m & v.astype(bool)
+    return keep
+
+
+# ------------------------------------------------------------ grouping
+
+
+def _ocol(column: Column, hold: list) -> _Col:
+    vals = np.ascontiguousarray(column.values)
+    valid = None if column.valid is None else np.ascontiguousarray(column.valid.astype(np.uint8))
+    hold.extend([vals, valid])
+    c = _Col()
+    c.dtype = _CODES[column.dtype]
+    c.values = vals.ctypes.data
+    c.valid = valid.ctypes.data if valid is not None else None
+    return c
+
+
+def _out_dtype(op: str, dt: str) -> str:
+    if op in ("len", "count"):
+        return INT64
+    if op in ("mean", "var", "std"):
+        return FLOAT64
+    return dt
+
+
+def groupby(frame, keep: np.ndarray, keys: list, aggs: list, maintain_order: bool = True):
+    """Runs the C oracle. `aggs`: (op, column name or None, ddof, min_count).
+    Returns (DataFrame pieces as (name, Column) lists, first_rows)."""
+    n = frame.height()
+    hold: list = []
+    kcols = (_Col * max(1, len(keys)))(*[_ocol(frame.column(k).column, hold) for k in keys])
+    names = []
+    for _, c, _, _ in aggs:
+        if c is not None and c not in names:
+            names.append(c)
+    vcols = (_Col * max(1, len(names)))(*[_ocol(frame.column(c).column, hold) for c in names])
+    oaggs = (_Agg * max(1, len(aggs)))()
+    for i, (op, c, ddof, mc) in enumerate(aggs):
+        oaggs[i].op = _AGG[op]
+        oaggs[i].col = names.index(c) if c is not None else 0
+        oaggs[i].ddof = ddof
+        oaggs[i].min_count = mc
+    keep8 = np.ascontiguousarray(keep.astype(np.uint8))
+    cap = max(n, 1)
+    groups = ctypes.c_int64(0)
+    first = np.empty(cap, dtype=np.int64)
+    kv = np.empty(max(1, len(keys)) * cap, dtype=np.int64)
+    kval = np.empty(max(1, len(keys)) * cap, dtype=np.uint8)
+    ab = np.empty(max(1, len(aggs)) * cap, dtype=np.uint64)
+    aval = np.empty(max(1, len(aggs)) * cap, dtype=np.uint8)
+    err = ctypes.create_string_buffer(256)
+    rc = lib().dfo_groupby(
+        n, len(keys), ctypes.addressof(kcols), keep8.ctypes.data, len(names), ctypes.addressof(vcols), len(aggs),
+        ctypes.addressof(oaggs), ctypes.addressof(groups), first.ctypes.data, kv.ctypes.data, kval.ctypes.data,
+        ab.ctypes.data, aval.ctypes.data, err, len(err),
+    )
+    if rc != 0:
+        raise OracleError(err.value.decode())
+    G = groups.value
+    out = []
+    for i, k in enumerate(keys):
+        src = frame.column(k).column
+        vals = kv[i * cap : i * cap + G]
+        valid = kval[i * cap : i * cap + G].astype(bool)
+        if src.dtype == STRING:
+            out.append((k, Column(STRING, vals.astype(np.uint32), valid, src.labels)))
+        else:
+            out.append((k, Column(src.dtype, vals.astype(np.int32 if src.dtype == INT32 else np.int64), valid)))
+    agg_cols = []
+    for i, (op, c, _, _) in enumerate(aggs):
+        dt = _out_dtype(op, frame.column(c).column.dtype if c is not None else INT64)
+        bits = ab[i * cap : i * cap + G]
+        valid = aval[i * cap : i * cap + G].astype(bool)
+        if dt == FLOAT64:
+            vals = bits.view(np.float64).copy()
+        elif dt == INT32:
+            vals = bits.view(np.int64).astype(np.int32)
+        else:
+            vals = bits.view(np.int64).copy()
+        agg_cols.append(Column(dt, vals, valid))
+    return out, agg_cols, first[:G].copy()
+
+
+def collect(plan):
+    """The oracle's answer for a LazyFrame: filter -> group_by -> agg."""
+    frame = plan.source
+    keep = kept_rows(frame, plan.filters)
+    specs, names = [], []
+    for e in plan.aggs:
+        op, target, kw = e.args
+        kw = dict(kw)
+        specs.append((op, None if target is None else target.args[0], int(kw.get("ddof", 1)), int(kw.get("min_count", 0))))
+        names.append(e.output_name())
+    key_cols, agg_cols, first = groupby(frame, keep, list(plan.keys), specs, plan.maintain_order)
+    series = [Series(k, c) for k, c in key_cols] + [Series(nm, c) for nm, c in zip(names, agg_cols)]
+    return DataFrame(series), first
+
+
+def dd_moments(gid: np.ndarray, x: np.ndarray, valid, G: int, ddof: int = 1):
+    """Double-double per-group sum (hi + lo), count and two-pass variance."""
+    n = len(x)
+    gid = np.ascontiguousarray(gid, dtype=np.int64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    v8 = None if valid is None else np.ascontiguousarray(np.asarray(valid, dtype=np.uint8))
+    hi = np.empty(max(G, 1))
+    lo = np.empty(max(G, 1))
+    cnt = np.empty(max(G, 1), dtype=np.int64)
+    var = np.empty(max(G, 1))
+    lib().dfo_dd_moments(n, gid.ctypes.data, x.ctypes.data, v8.ctypes.data if v8 is not None else None, G, ddof,
+                         hi.ctypes.data, lo.ctypes.data, cnt.ctypes.data, var.ctypes.data)
+    return hi[:G], lo[:G], cnt[:G], var[:G]

@@ -1,0 +1,257 @@
+"""Parity of the device path (libdfrocm through the provider boundary) with
+the CPU oracle on the same seeded inputs.
+
+Bar: keys, group order (under maintain_order), nulls, error texts and every
+integer aggregate bit-exact; Float64 sum/mean/var/std within
+|a - e| <= 1e-12 * max(1, |e|) (the reference's assert_close,
+tests/support/aggregate_table_reference.mojo:292-297); Float64 min/max
+bit-exact, including the sign of zero.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from dataframe_mojo_amd import Column, DataFrame, Series, col, datagen, len_, provider, runtime
+from dataframe_mojo_amd import _lib
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+INT64_MAX = (1 << 63) - 1
+FTOL = 1e-12
+
+
+def close(a, e, tol=FTOL):
+    if a is None or e is None:
+        return a is None and e is None
+    if isinstance(e, float) or isinstance(a, float):
+        if math.isnan(e):
+            return math.isnan(a)
+        if math.isinf(e):
+            return a == e
+        return abs(a - e) <= tol * max(1.0, abs(e))
+    return a == e
+
+
+def assert_same(got, exp, ordered, exact_float=(), n_keys=1):
+    assert got.columns == exp.columns, (got.columns, exp.columns)
+    grows, erows = got.rows(), exp.rows()
+    assert len(grows) == len(erows), (len(grows), len(erows))
+    if not ordered:
+        def sk(r):
+            return tuple((v is None, 0 if v is None else v) for v in r[:n_keys])
+
+        grows, erows = sorted(grows, key=sk), sorted(erows, key=sk)
+    for gr, er in zip(grows, erows):
+        for name, a, b in zip(got.columns, gr, er):
+            if name in exact_float and isinstance(b, float):
+                assert (np.float64(a).tobytes() == np.float64(b).tobytes()) or (math.isnan(a) and math.isnan(b)), (name, a, b)
+            else:
+                assert close(a, b), (name, a, b, gr, er)
+
+
+def both(lf, exact_float=()):
+    got = lf.collect(engine="accel")
+    exp, _ = oracle.collect(lf)
+    assert_same(got, exp, lf.maintain_order, exact_float, len(lf.keys))
+    return got
+
+
+def test_readme_sales_example():
+    df = DataFrame([
+        Series("region", Column.string(["east", "west", "east", "west"])),
+        Series("amount", Column.int64([100, 80, -20, 120])),
+    ])
+    out = (df.filter(col("amount") > 0).group_by("region", maintain_order=True)
+           .agg(col("amount").sum().alias("gross_sales")).collect(engine="accel"))
+    assert out.to_dict() == {"region": ["east", "west"], "gross_sales": [100, 200]}
+
+
+def test_wide_sum_cancels_and_null_group():
+    df = DataFrame([
+        Series("k", Column.int64([0, 9, 0, 0, 9], [False, True, False, False, True])),
+        Series("v", Column.int64([INT64_MAX, 3, 7, -INT64_MAX, 3])),
+    ])
+    out = both(df.lazy().group_by("k", maintain_order=True).agg(
+        col("v").sum().alias("s"), col("v").count().alias("n"), col("v").mean().alias("avg"),
+        col("v").min().alias("lo"), col("v").max().alias("hi")))
+    d = out.to_dict()
+    assert d["k"] == [None, 9] and d["s"] == [7, 6] and d["avg"][0] == 7 / 3
+
+
+def test_groupby_1m_golden():
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "groupby_1m.json")) as f:
+        gold = json.load(f)
+    cols = datagen.groupby_1m(gold["rows"], gold["seed"])
+    df = DataFrame([Series("key", Column.int64(cols["key"])), Series("amount", Column.int64(cols["amount"]))])
+    out = df.filter(col("amount") > 0).group_by("key", maintain_order=True).agg(col("amount").sum(), len_()).collect()
+    d = out.to_dict()
+    assert d["key"] == gold["keys"] and d["amount"] == gold["sum"] and d["len"] == gold["len"]
+
+
+def test_int64_overflow_is_a_data_fault():
+    bad = DataFrame([Series("k", Column.int64([1, 1])), Series("v", Column.int64([INT64_MAX, 1]))])
+    with pytest.raises(_lib.DataFault, match="Int64 expression sum overflow"):
+        bad.lazy().group_by("k").agg(col("v").sum()).collect()
+    ok = DataFrame([Series("k", Column.int64([1, 1, 1])), Series("v", Column.int64([INT64_MAX, 1, -1]))])
+    assert ok.lazy().group_by("k").agg(col("v").sum()).collect().to_dict()["v"] == [INT64_MAX]
+    i32 = DataFrame([Series("k", Column.int64([1, 1])), Series("v", Column.int32([2**31 - 1, 1]))])
+    with pytest.raises(_lib.DataFault, match="int32 expression sum overflow"):
+        i32.lazy().group_by("k").agg(col("v").sum()).collect()
+
+
+def test_zero_ties_and_nan():
+    df = DataFrame([
+        Series("k", Column.int64([1, 1, 2, 2, 3, 3, 4, 5])),
+        Series("x", Column.float64([0.0, -0.0, -0.0, 0.0, math.nan, 1.5, math.nan, math.inf])),
+    ])
+    both(df.lazy().group_by("k", maintain_order=True).agg(
+        col("x").min().alias("mn"), col("x").max().alias("mx"), col("x").sum().alias("s"),
+        col("x").mean().alias("m"), col("x").var().alias("v")), exact_float=("mn", "mx"))
+
+
+def test_empty_and_all_filtered():
+    df = DataFrame([Series("k", Column.int64([1, 2, 3])), Series("v", Column.int64([-1, -2, -3]))])
+    out = df.filter(col("v") > 0).group_by("k").agg(col("v").sum()).collect()
+    assert out.height() == 0
+    empty = DataFrame([Series("k", Column.int64([])), Series("v", Column.int64([]))])
+    assert empty.lazy().group_by("k").agg(col("v").sum()).collect().height() == 0
+
+
+def random_frame(n, seed, key_card=50):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=n)
+    x[rng.random(n) < 0.01] = np.nan
+    x[rng.random(n) < 0.01] = 0.0
+    x[rng.random(n) < 0.005] = -0.0
+    labels = [f"lbl{i}" for i in range(17)] + ["lbl3"]
+    return DataFrame([
+        Series("k", Column.int64(rng.integers(-key_card, key_card, n), rng.random(n) > 0.03)),
+        Series("k32", Column.int32(rng.integers(0, 7, n))),
+        Series("s", Column.dictionary(rng.integers(0, len(labels), n), labels, rng.random(n) > 0.05)),
+        Series("a", Column.int64(rng.integers(-10**6, 10**6, n), rng.random(n) > 0.1)),
+        Series("b", Column.int32(rng.integers(-1000, 1000, n))),
+        Series("x", Column.float64(x, rng.random(n) > 0.05)),
+        Series("w", Column.int64(rng.integers(-(2**50), 2**50, n))),
+    ])
+
+
+ALL_AGGS = [
+    len_(), col("a").sum().alias("a_sum"), col("a").count().alias("a_cnt"), col("a").min().alias("a_min"),
+    col("a").max().alias("a_max"), col("a").mean().alias("a_mean"), col("a").var().alias("a_var"),
+    col("b").sum().alias("b_sum"), col("b").min().alias("b_min"), col("b").std(ddof=0).alias("b_std0"),
+    col("x").sum().alias("x_sum"), col("x").mean().alias("x_mean"), col("x").var().alias("x_var"),
+    col("x").std().alias("x_std"), col("x").min().alias("x_min"), col("x").max().alias("x_max"),
+    col("x").count().alias("x_cnt"), col("w").sum().alias("w_sum"), col("s").count().alias("s_cnt"),
+    col("a").sum(min_count=3).alias("a_sum3"),
+]
+
+FILTERS = [
+    [],
+    [col("a") > 0],
+    [(col("b") % 3 != 0) | col("x").is_null()],
+    [~((col("x") > 0.25) & (col("s") == "lbl3"))],
+    [col("a").is_not_null(), col("x") <= 1.0],
+]
+
+KEYS = [["k"], ["k32"], ["s"], ["k", "s"], ["k32", "k", "s"]]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("fi", range(len(FILTERS)))
+@pytest.mark.parametrize("ki", range(len(KEYS)))
+def test_random_queries(seed, fi, ki):
+    df = random_frame(20000 + 977 * seed, seed)
+    for mo in (True, False):
+        lf = df.lazy()
+        for f in FILTERS[fi]:
+            lf = lf.filter(f)
+        lf = lf.group_by(*KEYS[ki], maintain_order=mo).agg(ALL_AGGS)
+        both(lf, exact_float=("x_min", "x_max"))
+
+
+@pytest.mark.parametrize("setting", [
+    {"lds_mode": 0}, {"lds_mode": 2}, {"lds_mode": 2, "lds_bytes": 8192}, {"direct_max_slots": 0},
+    {"direct_max_slots": 0, "lds_mode": 0}, {"direct_max_slots": 0, "hash_slots": 64},
+    {"block": 256, "wg_per_cu": 1}, {"block": 1024, "wg_per_cu": 4}, {"lds_probe": 1},
+])
+def test_table_modes_give_identical_results(setting, gpu_ctx):
+    df = random_frame(30000, 5, key_card=400)
+    lf = df.filter(col("a") > -500000).group_by("k", "s", maintain_order=True).agg(ALL_AGGS)
+    base = lf.collect()
+    try:
+        for k, v in setting.items():
+            gpu_ctx.tune(k, v)
+        got = lf.collect()
+    finally:
+        for k, v in {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26, "hash_slots": 0,
+                     "block": 512, "wg_per_cu": 2, "lds_probe": 4}.items():
+            gpu_ctx.tune(k, v)
+    # bitwise identical: every state is order independent
+    for name in base.columns:
+        a, b = base.column(name).column, got.column(name).column
+        assert np.array_equal(a.values.view(np.uint8), b.values.view(np.uint8)), name
+        assert np.array_equal(a.is_valid(), b.is_valid()), name
+    exp, _ = oracle.collect(lf)
+    assert_same(got, exp, True, ("x_min", "x_max"), 2)
+
+
+def test_device_generator_matches_host_twin():
+    import torch
+
+    n = 100003
+    u = datagen.device_uniform(n, 11, 3, -1000, 1000).cpu().numpy()
+    assert np.array_equal(u, datagen.uniform(11, 3, n, -1000, 1000))
+    v = datagen.device_validity(n, 11, 4, 0.05).cpu().numpy().view(np.uint64)
+    bits = np.unpackbits(v.view(np.uint8), bitorder="little")[:n].astype(bool)
+    assert np.array_equal(bits, datagen.validity_bits(11, 4, n, 0.05))
+    z = datagen.device_zipf(n, 11, 5, 1.1, 1000).cpu().numpy()
+    assert np.array_equal(z, datagen.zipf(11, 5, n, datagen.zipf_cdf(1.1, 1000)))
+    torch.cuda.synchronize()
+
+
+def _zipf_frame_from_device(cols, n):
+    """Host copies of the first n rows of device workload columns."""
+    x_valid = np.unpackbits(cols["x.valid"].cpu().numpy().view(np.uint8), bitorder="little")[:n].astype(bool)
+    return DataFrame([
+        Series("key", Column.int32(cols["key"][:n].cpu().numpy())),
+        Series("a", Column.int64(cols["a"][:n].cpu().numpy())),
+        Series("x", Column.float64(cols["x"][:n].cpu().numpy(), x_valid)),
+        Series("y", Column.int64(cols["y"][:n].cpu().numpy())),
+    ])
+
+
+ZIPF_AGGS = [
+    col("y").sum().alias("y_sum"), col("y").count().alias("y_cnt"), col("y").min().alias("y_min"),
+    col("y").max().alias("y_max"), col("x").sum().alias("x_sum"), col("x").mean().alias("x_mean"),
+    col("x").var().alias("x_var"), col("x").min().alias("x_min"),
+]
+
+
+def test_zipf_workload_prefix_matches_oracle(gpu_ctx):
+    """The 200M-row Zipf workload's generator and query, checked against the
+    oracle on a 3M-row prefix of the device-resident columns (wrapped in
+    place, zero copy)."""
+    from dataframe_mojo_amd.plan import compile_plan
+    from dataframe_mojo_amd.runtime import result_frame
+
+    n_total, n = 4_000_000, 3_000_000
+    cols = datagen.zipf_200m_device(n_total, seed=0)
+    host = _zipf_frame_from_device(cols, n)
+    lf = host.filter(col("a") > 0).group_by("key", maintain_order=True).agg(ZIPF_AGGS)
+    dplan = compile_plan(lf)
+    dev = {
+        "key": gpu_ctx.wrap(_lib.DFR_INT32, cols["key"].data_ptr(), n, keepalive=cols["key"]),
+        "a": gpu_ctx.wrap(_lib.DFR_INT64, cols["a"].data_ptr(), n, keepalive=cols["a"]),
+        "x": gpu_ctx.wrap(_lib.DFR_FLOAT64, cols["x"].data_ptr(), n, cols["x.valid"].data_ptr(), keepalive=cols),
+        "y": gpu_ctx.wrap(_lib.DFR_INT64, cols["y"].data_ptr(), n, keepalive=cols["y"]),
+    }
+    res = gpu_ctx.groupby(dplan, [dev[k] for k in dplan.keys], [dev[c] for c in dplan.columns])
+    got = result_frame(res, dplan, [dev[k] for k in dplan.keys])
+    exp, first = oracle.collect(lf)
+    assert_same(got, exp, True, ("x_min",))
+    assert np.array_equal(res.first_rows(), first)
