@@ -83,6 +83,27 @@ __global__ void k_validity(uint64_t *words, uint64_t n, uint64_t base, uint64_t 
   }
 }
 
+// Streaming read of n 16-byte words (the achievable HBM read rate): each
+// thread folds what it reads into one xor written at the end.
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+__global__ void k_stream_read(const u64x2 *p, uint64_t n, uint64_t *sink) {
+  uint64_t acc = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    u64x2 v = __builtin_nontemporal_load(p + i);
+    acc ^= v.x ^ v.y;
+  }
+  if (acc == 0x9E3779B97F4A7C15ull) sink[0] = acc;  // keeps the loads live
+}
+
+// Streaming copy of n 16-byte words.
+__global__ void k_stream_copy(const ulonglong2 *src, ulonglong2 *dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 int launch_blocks(uint64_t n) {
   uint64_t b = (n + 255) / 256;
   return (int)(b < 65536 ? (b ? b : 1) : 65536);
@@ -97,6 +118,57 @@ int finish() {
 }  // namespace
 
 extern "C" {
+
+// Device buffers for generated columns (the benchmark and the tests own
+// them; the provider library only wraps them).
+int dfrg_alloc(void **out, uint64_t bytes) {
+  hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int dfrg_free(void *p) { return (int)hipFree(p); }
+
+int dfrg_copy_to_host(void *dst, const void *src, uint64_t bytes) {
+  return (int)hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+}
+
+int dfrg_copy_to_device(void *dst, const void *src, uint64_t bytes) {
+  return (int)hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+}
+
+int dfrg_fill_zero(void *dst, uint64_t bytes) {
+  hipError_t e = hipMemset(dst, 0, bytes);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  return (int)e;
+}
+
+// Milliseconds of a streaming read (copy when dst != NULL) of `bytes`
+// (a multiple of 16), timed with HIP events; blocks = workgroups of 256.
+double dfrg_stream(const void *src, void *dst, uint64_t bytes, int blocks, int reps) {
+  uint64_t n = bytes / 16;
+  uint64_t *sink = nullptr;
+  if (hipMalloc(&sink, 8) != hipSuccess) return -1.0;
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  float best = 1e30f;
+  for (int r = 0; r < reps; ++r) {
+    hipEventRecord(a, 0);
+    if (dst)
+      hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, 0, (const ulonglong2 *)src, (ulonglong2 *)dst, n);
+    else
+      hipLaunchKernelGGL(k_stream_read, dim3(blocks), dim3(256), 0, 0, (const u64x2 *)src, n, sink);
+    hipEventRecord(b, 0);
+    hipEventSynchronize(b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    if (ms < best) best = ms;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  hipFree(sink);
+  return best;
+}
 
 // Uniform integers in [lo, hi] (inclusive); width 4 or 8 bytes.
 int dfrg_uniform(void *out, int width, uint64_t n, uint64_t seed, uint64_t stream, int64_t lo, int64_t hi) {

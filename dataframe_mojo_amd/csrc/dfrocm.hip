@@ -276,6 +276,12 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.hash_probe = (int)std::min<int64_t>(value, 1 << 30);
   else if (k == "hash_slots" && value >= 0)
     t.hash_slots = value;
+  else if (k == "agg_mode" && (value == 0 || value == 1))
+    t.agg_mode = (int)value;
+  else if (k == "part_lds_bytes" && value >= 4096 && value <= ctx->lds_per_block)
+    t.part_lds_bytes = (int)value;
+  else if (k == "scan_unroll" && (value == 1 || value == 2 || value == 4))
+    t.scan_unroll = (int)value;
   else
     return dfr_fail(error, DFR_INVALID, "bad tuning %s=%lld", key, (long long)value);
   return DFR_OK;

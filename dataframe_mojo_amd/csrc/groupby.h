@@ -16,6 +16,7 @@ struct Compiled {
   int grid = 1;
   int64_t chunk = 0;
   size_t lds_bytes = 0;
+  size_t c_lds_bytes = 0;  // partition tables (pass C)
 };
 
 int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **error);

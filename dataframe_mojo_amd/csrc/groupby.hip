@@ -10,6 +10,7 @@
 #include "../../include/dfrocm.h"
 #include "groupby.h"
 #include "kernels.h"
+#include "kernels_part.h"
 #include "runtime.h"
 
 using namespace dfr;
@@ -53,6 +54,8 @@ Tuning default_tuning(dfr_context *ctx) {
   t.lds_bytes = std::min(env_int("DFR_LDS_BYTES", 65536), ctx->lds_per_block);
   t.lds_probe = env_int("DFR_LDS_PROBE", 4);
   t.lds_mode = env_int("DFR_LDS_MODE", -1);
+  t.agg_mode = env_int("DFR_AGG_MODE", AGG_PARTITION);
+  t.part_lds_bytes = std::min(env_int("DFR_PART_LDS_BYTES", 98304), ctx->lds_per_block);
   return t;
 }
 
@@ -300,8 +303,13 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
   int l2g[kMaxWords];
   for (int i = 0; i < w; ++i) l2g[i] = i;
   uint64_t ident[kMaxWords];
+  int8_t wop[kMaxWords];
   ident[0] = 0;
-  if (P.has_first) ident[1] = kEmpty;
+  wop[0] = W_ADD;
+  if (P.has_first) {
+    ident[1] = kEmpty;
+    wop[1] = W_MIN_U;
+  }
   for (int c = 0; c < P.n_cols; ++c) {
     DevColAgg &A = P.cagg[c];
     memset(&A, 0, sizeof A);
@@ -312,26 +320,27 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
     A.gbase = (int16_t)w;
     A.lbase = (int16_t)lw;
     int o = 0;
-    auto put = [&](int8_t &field, int n, uint64_t id, bool lds) {
+    auto put = [&](int8_t &field, int n, uint64_t id, bool lds, int8_t op) {
       field = (int8_t)o;
       for (int i = 0; i < n; ++i) {
         ident[w] = id;
+        wop[w] = (n == 2) ? (i == 0 ? W_ADD128 : W_HI) : op;
         if (lds) l2g[lw++] = w;
         ++w;
         ++o;
       }
     };
-    if (mask[c] & M_VCOUNT) put(A.o_count, 1, 0, true);
-    if (mask[c] & M_ISUM64) put(A.o_isum, 1, 0, true);
-    if (mask[c] & M_ISUM128) put(A.o_isum, 2, 0, true);
-    if (mask[c] & M_FSUM) put(A.o_fsum, 2, 0, true);
-    if (mask[c] & M_FSUMSQ) put(A.o_sq, 2, 0, true);
-    if (mask[c] & M_IMIN) put(A.o_min, 1, (uint64_t)INT64_MAX, true);
-    if (mask[c] & M_IMAX) put(A.o_max, 1, (uint64_t)INT64_MIN, true);
-    if (mask[c] & M_FMIN) put(A.o_min, 1, kEmpty, true);
-    if (mask[c] & M_FMAX) put(A.o_max, 1, 0, true);
-    if (mask[c] & M_FLAGS) put(A.o_flags, 1, 0, false);
-    if (mask[c] & M_ZFIRST) put(A.o_zfirst, 1, kEmpty, false);
+    if (mask[c] & M_VCOUNT) put(A.o_count, 1, 0, true, W_ADD);
+    if (mask[c] & M_ISUM64) put(A.o_isum, 1, 0, true, W_ADD);
+    if (mask[c] & M_ISUM128) put(A.o_isum, 2, 0, true, W_ADD128);
+    if (mask[c] & M_FSUM) put(A.o_fsum, 2, 0, true, W_ADD128);
+    if (mask[c] & M_FSUMSQ) put(A.o_sq, 2, 0, true, W_ADD128);
+    if (mask[c] & M_IMIN) put(A.o_min, 1, (uint64_t)INT64_MAX, true, W_MIN_S);
+    if (mask[c] & M_IMAX) put(A.o_max, 1, (uint64_t)INT64_MIN, true, W_MAX_S);
+    if (mask[c] & M_FMIN) put(A.o_min, 1, kEmpty, true, W_MIN_U);
+    if (mask[c] & M_FMAX) put(A.o_max, 1, 0, true, W_MAX_U);
+    if (mask[c] & M_FLAGS) put(A.o_flags, 1, 0, false, W_OR);
+    if (mask[c] & M_ZFIRST) put(A.o_zfirst, 1, kEmpty, false, W_MIN_U);
     // fixed-point scales: N values below 2^B each sum below 2^126
     if (mask[c] & (M_FSUM | M_FSUMSQ)) {
       dfr_column *col = C.cols[c];
@@ -352,9 +361,29 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
   }
   P.words = w;
   P.lds_words = lw;
-  for (int i = 0; i < w; ++i) P.ident[i] = ident[i];
+  for (int i = 0; i < w; ++i) {
+    P.ident[i] = ident[i];
+    P.wop[i] = wop[i];
+  }
   for (int i = 0; i < lw; ++i) P.lds_to_g[i] = (int8_t)l2g[i];
   C.maintain_order = q->maintain_order != 0;
+  // spilled row records: [key | vmask << key_bits] [meta] [row] [values]
+  P.n_vals = 0;
+  for (int c = 0; c < kMaxCols; ++c) P.val_of[c] = -1;
+  for (int c = 0; c < P.n_cols; ++c)
+    if (P.cagg[c].mask) P.val_of[c] = (int8_t)P.n_vals++;
+  P.key_bits = bitlen_u64(P.domain > 0 ? P.domain - 1 : 0);
+  P.vmask_in_key = (P.key_bits + P.n_vals <= 64) ? 1 : 0;
+  // the row index travels with a record when the state orders by row:
+  // FIRST (maintain_order) or a first-zero word (signed-zero ties)
+  bool need_row = P.has_first != 0;
+  for (int c = 0; c < P.n_cols; ++c)
+    if (P.cagg[c].mask & M_ZFIRST) need_row = true;
+  int rw = 1;
+  P.rec_meta = P.vmask_in_key ? -1 : rw++;
+  P.rec_row = need_row ? rw++ : -1;
+  P.rec_val0 = rw;
+  P.rw = rw + P.n_vals;
   return DFR_OK;
 }
 
@@ -406,9 +435,49 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots) {
     P.lds_entries = 0;
   }
   P.lds_probe = T.lds_probe;
+  P.agg_mode = T.agg_mode;
+  if (P.agg_mode == AGG_PARTITION) {
+    // partition tables: E entries of (key + W words) within part_lds_bytes
+    const int W = P.words;
+    uint64_t ce = 64;
+    while (ce * 2 * 8ull * (W + 1) <= (uint64_t)T.part_lds_bytes && ce < 16384) ce *= 2;
+    P.c_entries = (int32_t)ce;
+    P.c_log2 = log2_exact(ce);
+    const uint64_t groups_est = std::max<uint64_t>(1, std::min<uint64_t>(D, (uint64_t)std::max<int64_t>(rows, 1)));
+    uint64_t np = next_pow2((2 * groups_est + ce - 1) / ce);
+    np = std::min<uint64_t>(std::max<uint64_t>(np, 1), 4096);
+    P.n_parts = (int32_t)np;
+    P.part_log2 = log2_exact(np);
+    // the scan's LDS table (full state words) next to two histograms
+    const int hist_bytes = (2 * P.n_parts + 2) * 4;
+    const int avail = std::max(0, std::min(T.lds_bytes, ctx->lds_per_block) - hist_bytes);
+    const uint64_t direct_e = (uint64_t)avail / (8ull * W);
+    int m = T.lds_mode;
+    if (m < 0) m = (D <= direct_e) ? LDS_DIRECT : LDS_HASHED;
+    if (m == LDS_DIRECT && D > direct_e) m = LDS_HASHED;
+    if (m == LDS_DIRECT) {
+      P.lds_mode = LDS_DIRECT;
+      P.lds_entries = (int32_t)std::max<uint64_t>(D, 1);
+      P.lds_log2 = 1;
+    } else if (m == LDS_HASHED) {
+      uint64_t e = 1;
+      while (e * 2 * 8ull * (W + 1) <= (uint64_t)avail && e < 16384) e *= 2;
+      if (e < 64) {
+        P.lds_mode = LDS_NONE;
+        P.lds_entries = 0;
+      } else {
+        P.lds_mode = LDS_HASHED;
+        P.lds_entries = (int32_t)e;
+        P.lds_log2 = log2_exact(e);
+      }
+    } else {
+      P.lds_mode = LDS_NONE;
+      P.lds_entries = 0;
+    }
+  }
   // launch shape: persistent workgroups over contiguous row chunks
   C.block = std::max(64, std::min(1024, T.block));
-  int64_t step = 2 * (int64_t)C.block;
+  int64_t step = 8 * (int64_t)C.block;  // 2 rows x up to 4 pairs per thread
   int64_t blocks = (int64_t)ctx->cus * std::max(1, T.wg_per_cu);
   int64_t chunk = (rows + blocks - 1) / std::max<int64_t>(blocks, 1);
   chunk = std::max<int64_t>(step, (chunk + step - 1) / step * step);
@@ -417,6 +486,16 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots) {
   C.lds_bytes = P.lds_mode == LDS_NONE
                     ? 0
                     : (size_t)P.lds_entries * 8 * (LW + (P.lds_mode == LDS_HASHED ? 1 : 0));
+  if (P.agg_mode == AGG_PARTITION) {
+    P.n_segs = C.grid;
+    P.seg_rows = chunk;
+    P.seg_partials = std::max(P.lds_entries, 1);
+    const int W = P.words;
+    C.lds_bytes = (P.lds_mode == LDS_NONE ? 0
+                                          : (size_t)P.lds_entries * 8 * (W + (P.lds_mode == LDS_HASHED ? 1 : 0))) +
+                  (size_t)(2 * P.n_parts + 2) * 4;
+    C.c_lds_bytes = (size_t)P.c_entries * 8 * (W + 1);
+  }
 }
 
 // ------------------------------------------------------------ launch
@@ -466,6 +545,56 @@ int scan_bits(dfr_context *ctx, Scratch &S, const uint64_t *bits, uint64_t n, ui
   return DFR_OK;
 }
 
+template <class K>
+int set_lds(K kernel, size_t bytes, char **error) {
+  if (bytes > 65536)
+    HIP_TRY(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return DFR_OK;
+}
+
+template <int LDSM, int NC, int U>
+int launch_scan(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
+  int st;
+  if ((st = set_lds(k_scan_spill<LDSM, NC, U>, C.lds_bytes, error))) return st;
+  hipLaunchKernelGGL((k_scan_spill<LDSM, NC, U>), dim3(C.grid), dim3(C.block), C.lds_bytes, ctx->stream, dplan,
+                     C.chunk);
+  return DFR_OK;
+}
+
+template <int LDSM>
+int launch_scan_cols(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
+  const int nc = C.plan.n_cols;
+  if (nc <= 4 && ctx->tune.scan_unroll == 1) return launch_scan<LDSM, 4, 1>(ctx, C, dplan, error);
+  if (nc <= 4 && ctx->tune.scan_unroll == 2) return launch_scan<LDSM, 4, 2>(ctx, C, dplan, error);
+  if (nc <= 4) return launch_scan<LDSM, 4, 4>(ctx, C, dplan, error);
+  if (nc <= 8) return launch_scan<LDSM, 8, 2>(ctx, C, dplan, error);
+  return launch_scan<LDSM, kMaxCols, 1>(ctx, C, dplan, error);
+}
+
+// Passes A, B and C of the partitioned aggregation.
+int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
+  const DevPlan &P = C.plan;
+  int st;
+  switch (P.lds_mode) {
+    case LDS_DIRECT: st = launch_scan_cols<LDS_DIRECT>(ctx, C, dplan, error); break;
+    case LDS_HASHED: st = launch_scan_cols<LDS_HASHED>(ctx, C, dplan, error); break;
+    default: st = launch_scan_cols<LDS_NONE>(ctx, C, dplan, error);
+  }
+  if (st) return st;
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
+  hipLaunchKernelGGL(k_part_offsets, dim3(P.n_parts), dim3(kBlock), 0, ctx->stream, dplan);
+  hipLaunchKernelGGL(k_part_base, dim3(1), dim3(kBlock), 0, ctx->stream, dplan);
+  const size_t cur_bytes = (size_t)2 * P.n_parts * 4;
+  if ((st = set_lds(k_part_scatter, cur_bytes, error))) return st;
+  hipLaunchKernelGGL(k_part_scatter, dim3(P.n_segs), dim3(1024), cur_bytes, ctx->stream, dplan);
+  if ((st = set_lds(k_part_aggregate, C.c_lds_bytes, error))) return st;
+  hipLaunchKernelGGL(k_part_aggregate, dim3(P.n_parts), dim3(512), C.c_lds_bytes, ctx->stream, dplan);
+  ctx->obs.launches += 5;
+  HIP_TRY(hipGetLastError());
+  return DFR_OK;
+}
+
 int launch_aggregate(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
   const DevPlan &P = C.plan;
   dim3 grid(C.grid), block(C.block);
@@ -508,6 +637,17 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
       return st;
     if (P.mode == MODE_HASH && (st = S.get(P.cap, &P.keys_tab, error))) return st;
     P.keys_tab = P.mode == MODE_HASH ? P.keys_tab : nullptr;
+    if (P.agg_mode == AGG_PARTITION) {
+      const uint64_t S_ = (uint64_t)P.n_segs, NP = (uint64_t)P.n_parts, W = (uint64_t)P.words;
+      const uint64_t raw_words = S_ * (uint64_t)P.seg_rows * (uint64_t)P.rw;
+      const uint64_t par_words = S_ * (uint64_t)P.seg_partials * (1 + W);
+      if ((st = S.get(raw_words, &P.spill, error)) || (st = S.get(raw_words, &P.part_raw, error)) ||
+          (st = S.get(par_words, &P.partials, error)) || (st = S.get(par_words, &P.part_par, error)) ||
+          (st = S.get(S_ * NP, &P.hist_raw, error)) || (st = S.get(S_ * NP, &P.hist_par, error)) ||
+          (st = S.get(S_, &P.cnt_raw, error)) || (st = S.get(S_, &P.cnt_par, error)) ||
+          (st = S.get(NP + 1, &P.base_raw, error)) || (st = S.get(NP + 1, &P.base_par, error)))
+        return st;
+    }
     P.fault = fault;
     const uint64_t occ_words = (P.cap + 63) / 64;
     uint64_t *occ, *occ_prefix, *order;
@@ -525,8 +665,13 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
       ctx->obs.launches++;
     }
     HIP_TRY(hipEventRecord(ctx->evp[0], ctx->stream));
-    if (C.rows > 0 && (st = launch_aggregate(ctx, C, dplan, error))) return st;
-    HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
+    if (P.agg_mode == AGG_PARTITION) {
+      if (C.rows > 0 && (st = launch_partitioned(ctx, C, dplan, error))) return st;
+      if (C.rows == 0) HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
+    } else {
+      if (C.rows > 0 && (st = launch_aggregate(ctx, C, dplan, error))) return st;
+      HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
+    }
     {
       uint64_t threads = occ_words * 64;
       hipLaunchKernelGGL(k_occupancy, dim3((unsigned)((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0,
@@ -547,10 +692,12 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
     const uint64_t G = hs[0];
     const uint32_t *hf = (const uint32_t *)(hs + 1);
     if (hf[0]) {
-      if (P.mode != MODE_HASH || attempt >= 3)
+      // a full hash table: retry once at load <= 1/2 of every possible group
+      uint64_t need = next_pow2(std::max<uint64_t>(64, 2 * std::min<uint64_t>(P.domain, (uint64_t)C.rows)));
+      if (P.mode != MODE_HASH || attempt >= 4 || P.cap >= need)
         return dfr_fail(error, DFR_CAPACITY, "group table capacity exceeded");
-      hash_slots = (int64_t)P.cap * 4;
-      continue;  // retry with a larger table
+      hash_slots = (int64_t)std::max<uint64_t>(P.cap * 4, need);
+      continue;
     }
     P.groups = total;
     P.order = order;

@@ -27,8 +27,40 @@ namespace dfr {
 
 constexpr int kBlock = 256;
 
+// The plan is read through the constant address space: it does not change
+// during a launch, so uniform plan fields become scalar loads held in SGPRs.
+#define DFR_CONSTANT __attribute__((address_space(4)))
+typedef const DFR_CONSTANT DevPlan &PlanRef;
+__device__ __forceinline__ PlanRef plan_ref(const DevPlan *p) { return *(const DFR_CONSTANT DevPlan *)p; }
+
+// A register copy of one plan record (dword by dword, as scalar loads).
+template <class T>
+__device__ __forceinline__ T cget(const DFR_CONSTANT T &x) {
+  static_assert(sizeof(T) % 4 == 0, "plan records are dword multiples");
+  T out;
+  const DFR_CONSTANT uint32_t *src = (const DFR_CONSTANT uint32_t *)&x;
+  uint32_t *dst = (uint32_t *)&out;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
+  return out;
+}
+
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Column data, validity words and records live in device (global) memory:
+// reading them through the global address space keeps them off the flat
+// path, whose operations count against both the memory and the LDS
+// counters.
+#define DFR_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const DFR_GLOBAL T *gp(const T *p) {
+  return (const DFR_GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ DFR_GLOBAL T *gpw(T *p) {
+  return (DFR_GLOBAL T *)p;
+}
 
 // ---------------------------------------------------------------- helpers
 
@@ -42,12 +74,12 @@ __device__ __forceinline__ uint64_t atomic_load_u64(const uint64_t *p) {
 
 // Two rows' values of one column (rows r and r+1; `two` is false for the
 // last odd row of a range).
-__device__ __forceinline__ void load_pair(const DevCol &C, int64_t r, bool two,
+__device__ __forceinline__ void load_pair(const DevCol C, int64_t r, bool two,
                                           bool vec, uint64_t &a, uint64_t &b) {
   if (C.dtype == T_INT64 || C.dtype == T_FLOAT64) {
-    const uint64_t *p = (const uint64_t *)C.values + r;
+    const DFR_GLOBAL uint64_t *p = gp((const uint64_t *)C.values) + r;
     if (vec && two) {
-      u64x2 q = __builtin_nontemporal_load((const u64x2 *)p);
+      u64x2 q = __builtin_nontemporal_load((const DFR_GLOBAL u64x2 *)p);
       a = q.x;
       b = q.y;
     } else {
@@ -55,10 +87,10 @@ __device__ __forceinline__ void load_pair(const DevCol &C, int64_t r, bool two,
       b = two ? p[1] : 0;
     }
   } else {
-    const uint32_t *p = (const uint32_t *)C.values + r;
+    const DFR_GLOBAL uint32_t *p = gp((const uint32_t *)C.values) + r;
     uint32_t x, y;
     if (vec && two) {
-      u32x2 q = __builtin_nontemporal_load((const u32x2 *)p);
+      u32x2 q = __builtin_nontemporal_load((const DFR_GLOBAL u32x2 *)p);
       x = q.x;
       y = q.y;
     } else {
@@ -75,15 +107,16 @@ __device__ __forceinline__ void load_pair(const DevCol &C, int64_t r, bool two,
   }
 }
 
-__device__ __forceinline__ uint32_t load_valid_pair(const DevCol &C, int64_t r,
+__device__ __forceinline__ uint32_t load_valid_pair(const DevCol C, int64_t r,
                                                     bool two) {
   if (!C.valid) return 3u;
+  const DFR_GLOBAL uint64_t *vw = gp(C.valid);
   int64_t bit = C.vbit + r;
   int s = (int)(bit & 63);
-  uint64_t w = C.valid[bit >> 6];
+  uint64_t w = vw[bit >> 6];
   uint32_t v0 = (uint32_t)(w >> s) & 1u;
   uint32_t v1 = 0;
-  if (two) v1 = (s < 63) ? ((uint32_t)(w >> (s + 1)) & 1u) : ((uint32_t)C.valid[(bit >> 6) + 1] & 1u);
+  if (two) v1 = (s < 63) ? ((uint32_t)(w >> (s + 1)) & 1u) : ((uint32_t)vw[(bit >> 6) + 1] & 1u);
   return v0 | (v1 << 1);
 }
 
@@ -125,12 +158,12 @@ __device__ __forceinline__ bool leaf_true(const DevLeaf &L, bool is_float,
   return (L.flags & 2) ? !t : t;
 }
 
-__device__ __forceinline__ uint32_t truth_bit(const DevPlan &P, uint32_t m) {
+__device__ __forceinline__ uint32_t truth_bit(PlanRef P, uint32_t m) {
   return (P.truth[m >> 5] >> (m & 31)) & 1u;
 }
 
 // HBM slot of a packed key (inserting it in the hash table when new).
-__device__ __noinline__ uint64_t global_slot_hash(const DevPlan &P, uint64_t key) {
+__device__ __noinline__ uint64_t global_slot_hash(PlanRef P, uint64_t key) {
   uint64_t mask = P.cap - 1;
   uint64_t i = hash_index(key, P.cap_log2);
   for (int p = 0; p < P.max_probe; ++p) {
@@ -148,7 +181,7 @@ __device__ __noinline__ uint64_t global_slot_hash(const DevPlan &P, uint64_t key
   return kEmpty;
 }
 
-__device__ __forceinline__ uint64_t global_slot(const DevPlan &P, uint64_t key) {
+__device__ __forceinline__ uint64_t global_slot(PlanRef P, uint64_t key) {
   if (P.mode == MODE_DIRECT) return key;
   return global_slot_hash(P, key);
 }
@@ -161,7 +194,7 @@ __device__ __forceinline__ void add128(uint64_t *p, uint64_t lo, uint64_t hi) {
 }
 
 // The rare-event words (FLAGS, ZFIRST) live in HBM only.
-__device__ __noinline__ void rare_update(const DevPlan &P, uint64_t packed, int gword,
+__device__ __noinline__ void rare_update(PlanRef P, uint64_t packed, int gword,
                                          uint64_t value, bool is_min) {
   uint64_t g = global_slot(P, packed);
   if (g == kEmpty) return;
@@ -174,7 +207,7 @@ __device__ __noinline__ void rare_update(const DevPlan &P, uint64_t packed, int 
 
 // Folds one kept row into a group's words. `LDS` selects the ds_* form.
 template <bool LDS>
-__device__ __forceinline__ void fold_row(const DevPlan &P, uint64_t *st, int64_t row,
+__device__ __forceinline__ void fold_row(PlanRef P, uint64_t *st, int64_t row,
                                          const uint64_t *vals, uint32_t validbits,
                                          uint64_t packed) {
   atomicAdd((unsigned long long *)&st[0], 1ull);
@@ -184,7 +217,7 @@ __device__ __forceinline__ void fold_row(const DevPlan &P, uint64_t *st, int64_t
 #pragma unroll
   for (int c = 0; c < kMaxCols; ++c) {
     if (c >= P.n_cols) continue;
-    const DevColAgg &A = P.cagg[c];
+    const DevColAgg A = cget(P.cagg[c]);
     uint32_t m = A.mask;
     if (!m || !((validbits >> c) & 1u)) continue;
     uint64_t *b = st + (LDS ? A.lbase : A.gbase);
@@ -234,7 +267,7 @@ __device__ __forceinline__ void fold_row(const DevPlan &P, uint64_t *st, int64_t
 }
 
 template <int LDSM>
-__device__ __forceinline__ int lds_find(const DevPlan &P, uint64_t *lkeys, uint64_t key) {
+__device__ __forceinline__ int lds_find(PlanRef P, uint64_t *lkeys, uint64_t key) {
   if (LDSM == LDS_DIRECT) return (int)key;
   const int E = P.lds_entries;
   int h = (int)hash_index(key, P.lds_log2);
@@ -252,14 +285,14 @@ __device__ __forceinline__ int lds_find(const DevPlan &P, uint64_t *lkeys, uint6
 }
 
 template <int LDSM>
-__device__ __forceinline__ void fold_kept(const DevPlan &P, uint64_t *lkeys, uint64_t *lst,
+__device__ __forceinline__ void fold_kept(PlanRef P, uint64_t *lkeys, uint64_t *lst,
                                           int64_t row, const uint64_t *vals,
                                           uint32_t validbits) {
   uint64_t packed = 0;
 #pragma unroll
   for (int k = 0; k < kMaxKeys; ++k) {
     if (k >= P.n_keys) continue;
-    const DevKey &K = P.keys[k];
+    const DevKey K = cget(P.keys[k]);
     uint64_t digit = ((validbits >> k) & 1u) ? (vals[k] - (uint64_t)K.minv) : (K.radix - 1);
     packed += digit * K.stride;
   }
@@ -276,7 +309,7 @@ __device__ __forceinline__ void fold_kept(const DevPlan &P, uint64_t *lkeys, uin
 }
 
 template <int LDSM>
-__device__ __forceinline__ void process_pair(const DevPlan &P, uint64_t *lkeys, uint64_t *lst,
+__device__ __forceinline__ void process_pair(PlanRef P, uint64_t *lkeys, uint64_t *lst,
                                              int64_t r, int64_t end) {
   if (r >= end) return;
   const bool two = r + 1 < end;
@@ -285,8 +318,8 @@ __device__ __forceinline__ void process_pair(const DevPlan &P, uint64_t *lkeys, 
 #pragma unroll
   for (int c = 0; c < kMaxCols; ++c) {
     if (c >= P.n_cols) continue;
-    load_pair(P.cols[c], r, two, P.vec_ok, v0[c], v1[c]);
-    valid |= load_valid_pair(P.cols[c], r, two) << (2 * c);
+    load_pair(cget(P.cols[c]), r, two, P.vec_ok, v0[c], v1[c]);
+    valid |= load_valid_pair(cget(P.cols[c]), r, two) << (2 * c);
   }
   uint32_t va = 0, vb = 0;  // validity of row r / r+1, bit c
 #pragma unroll
@@ -302,7 +335,7 @@ __device__ __forceinline__ void process_pair(const DevPlan &P, uint64_t *lkeys, 
       if (c >= P.n_cols || !((P.leaf_cols >> c) & 1)) continue;
       const bool fl = P.cols[c].is_float;
       for (int l = P.leaf_begin[c]; l < P.leaf_begin[c + 1]; ++l) {
-        const DevLeaf &L = P.leaves[l];
+        const DevLeaf L = cget(P.leaves[l]);
         m0 |= (uint32_t)leaf_true(L, fl, v0[c], (va >> c) & 1u) << L.bit;
         m1 |= (uint32_t)leaf_true(L, fl, v1[c], (vb >> c) & 1u) << L.bit;
       }
@@ -314,14 +347,14 @@ __device__ __forceinline__ void process_pair(const DevPlan &P, uint64_t *lkeys, 
 }
 
 // Merges one LDS entry into its HBM slot.
-__device__ __forceinline__ void merge_entry(const DevPlan &P, const uint64_t *s, uint64_t key) {
+__device__ __forceinline__ void merge_entry(PlanRef P, const uint64_t *s, uint64_t key) {
   uint64_t g = global_slot(P, key);
   if (g == kEmpty) return;
   uint64_t *d = P.state + g * (uint64_t)P.words;
   atomicAdd((unsigned long long *)&d[0], (unsigned long long)s[0]);
   if (P.has_first) atomicMin((unsigned long long *)&d[1], (unsigned long long)s[1]);
   for (int c = 0; c < P.n_cols; ++c) {
-    const DevColAgg &A = P.cagg[c];
+    const DevColAgg A = cget(P.cagg[c]);
     uint32_t m = A.mask;
     if (!m) continue;
     const uint64_t *a = s + A.lbase;
@@ -347,7 +380,7 @@ __global__ void __launch_bounds__(1024) k_aggregate(const DevPlan *__restrict__ 
                                                     int64_t row_begin, int64_t row_end,
                                                     int64_t chunk) {
   extern __shared__ uint64_t smem[];
-  const DevPlan &P = *plan;
+  PlanRef P = plan_ref(plan);
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + (LDSM == LDS_HASHED ? P.lds_entries : 0);
   if (LDSM != LDS_NONE) {
@@ -377,7 +410,7 @@ __global__ void __launch_bounds__(1024) k_aggregate(const DevPlan *__restrict__ 
 // ------------------------------------------------------ table set-up
 
 __global__ void k_init_state(const DevPlan *__restrict__ plan) {
-  const DevPlan &P = *plan;
+  PlanRef P = plan_ref(plan);
   const uint32_t W = (uint32_t)P.words;
   // each block covers kBlock slots, coalesced over their words
   for (uint64_t s0 = (uint64_t)blockIdx.x * kBlock; s0 < P.cap; s0 += (uint64_t)gridDim.x * kBlock) {
@@ -392,7 +425,7 @@ __global__ void k_init_state(const DevPlan *__restrict__ plan) {
 // -------------------------------------------------- compaction / order
 
 __global__ void k_occupancy(const DevPlan *__restrict__ plan, uint64_t *bits, uint64_t nwords) {
-  const DevPlan &P = *plan;
+  PlanRef P = plan_ref(plan);
   uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool occ = s < P.cap && P.state[s * (uint64_t)P.words] != 0;
   uint64_t ball = __ballot(occ);
@@ -469,7 +502,7 @@ __device__ __forceinline__ uint64_t rank_of(const uint64_t *bits, const uint64_t
 
 __global__ void k_compact(const DevPlan *__restrict__ plan, const uint64_t *bits,
                           const uint64_t *prefix, uint64_t *order) {
-  const DevPlan &P = *plan;
+  PlanRef P = plan_ref(plan);
   uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.cap) return;
   if ((bits[s >> 6] >> (s & 63)) & 1) order[rank_of(bits, prefix, s)] = s;
@@ -477,7 +510,7 @@ __global__ void k_compact(const DevPlan *__restrict__ plan, const uint64_t *bits
 
 __global__ void k_mark_first(const DevPlan *__restrict__ plan, const uint64_t *order,
                              uint64_t groups, uint64_t *rowbits) {
-  const DevPlan &P = *plan;
+  PlanRef P = plan_ref(plan);
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= groups) return;
   uint64_t r = P.state[order[i] * (uint64_t)P.words + 1];
@@ -487,7 +520,7 @@ __global__ void k_mark_first(const DevPlan *__restrict__ plan, const uint64_t *o
 __global__ void k_rank_first(const DevPlan *__restrict__ plan, const uint64_t *order_in,
                              uint64_t groups, const uint64_t *rowbits, const uint64_t *rowprefix,
                              uint64_t *order_out) {
-  const DevPlan &P = *plan;
+  PlanRef P = plan_ref(plan);
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= groups) return;
   uint64_t s = order_in[i];
@@ -529,14 +562,14 @@ __device__ __forceinline__ double finish_var(const DevColAgg &A, const uint64_t 
   return num / den;
 }
 
-__device__ AggOut finish_agg(const DevPlan &P, const DevAgg &G, const uint64_t *slot) {
+__device__ AggOut finish_agg(PlanRef P, const DevAgg &G, const uint64_t *slot) {
   AggOut o = {0, true};
   uint64_t rows = slot[0];
   if (G.op == 0) {  // LEN
     o.bits = rows;
     return o;
   }
-  const DevColAgg &A = P.cagg[G.col];
+  const DevColAgg A = cget(P.cagg[G.col]);
   const uint64_t *b = slot + A.gbase;
   uint64_t n = (A.mask & M_VCOUNT) ? b[A.o_count] : rows;
   uint64_t flags = (A.mask & M_FLAGS) ? b[A.o_flags] : 0;
@@ -621,7 +654,7 @@ __device__ __forceinline__ void store_value(void *dst, int dtype, uint64_t i, ui
 }
 
 __global__ void k_finalize(const DevPlan *__restrict__ plan) {
-  const DevPlan &P = *plan;
+  PlanRef P = plan_ref(plan);
   const uint64_t G = *P.groups;
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < G;
@@ -633,7 +666,7 @@ __global__ void k_finalize(const DevPlan *__restrict__ plan) {
   if (active) packed = (P.mode == MODE_DIRECT) ? s : P.keys_tab[s];
   if (active && P.out_first) P.out_first[i] = slot[1];
   for (int k = 0; k < P.n_keys; ++k) {
-    const DevKey &K = P.keys[k];
+    const DevKey K = cget(P.keys[k]);
     uint64_t digit = (packed / K.stride) % K.radix;
     bool valid = !(K.nullable && digit == K.radix - 1);
     uint64_t v = valid ? (uint64_t)K.minv + digit : 0;
@@ -642,7 +675,7 @@ __global__ void k_finalize(const DevPlan *__restrict__ plan) {
     if (writer) P.out_key_valid[k][i >> 6] = ball;
   }
   for (int a = 0; a < P.n_aggs; ++a) {
-    const DevAgg &Ga = P.aggs[a];
+    const DevAgg Ga = cget(P.aggs[a]);
     AggOut o = {0, false};
     if (active) o = finish_agg(P, Ga, slot);
     if (active) store_value(Ga.out_values, Ga.out_dtype, i, o.bits);

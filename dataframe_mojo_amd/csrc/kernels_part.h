@@ -1,0 +1,448 @@
+// kernels_part.h — partitioned aggregation: no device-scope atomics in HBM
+// on the per-row path.
+//
+//   pass A  k_scan_spill      fused scan; each workgroup folds kept rows into
+//                             a first-come LDS table and spills the rows that
+//                             miss it as compact records into its own
+//                             segment; at the end the LDS table leaves as
+//                             partial-state records. LDS histograms count the
+//                             records of each key partition.
+//   pass B  k_part_offsets,   per-(segment, partition) offsets (scan over
+//           k_part_base,      segments, then over partitions), and a scatter
+//           k_part_scatter    of every record into partition order
+//   pass C  k_part_aggregate  one workgroup per partition folds its raw
+//                             records and merges its partial states in an LDS
+//                             table, then stores each group's final state
+//                             into the HBM group table with plain stores
+//
+// Every state word merges associatively and commutatively (integer adds,
+// min/max, or), so the result does not depend on which workgroup saw which
+// row or in what order.
+#pragma once
+#include "kernels.h"
+
+namespace dfr {
+
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+
+__device__ __forceinline__ uint32_t part_of(PlanRef P, uint64_t key) {
+  return P.part_log2 ? (uint32_t)((key * kGolden) >> (64 - P.part_log2)) : 0u;
+}
+
+// LDS index inside a partition's table: the hash bits below the partition's
+__device__ __forceinline__ int part_slot(PlanRef P, uint64_t key) {
+  uint64_t h = (key * kGolden) << P.part_log2;
+  return (int)(h >> (64 - P.c_log2));
+}
+
+// Folds one kept row into a full-layout state (every word, FLAGS and ZFIRST
+// included). `val(c)` and `ok(c)` give column c's value and validity.
+template <int NC = kMaxCols, class Val, class Ok>
+__device__ __forceinline__ void fold_full(PlanRef P, uint64_t *st, int64_t row, Val val, Ok ok) {
+  atomicAdd((unsigned long long *)&st[0], 1ull);
+  if (P.has_first) {
+    if ((uint64_t)row < st[1]) atomicMin((unsigned long long *)&st[1], (unsigned long long)row);
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (c >= P.n_cols) continue;
+    const DevColAgg A = cget(P.cagg[c]);
+    const uint32_t m = A.mask;
+    if (!m || !ok(c)) continue;
+    uint64_t *b = st + A.gbase;
+    const uint64_t v = val(c);
+    if (m & M_VCOUNT) atomicAdd((unsigned long long *)&b[A.o_count], 1ull);
+    if (m & M_ISUM64) atomicAdd((unsigned long long *)&b[A.o_isum], (unsigned long long)v);
+    if (m & M_ISUM128) add128(&b[A.o_isum], v, ((int64_t)v < 0) ? ~0ull : 0ull);
+    if (m & M_IMIN) {
+      if ((int64_t)v < (int64_t)b[A.o_min]) atomicMin((long long *)&b[A.o_min], (long long)v);
+    }
+    if (m & M_IMAX) {
+      if ((int64_t)v > (int64_t)b[A.o_max]) atomicMax((long long *)&b[A.o_max], (long long)v);
+    }
+    if (m & (M_FSUM | M_FSUMSQ | M_FMIN | M_FMAX | M_FLAGS | M_ZFIRST)) {
+      const uint64_t bits = A.int_as_float ? f64_bits((double)(int64_t)v) : v;
+      const uint32_t ex = (uint32_t)(bits >> 52) & 0x7ffu;
+      bool nan = false;
+      if (ex == 0x7ffu) {
+        nan = (bits << 12) != 0;
+        if (m & M_FLAGS)
+          atomicOr((unsigned long long *)&b[A.o_flags],
+                   (unsigned long long)(nan ? FL_NAN : ((bits >> 63) ? FL_NINF : FL_PINF)));
+      } else {
+        if (m & M_FSUM) {
+          uint64_t lo, hi;
+          fix_of(bits, A.fshift, lo, hi);
+          add128(&b[A.o_fsum], lo, hi);
+        }
+        if (m & M_FSUMSQ) {
+          uint64_t lo, hi;
+          fixsq_of(bits, A.fshift2, lo, hi);
+          add128(&b[A.o_sq], lo, hi);
+        }
+        if ((m & M_ZFIRST) && (bits << 1) == 0)
+          atomicMin((unsigned long long *)&b[A.o_zfirst],
+                    (unsigned long long)(((uint64_t)row << 1) | (bits >> 63)));
+      }
+      if (!nan) {
+        const uint64_t k = ord_key(bits);
+        if (m & M_FMIN) {
+          if (k < b[A.o_min]) atomicMin((unsigned long long *)&b[A.o_min], (unsigned long long)k);
+        }
+        if (m & M_FMAX) {
+          if (k > b[A.o_max]) atomicMax((unsigned long long *)&b[A.o_max], (unsigned long long)k);
+        }
+      }
+    }
+  }
+}
+
+// Merges a partial state `s` (W words) into `d`, word by word.
+__device__ __forceinline__ void merge_full(PlanRef P, uint64_t *d, const uint64_t *s) {
+  for (int w = 0; w < P.words; ++w) {
+    const uint64_t v = s[w];
+    switch (P.wop[w]) {
+      case W_ADD:
+        if (v) atomicAdd((unsigned long long *)&d[w], (unsigned long long)v);
+        break;
+      case W_ADD128:
+        add128(&d[w], v, s[w + 1]);
+        break;
+      case W_MIN_S:
+        if ((int64_t)v < (int64_t)d[w]) atomicMin((long long *)&d[w], (long long)v);
+        break;
+      case W_MAX_S:
+        if ((int64_t)v > (int64_t)d[w]) atomicMax((long long *)&d[w], (long long)v);
+        break;
+      case W_MIN_U:
+        if (v < d[w]) atomicMin((unsigned long long *)&d[w], (unsigned long long)v);
+        break;
+      case W_MAX_U:
+        if (v > d[w]) atomicMax((unsigned long long *)&d[w], (unsigned long long)v);
+        break;
+      case W_OR:
+        if (v) atomicOr((unsigned long long *)&d[w], (unsigned long long)v);
+        break;
+      default:  // W_HI: merged with its low word
+        break;
+    }
+  }
+}
+
+// Wave-aggregated append of one record slot per participating lane: returns
+// the record index for lanes with `take`, from a shared LDS counter.
+__device__ __forceinline__ uint32_t wave_append(bool take, uint32_t *counter) {
+  const uint64_t m = __ballot(take);
+  const int lane = __lane_id();
+  const int rank = __popcll(m & ((1ull << lane) - 1ull));
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (take && lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  if (take) base = (uint32_t)__shfl((int)base, leader);
+  return base + (uint32_t)rank;
+}
+
+// One kept-or-not row of pass A: predicate already applied (`kept`); folds
+// into the LDS table or spills a record. Every lane of the wave calls it.
+template <int LDSM, int NC>
+__device__ __forceinline__ void spill_row(PlanRef P, uint64_t *lkeys, uint64_t *lst, uint64_t *seg,
+                                          uint32_t *cnt, uint32_t *hraw, bool kept, int64_t row,
+                                          const uint64_t (&vals)[NC], uint32_t vbits) {
+  uint64_t packed = 0;
+  int e = -1;
+  if (kept) {
+#pragma unroll
+    for (int k = 0; k < kMaxKeys && k < NC; ++k) {
+      if (k >= P.n_keys) continue;
+      const DevKey K = cget(P.keys[k]);
+      const uint64_t digit = ((vbits >> k) & 1u) ? (vals[k] - (uint64_t)K.minv) : (K.radix - 1);
+      packed += digit * K.stride;
+    }
+    if (LDSM != LDS_NONE) e = lds_find<LDSM>(P, lkeys, packed);
+    if (e >= 0)
+      fold_full<NC>(P, lst + (int64_t)e * P.words, row, [&](int c) { return vals[c]; },
+                    [&](int c) { return ((vbits >> c) & 1u) != 0; });
+  }
+  const bool spill = kept && e < 0;
+  const uint32_t at = wave_append(spill, cnt);
+  if (spill) {
+    DFR_GLOBAL uint64_t *rec = gpw(seg) + (uint64_t)at * P.rw;
+    uint64_t vmask = 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (c >= P.n_cols || P.val_of[c] < 0) continue;
+      vmask |= (uint64_t)((vbits >> c) & 1u) << P.val_of[c];
+      rec[P.rec_val0 + P.val_of[c]] = vals[c];
+    }
+    rec[0] = P.vmask_in_key ? (packed | (vmask << P.key_bits)) : packed;
+    if (P.rec_meta >= 0) rec[P.rec_meta] = vmask;
+    if (P.rec_row >= 0) rec[P.rec_row] = (uint64_t)row;
+    atomicAdd(&hraw[part_of(P, packed)], 1u);
+  }
+}
+
+// U row pairs per thread: every load of the U pairs is issued before any of
+// them is processed, so each wave keeps U * NC * 16 bytes in flight.
+template <int LDSM, int NC, int U>
+__device__ __forceinline__ void spill_tile(PlanRef P, uint64_t *lkeys, uint64_t *lst, uint64_t *seg,
+                                           uint32_t *cnt, uint32_t *hraw, int64_t base, int64_t end) {
+  uint64_t v[U][2][NC];
+  uint32_t vb[U][2];
+  bool any[U], two[U];
+  const int64_t stride = 2 * (int64_t)blockDim.x;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t r = base + u * stride + 2 * (int64_t)threadIdx.x;
+    any[u] = r < end;
+    two[u] = r + 1 < end;
+    uint32_t valid = 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      v[u][0][c] = v[u][1][c] = 0;
+      if (c >= P.n_cols || !any[u]) continue;
+      load_pair(cget(P.cols[c]), r, two[u], P.vec_ok, v[u][0][c], v[u][1][c]);
+      valid |= load_valid_pair(cget(P.cols[c]), r, two[u]) << (2 * c);
+    }
+    vb[u][0] = vb[u][1] = 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      vb[u][0] |= ((valid >> (2 * c)) & 1u) << c;
+      vb[u][1] |= ((valid >> (2 * c + 1)) & 1u) << c;
+    }
+  }
+  // one rolled loop over the 2U rows keeps a single copy of the row code;
+  // the row's values are picked from the tile registers with selects
+#pragma unroll 1
+  for (int i = 0; i < 2 * U; ++i) {
+    uint64_t vals[NC];
+    uint32_t vbits = 0;
+    bool in = false;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) vals[c] = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        if (i != 2 * u + j) continue;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) vals[c] = v[u][j][c];
+        vbits = vb[u][j];
+        in = j ? two[u] : any[u];
+      }
+    const int64_t row = base + (i >> 1) * stride + 2 * (int64_t)threadIdx.x + (i & 1);
+    bool kept = in;
+    if (in && P.n_leaves) {
+      uint32_t m = 0;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        if (c >= P.n_cols || !((P.leaf_cols >> c) & 1)) continue;
+        const bool fl = P.cols[c].is_float;
+        for (int l = P.leaf_begin[c]; l < P.leaf_begin[c + 1]; ++l) {
+          const DevLeaf L = cget(P.leaves[l]);
+          m |= (uint32_t)leaf_true(L, fl, vals[c], (vbits >> c) & 1u) << L.bit;
+        }
+      }
+      kept = truth_bit(P, m) != 0;
+    }
+    spill_row<LDSM, NC>(P, lkeys, lst, seg, cnt, hraw, kept, row, vals, vbits);
+  }
+}
+
+// Pass A. Workgroup s folds rows [s * chunk, (s + 1) * chunk); NC bounds
+// the number of input columns, U the row pairs a thread loads at once.
+template <int LDSM, int NC, int U>
+__global__ void __launch_bounds__(1024) k_scan_spill(const DevPlan *__restrict__ plan, int64_t chunk) {
+  extern __shared__ uint64_t smem[];
+  PlanRef P = plan_ref(plan);
+  const int E = P.lds_entries, W = P.words, NP = P.n_parts;
+  uint64_t *lkeys = smem;
+  uint64_t *lst = smem + (LDSM == LDS_HASHED ? E : 0);
+  uint32_t *hraw = (uint32_t *)(lst + (LDSM == LDS_NONE ? 0 : (size_t)E * W));
+  uint32_t *hpar = hraw + NP;
+  uint32_t *cnt = hpar + NP;  // [0] raw records, [1] partial records
+  if (LDSM != LDS_NONE) {
+    for (int i = threadIdx.x; i < E; i += blockDim.x) {
+      if (LDSM == LDS_HASHED) lkeys[i] = kEmpty;
+      for (int w = 0; w < W; ++w) lst[i * W + w] = P.ident[w];
+    }
+  }
+  for (int i = threadIdx.x; i < 2 * NP + 2; i += blockDim.x) hraw[i] = 0;
+  __syncthreads();
+  const int s = blockIdx.x;
+  uint64_t *seg = P.spill + (uint64_t)s * (uint64_t)P.seg_rows * P.rw;
+  const int64_t b0 = (int64_t)s * chunk;
+  const int64_t b1 = min(b0 + chunk, P.rows);
+  const int64_t step = 2 * (int64_t)blockDim.x * U;
+  for (int64_t base = b0; base < b1; base += step)
+    spill_tile<LDSM, NC, U>(P, lkeys, lst, seg, &cnt[0], hraw, base, b1);
+  __syncthreads();
+  if (LDSM != LDS_NONE) {
+    uint64_t *pseg = P.partials + (uint64_t)s * (uint64_t)P.seg_partials * (1 + W);
+    const int Eround = (E + 63) & ~63;
+    for (int e = threadIdx.x; e < Eround; e += blockDim.x) {
+      const uint64_t *st = lst + (int64_t)e * W;
+      const bool occ = e < E && st[0] != 0;
+      const uint64_t key = (LDSM == LDS_DIRECT) ? (uint64_t)e : (e < E ? lkeys[e] : 0);
+      const uint32_t at = wave_append(occ, &cnt[1]);
+      if (occ) {
+        DFR_GLOBAL uint64_t *rec = gpw(pseg) + (uint64_t)at * (1 + W);
+        rec[0] = key;
+        for (int w = 0; w < W; ++w) rec[1 + w] = st[w];
+        atomicAdd(&hpar[part_of(P, key)], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < NP; p += blockDim.x) {
+    P.hist_raw[(uint64_t)s * NP + p] = hraw[p];
+    P.hist_par[(uint64_t)s * NP + p] = hpar[p];
+  }
+  if (threadIdx.x == 0) {
+    P.cnt_raw[s] = cnt[0];
+    P.cnt_par[s] = cnt[1];
+  }
+}
+
+// Pass B, step 1: per partition p, exclusive scan over segments of the
+// histograms (in place), partition totals into base_*[p].
+__global__ void k_part_offsets(const DevPlan *__restrict__ plan) {
+  PlanRef P = plan_ref(plan);
+  const int p = blockIdx.x, NP = P.n_parts, S = P.n_segs;
+  for (int which = 0; which < 2; ++which) {
+    uint32_t *h = which ? P.hist_par : P.hist_raw;
+    uint64_t carry = 0;
+    for (int s0 = 0; s0 < S; s0 += kBlock) {
+      const int s = s0 + threadIdx.x;
+      const uint64_t x = s < S ? h[(uint64_t)s * NP + p] : 0;
+      uint64_t tot;
+      const uint64_t ex = block_exclusive_scan(x, &tot);
+      if (s < S) h[(uint64_t)s * NP + p] = (uint32_t)(carry + ex);
+      carry += tot;
+    }
+    if (threadIdx.x == 0) (which ? P.base_par : P.base_raw)[p] = carry;
+  }
+}
+
+// Pass B, step 2 (one workgroup): partition starts from the totals.
+__global__ void k_part_base(const DevPlan *__restrict__ plan) {
+  PlanRef P = plan_ref(plan);
+  const int NP = P.n_parts;
+  for (int which = 0; which < 2; ++which) {
+    uint64_t *b = which ? P.base_par : P.base_raw;
+    uint64_t carry = 0;
+    for (int p0 = 0; p0 < NP; p0 += kBlock) {
+      const int p = p0 + threadIdx.x;
+      const uint64_t x = p < NP ? b[p] : 0;
+      uint64_t tot;
+      const uint64_t ex = block_exclusive_scan(x, &tot);
+      if (p < NP) b[p] = carry + ex;
+      carry += tot;
+    }
+    if (threadIdx.x == 0) b[NP] = carry;
+  }
+}
+
+// Pass B, step 3: segment s scatters its records into partition order.
+__global__ void __launch_bounds__(1024) k_part_scatter(const DevPlan *__restrict__ plan) {
+  extern __shared__ uint32_t cur[];  // [2 * n_parts]
+  PlanRef P = plan_ref(plan);
+  const int s = blockIdx.x, NP = P.n_parts, W = P.words;
+  for (int i = threadIdx.x; i < 2 * NP; i += blockDim.x) cur[i] = 0;
+  __syncthreads();
+  const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
+  {
+    const uint32_t n = P.cnt_raw[s];
+    const DFR_GLOBAL uint64_t *seg = gp(P.spill) + (uint64_t)s * (uint64_t)P.seg_rows * P.rw;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const DFR_GLOBAL uint64_t *rec = seg + (uint64_t)i * P.rw;
+      const uint32_t p = part_of(P, rec[0] & keymask);
+      const uint64_t pos = P.base_raw[p] + P.hist_raw[(uint64_t)s * NP + p] + atomicAdd(&cur[p], 1u);
+      DFR_GLOBAL uint64_t *dst = gpw(P.part_raw) + pos * P.rw;
+      for (int w = 0; w < P.rw; ++w) dst[w] = rec[w];
+    }
+  }
+  {
+    const uint32_t n = P.cnt_par[s];
+    const DFR_GLOBAL uint64_t *seg = gp(P.partials) + (uint64_t)s * (uint64_t)P.seg_partials * (1 + W);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const DFR_GLOBAL uint64_t *rec = seg + (uint64_t)i * (1 + W);
+      const uint32_t p = part_of(P, rec[0]);
+      const uint64_t pos = P.base_par[p] + P.hist_par[(uint64_t)s * NP + p] + atomicAdd(&cur[NP + p], 1u);
+      DFR_GLOBAL uint64_t *dst = gpw(P.part_par) + pos * (1 + W);
+      for (int w = 0; w < 1 + W; ++w) dst[w] = rec[w];
+    }
+  }
+}
+
+// LDS probe of a partition table (first-come, linear probing); -1 if full.
+__device__ __forceinline__ int part_find(PlanRef P, uint64_t *lkeys, uint64_t key) {
+  const int E = P.c_entries;
+  int h = part_slot(P, key);
+  for (int p = 0; p < P.lds_probe * 4; ++p) {
+    const uint64_t k = lkeys[h];
+    if (k == key) return h;
+    if (k == kEmpty) {
+      const unsigned long long old = atomicCAS((unsigned long long *)&lkeys[h], (unsigned long long)kEmpty,
+                                               (unsigned long long)key);
+      if (old == kEmpty || old == key) return h;
+    }
+    h = (h + 1) & (E - 1);
+  }
+  return -1;
+}
+
+// Pass C: partition p.
+__global__ void __launch_bounds__(1024) k_part_aggregate(const DevPlan *__restrict__ plan) {
+  extern __shared__ uint64_t smem[];
+  PlanRef P = plan_ref(plan);
+  const int p = blockIdx.x, E = P.c_entries, W = P.words;
+  uint64_t *lkeys = smem;
+  uint64_t *lst = smem + E;
+  for (int i = threadIdx.x; i < E; i += blockDim.x) {
+    lkeys[i] = kEmpty;
+    for (int w = 0; w < W; ++w) lst[i * W + w] = P.ident[w];
+  }
+  __syncthreads();
+  const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
+  for (uint64_t i = P.base_raw[p] + threadIdx.x; i < P.base_raw[p + 1]; i += blockDim.x) {
+    const DFR_GLOBAL uint64_t *rec = gp(P.part_raw) + i * P.rw;
+    const uint64_t key = rec[0] & keymask;
+    const uint64_t vmask = P.vmask_in_key ? (rec[0] >> P.key_bits) : (P.rec_meta >= 0 ? rec[P.rec_meta] : ~0ull);
+    const int64_t row = P.rec_row >= 0 ? (int64_t)rec[P.rec_row] : 0;
+    auto val = [&](int c) { return rec[P.rec_val0 + P.val_of[c]]; };
+    auto ok = [&](int c) { return ((vmask >> P.val_of[c]) & 1ull) != 0; };
+    const int e = part_find(P, lkeys, key);
+    if (e >= 0) {
+      fold_full(P, lst + (int64_t)e * W, row, val, ok);
+    } else {
+      const uint64_t g = global_slot(P, key);
+      if (g != kEmpty) fold_full(P, P.state + g * (uint64_t)W, row, val, ok);
+    }
+  }
+  for (uint64_t i = P.base_par[p] + threadIdx.x; i < P.base_par[p + 1]; i += blockDim.x) {
+    const DFR_GLOBAL uint64_t *rec = gp(P.part_par) + i * (1 + W);
+    const uint64_t key = rec[0];
+    uint64_t part[kMaxWords];
+    for (int w = 0; w < W; ++w) part[w] = rec[1 + w];
+    const int e = part_find(P, lkeys, key);
+    if (e >= 0) {
+      merge_full(P, lst + (int64_t)e * W, part);
+    } else {
+      const uint64_t g = global_slot(P, key);
+      if (g != kEmpty) merge_full(P, P.state + g * (uint64_t)W, part);
+    }
+  }
+  __syncthreads();
+  // every key of this partition lives only here: plain stores
+  for (int e = threadIdx.x; e < E; e += blockDim.x) {
+    const uint64_t key = lkeys[e];
+    if (key == kEmpty) continue;
+    const uint64_t g = global_slot(P, key);
+    if (g == kEmpty) continue;
+    uint64_t *d = P.state + g * (uint64_t)W;
+    const uint64_t *s = lst + (int64_t)e * W;
+    for (int w = 0; w < W; ++w) d[w] = s[w];
+  }
+}
+
+}  // namespace dfr

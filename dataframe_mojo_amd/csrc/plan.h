@@ -53,6 +53,25 @@ enum : uint32_t {
 enum : uint64_t { FL_NAN = 1, FL_PINF = 2, FL_NINF = 4 };
 
 enum : int32_t { MODE_DIRECT = 0, MODE_HASH = 1 };
+
+// merge operation of each state word (partial states are folded word by word)
+enum : int8_t {
+  W_ADD = 0,     // 64-bit add
+  W_ADD128 = 1,  // low word of a 128-bit add (the next word is its high word)
+  W_HI = 2,      // high word of a 128-bit add (merged with its low word)
+  W_MIN_S = 3,   // signed min
+  W_MAX_S = 4,   // signed max
+  W_MIN_U = 5,   // unsigned min
+  W_MAX_U = 6,   // unsigned max
+  W_OR = 7,
+};
+
+// How kept rows reach the group table.
+enum : int32_t {
+  AGG_ATOMIC = 0,     // LDS table, misses fold into HBM with device atomics
+  AGG_PARTITION = 1,  // LDS table, misses spill to records that are
+                      // radix-partitioned by key and folded per partition
+};
 enum : int32_t { LDS_NONE = 0, LDS_DIRECT = 1, LDS_HASHED = 2 };
 
 // dtypes (same values as dfrocm.h)
@@ -138,6 +157,35 @@ struct DevPlan {
   int8_t leaf_begin[kMaxCols + 1];  // leaves sorted by column
   DevLeaf leaves[kMaxLeaves];
   DevAgg aggs[kMaxAggs];
+  // ---- partitioned aggregation (AGG_PARTITION)
+  int32_t agg_mode;
+  int32_t rw;            // words per spilled row record
+  int32_t key_bits;      // bits of the packed key (vmask sits above them when it fits)
+  int32_t vmask_in_key;  // record word 0 = key | vmask << key_bits
+  int32_t n_vals;        // value words per record (columns with aggregates)
+  int32_t rec_row;       // word index of the row (has_first) or -1
+  int32_t rec_meta;      // word index of the validity mask word or -1
+  int32_t rec_val0;      // first value word
+  int8_t val_of[kMaxCols];  // record value index of each column or -1
+  int8_t wop[kMaxWords];    // merge op of each state word
+  int32_t part_log2;     // partitions = 1 << part_log2 (hash mode) ...
+  int32_t n_parts;       // number of partitions
+  int32_t pk_log2;       // direct mode: keys per partition = 1 << pk_log2
+  int32_t c_entries;     // LDS entries of a partition table (hash mode)
+  int32_t c_log2;
+  int32_t n_segs;        // workgroups of pass A (one segment each)
+  int64_t seg_rows;      // record capacity of a spill segment
+  int64_t seg_partials;  // partial-state capacity of a segment
+  uint64_t *spill;       // n_segs * seg_rows * rw
+  uint64_t *partials;    // n_segs * seg_partials * (1 + words)
+  uint32_t *hist_raw;    // [n_segs][n_parts]
+  uint32_t *hist_par;
+  uint32_t *cnt_raw;     // [n_segs]
+  uint32_t *cnt_par;
+  uint64_t *base_raw;    // [n_parts + 1] partition starts
+  uint64_t *base_par;
+  uint64_t *part_raw;    // records in partition order
+  uint64_t *part_par;
   // outputs of finalize
   void *out_keys[kMaxKeys];
   uint64_t *out_key_valid[kMaxKeys];

@@ -138,10 +138,17 @@ def _libgen():
             raise RuntimeError(f"{path} is not built; run python -m dataframe_mojo_amd.build")
         lib = ctypes.CDLL(path)
         u64, i64, vp = ctypes.c_uint64, ctypes.c_int64, ctypes.c_void_p
+        lib.dfrg_alloc.argtypes = [ctypes.POINTER(vp), u64]
+        lib.dfrg_free.argtypes = [vp]
+        lib.dfrg_copy_to_host.argtypes = [vp, vp, u64]
+        lib.dfrg_copy_to_device.argtypes = [vp, vp, u64]
+        lib.dfrg_fill_zero.argtypes = [vp, u64]
         lib.dfrg_uniform.argtypes = [vp, ctypes.c_int, u64, u64, u64, i64, i64]
         lib.dfrg_zipf_i32.argtypes = [vp, u64, u64, u64, vp, u64, i64]
         lib.dfrg_normal_f64.argtypes = [vp, u64, u64, u64, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         lib.dfrg_validity.argtypes = [vp, u64, u64, u64, u64]
+        lib.dfrg_stream.argtypes = [vp, vp, u64, ctypes.c_int, ctypes.c_int]
+        lib.dfrg_stream.restype = ctypes.c_double
         _gen = lib
     return _gen
 
@@ -151,60 +158,116 @@ def _check(rc: int, what: str):
         raise RuntimeError(f"{what} failed with HIP error {rc}")
 
 
-def device_uniform(n, seed, stream, lo, hi, dtype="int64", device="cuda"):
-    import torch
+class DeviceArray:
+    """A device buffer of `n` elements of a numpy dtype (HIP memory owned by
+    libdfrgen; the same HIP runtime as libdfrocm)."""
 
-    t = torch.empty(n, dtype=getattr(torch, dtype), device=device)
-    width = t.element_size()
-    _check(_libgen().dfrg_uniform(t.data_ptr(), width, n, seed, stream, lo, hi), "dfrg_uniform")
+    def __init__(self, n: int, dtype):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(_libgen().dfrg_alloc(ctypes.byref(p), max(self.nbytes, 16)), "dfrg_alloc")
+        self.ptr = p.value
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def numpy(self, count: int | None = None, start: int = 0) -> np.ndarray:
+        count = self.n - start if count is None else count
+        out = np.empty(count, dtype=self.dtype)
+        if count:
+            _check(_libgen().dfrg_copy_to_host(out.ctypes.data, self.ptr + start * self.dtype.itemsize,
+                                               count * self.dtype.itemsize), "dfrg_copy_to_host")
+        return out
+
+    @staticmethod
+    def from_numpy(a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(a.shape[0], a.dtype)
+        if a.nbytes:
+            _check(_libgen().dfrg_copy_to_device(d.ptr, a.ctypes.data, a.nbytes), "dfrg_copy_to_device")
+        return d
+
+    def free(self):
+        if self.ptr:
+            _libgen().dfrg_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_uniform(n, seed, stream, lo, hi, dtype=np.int64) -> DeviceArray:
+    t = DeviceArray(n, dtype)
+    _check(_libgen().dfrg_uniform(t.ptr, t.dtype.itemsize, n, seed, stream, lo, hi), "dfrg_uniform")
     return t
 
 
-def device_zipf(n, seed, stream, s, m, device="cuda"):
-    import torch
-
-    cdf = torch.from_numpy(zipf_cdf(s, m).view(np.int64)).to(device)
-    t = torch.empty(n, dtype=torch.int32, device=device)
-    _check(_libgen().dfrg_zipf_i32(t.data_ptr(), n, seed, stream, cdf.data_ptr(), m, 1), "dfrg_zipf_i32")
+def device_zipf(n, seed, stream, s, m) -> DeviceArray:
+    cdf = DeviceArray.from_numpy(zipf_cdf(s, m))
+    t = DeviceArray(n, np.int32)
+    _check(_libgen().dfrg_zipf_i32(t.ptr, n, seed, stream, cdf.ptr, m, 1), "dfrg_zipf_i32")
+    cdf.free()
     return t
 
 
-def device_normal(n, seed, stream, mu=0.0, sigma=1.0, lognormal=False, device="cuda"):
-    import torch
-
-    t = torch.empty(n, dtype=torch.float64, device=device)
-    _check(_libgen().dfrg_normal_f64(t.data_ptr(), n, seed, stream, mu, sigma, int(lognormal)), "dfrg_normal_f64")
+def device_normal(n, seed, stream, mu=0.0, sigma=1.0, lognormal=False) -> DeviceArray:
+    t = DeviceArray(n, np.float64)
+    _check(_libgen().dfrg_normal_f64(t.ptr, n, seed, stream, mu, sigma, int(lognormal)), "dfrg_normal_f64")
     return t
 
 
-def device_validity(n, seed, stream, p_null, device="cuda"):
-    """Validity as int64 words (LSB-first, bit r of word r // 64)."""
-    import torch
-
-    t = torch.empty((n + 63) // 64 + 1, dtype=torch.int64, device=device)
-    t[-1] = 0
-    _check(_libgen().dfrg_validity(t.data_ptr(), n, seed, stream, null_threshold(p_null)), "dfrg_validity")
+def device_validity(n, seed, stream, p_null) -> DeviceArray:
+    """Validity as uint64 words (LSB-first, bit r of word r // 64), with one
+    spare zero word at the end."""
+    t = DeviceArray((n + 63) // 64 + 1, np.uint64)
+    _check(_libgen().dfrg_fill_zero(t.ptr, t.nbytes), "dfrg_fill_zero")
+    _check(_libgen().dfrg_validity(t.ptr, n, seed, stream, null_threshold(p_null)), "dfrg_validity")
     return t
 
 
-def zipf_200m_device(n: int = 200_000_000, seed: int = 0, device="cuda") -> dict:
+def stream_rate(nbytes: int, copy: bool = False, blocks: int = 0, reps: int = 5) -> float:
+    """Achievable HBM rate in bytes/s: a streaming read (or copy) of nbytes,
+    best of `reps`. A copy counts read + write bytes."""
+    nbytes = nbytes // 16 * 16
+    src = DeviceArray(nbytes // 8, np.uint64)
+    _check(_libgen().dfrg_fill_zero(src.ptr, src.nbytes), "dfrg_fill_zero")
+    dst = DeviceArray(nbytes // 8, np.uint64) if copy else None
+    blocks = blocks or 256 * 64
+    ms = _libgen().dfrg_stream(src.ptr, dst.ptr if dst else None, nbytes, blocks, reps)
+    if ms <= 0:
+        raise RuntimeError("dfrg_stream failed")
+    return (2 if copy else 1) * nbytes / (ms / 1e3)
+
+
+def valid_from_words(words: DeviceArray, n: int) -> np.ndarray:
+    """Host boolean validity of the first n rows of device validity words."""
+    w = words.numpy((n + 63) // 64)
+    return np.unpackbits(w.view(np.uint8), bitorder="little")[:n].astype(bool)
+
+
+def zipf_200m_device(n: int = 200_000_000, seed: int = 0) -> dict:
     """Device columns of the 200M-row Zipf workload (see module doc)."""
     return {
-        "key": device_zipf(n, seed, STREAMS["key"], 1.1, 1_000_000, device),
-        "a": device_uniform(n, seed, STREAMS["a"], -1_000_000_000, 1_000_000_000, device=device),
-        "x": device_normal(n, seed, STREAMS["x"], 0.0, 1.0, device=device),
-        "x.valid": device_validity(n, seed, STREAMS["x.valid"], 0.05, device),
-        "y": device_uniform(n, seed, STREAMS["y"], 0, 1_000_000, device=device),
+        "key": device_zipf(n, seed, STREAMS["key"], 1.1, 1_000_000),
+        "a": device_uniform(n, seed, STREAMS["a"], -1_000_000_000, 1_000_000_000),
+        "x": device_normal(n, seed, STREAMS["x"], 0.0, 1.0),
+        "x.valid": device_validity(n, seed, STREAMS["x.valid"], 0.05),
+        "y": device_uniform(n, seed, STREAMS["y"], 0, 1_000_000),
     }
 
 
-def dict_500m_device(n: int = 500_000_000, seed: int = 0, device="cuda") -> dict:
+def dict_500m_device(n: int = 500_000_000, seed: int = 0) -> dict:
     """Device columns of the 500M-row dictionary-key workload."""
     return {
-        "k1": device_uniform(n, seed, STREAMS["k1"], 0, 999, dtype="int32", device=device),
-        "k2": device_uniform(n, seed, STREAMS["k2"], 0, 9_999_999, device=device),
-        "v": device_normal(n, seed, STREAMS["v"], 0.0, 2.0, lognormal=True, device=device),
-        "v.valid": device_validity(n, seed, STREAMS["v.valid"], 0.02, device),
+        "k1": device_uniform(n, seed, STREAMS["k1"], 0, 999, dtype=np.uint32),
+        "k2": device_uniform(n, seed, STREAMS["k2"], 0, 9_999_999),
+        "v": device_normal(n, seed, STREAMS["v"], 0.0, 2.0, lognormal=True),
+        "v.valid": device_validity(n, seed, STREAMS["v.valid"], 0.02),
     }
 
 

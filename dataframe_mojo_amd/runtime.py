@@ -3,7 +3,7 @@
 A `Context` owns one device, one HIP stream and one HBM budget
 (DATAFRAME_ACCEL_MEMORY_LIMIT, as dataframe_metal/metal.mojo:36-46 reads it).
 Columns reach the device by upload (host arrays) or by wrapping device
-memory that already holds them (zero copy; e.g. torch tensors), which is the
+memory that already holds them (zero copy), which is the
 device column hand-off of storage/residence.mojo.
 """
 from __future__ import annotations
@@ -228,13 +228,15 @@ class Context:
         )
         return DeviceColumn(self, h, dtype, length, labels, keepalive)
 
-    def wrap_tensor(self, values, valid_words=None, labels=None, dict_codes=False) -> DeviceColumn:
-        """Wraps torch device tensors (values; optional int64 validity words)."""
-        import torch
-
-        dt = {torch.int32: _lib.DFR_DICT if dict_codes else _lib.DFR_INT32, torch.int64: _lib.DFR_INT64,
-              torch.float64: _lib.DFR_FLOAT64}[values.dtype]
-        return self.wrap(dt, values.data_ptr(), values.numel(), valid_words.data_ptr() if valid_words is not None else 0,
+    def wrap_array(self, values, valid_words=None, labels=None, dict_codes=False, length=None) -> DeviceColumn:
+        """Wraps device arrays in place (anything with data_ptr(), dtype and
+        n: datagen.DeviceArray); `valid_words` are uint64 LSB-first words."""
+        dt = {np.dtype(np.int32): _lib.DFR_INT32, np.dtype(np.uint32): _lib.DFR_DICT,
+              np.dtype(np.int64): _lib.DFR_INT64, np.dtype(np.float64): _lib.DFR_FLOAT64}[np.dtype(values.dtype)]
+        if dict_codes:
+            dt = _lib.DFR_DICT
+        n = values.n if length is None else length
+        return self.wrap(dt, values.data_ptr(), n, valid_words.data_ptr() if valid_words is not None else 0,
                          labels=labels, keepalive=(values, valid_words))
 
     # queries ---------------------------------------------------------------

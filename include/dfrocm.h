@@ -107,7 +107,10 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
 /* Sets one launch or table parameter of the context (no counterpart in the
  * reference: the autotuner's knob). Keys: "block", "wg_per_cu",
  * "lds_bytes", "lds_probe", "lds_mode" (-1 auto, 0 none, 1 direct,
- * 2 hashed), "direct_max_slots", "hash_probe", "hash_slots". */
+ * 2 hashed), "direct_max_slots", "hash_probe", "hash_slots", "agg_mode"
+ * (1 partitioned: rows that miss the LDS table are spilled and folded per
+ * key partition; 0 atomic: they fold into HBM with device atomics),
+ * "part_lds_bytes" (LDS of a partition table). */
 int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
 
 /* ---- device columns (the column hand-off) ------------------------------ */

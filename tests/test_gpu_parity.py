@@ -201,27 +201,22 @@ def test_table_modes_give_identical_results(setting, gpu_ctx):
 
 
 def test_device_generator_matches_host_twin():
-    import torch
-
     n = 100003
-    u = datagen.device_uniform(n, 11, 3, -1000, 1000).cpu().numpy()
+    u = datagen.device_uniform(n, 11, 3, -1000, 1000).numpy()
     assert np.array_equal(u, datagen.uniform(11, 3, n, -1000, 1000))
-    v = datagen.device_validity(n, 11, 4, 0.05).cpu().numpy().view(np.uint64)
-    bits = np.unpackbits(v.view(np.uint8), bitorder="little")[:n].astype(bool)
-    assert np.array_equal(bits, datagen.validity_bits(11, 4, n, 0.05))
-    z = datagen.device_zipf(n, 11, 5, 1.1, 1000).cpu().numpy()
+    v = datagen.device_validity(n, 11, 4, 0.05)
+    assert np.array_equal(datagen.valid_from_words(v, n), datagen.validity_bits(11, 4, n, 0.05))
+    z = datagen.device_zipf(n, 11, 5, 1.1, 1000).numpy()
     assert np.array_equal(z, datagen.zipf(11, 5, n, datagen.zipf_cdf(1.1, 1000)))
-    torch.cuda.synchronize()
 
 
 def _zipf_frame_from_device(cols, n):
     """Host copies of the first n rows of device workload columns."""
-    x_valid = np.unpackbits(cols["x.valid"].cpu().numpy().view(np.uint8), bitorder="little")[:n].astype(bool)
     return DataFrame([
-        Series("key", Column.int32(cols["key"][:n].cpu().numpy())),
-        Series("a", Column.int64(cols["a"][:n].cpu().numpy())),
-        Series("x", Column.float64(cols["x"][:n].cpu().numpy(), x_valid)),
-        Series("y", Column.int64(cols["y"][:n].cpu().numpy())),
+        Series("key", Column.int32(cols["key"].numpy(n))),
+        Series("a", Column.int64(cols["a"].numpy(n))),
+        Series("x", Column.float64(cols["x"].numpy(n), datagen.valid_from_words(cols["x.valid"], n))),
+        Series("y", Column.int64(cols["y"].numpy(n))),
     ])
 
 
@@ -245,10 +240,10 @@ def test_zipf_workload_prefix_matches_oracle(gpu_ctx):
     lf = host.filter(col("a") > 0).group_by("key", maintain_order=True).agg(ZIPF_AGGS)
     dplan = compile_plan(lf)
     dev = {
-        "key": gpu_ctx.wrap(_lib.DFR_INT32, cols["key"].data_ptr(), n, keepalive=cols["key"]),
-        "a": gpu_ctx.wrap(_lib.DFR_INT64, cols["a"].data_ptr(), n, keepalive=cols["a"]),
-        "x": gpu_ctx.wrap(_lib.DFR_FLOAT64, cols["x"].data_ptr(), n, cols["x.valid"].data_ptr(), keepalive=cols),
-        "y": gpu_ctx.wrap(_lib.DFR_INT64, cols["y"].data_ptr(), n, keepalive=cols["y"]),
+        "key": gpu_ctx.wrap_array(cols["key"], length=n),
+        "a": gpu_ctx.wrap_array(cols["a"], length=n),
+        "x": gpu_ctx.wrap_array(cols["x"], cols["x.valid"], length=n),
+        "y": gpu_ctx.wrap_array(cols["y"], length=n),
     }
     res = gpu_ctx.groupby(dplan, [dev[k] for k in dplan.keys], [dev[c] for c in dplan.columns])
     got = result_frame(res, dplan, [dev[k] for k in dplan.keys])
