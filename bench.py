@@ -86,6 +86,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="first run one streaming copy of this many bytes (a known-size kernel for byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -95,6 +97,9 @@ def main() -> int:
     from dataframe_mojo_amd import datagen, runtime
     from dataframe_mojo_amd.plan import compile_plan
 
+    if args.copy_calibration:
+        rate = datagen.stream_rate(args.copy_calibration, copy=True, reps=1)
+        log(f"calibration copy of {args.copy_calibration} bytes: {rate / 1e9:.0f} GB/s (read + write)")
     n = args.rows
     log(f"generating zipf_200m columns on the device: {n} rows")
     cols = datagen.zipf_200m_device(n, args.seed)

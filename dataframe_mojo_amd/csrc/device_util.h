@@ -53,28 +53,40 @@ DFR_HD void shr128(uint64_t &lo, uint64_t &hi, int s) {
   }
 }
 
+// Shifts of the 64-bit m into a 128-bit result without branches: left by
+// sh in [0, 127] (bits above 127 are dropped) or right by -sh.
+DFR_HD void shift_u64_to_128(uint64_t m, int sh, uint64_t &lo, uint64_t &hi) {
+  const int l = sh < 0 ? 0 : sh;      // left amount
+  const int r = sh < 0 ? -sh : 0;     // right amount
+  const int l6 = l & 63, r6 = r & 63;
+  const uint64_t ml = m << l6;
+  const uint64_t spill = l6 ? (m >> (64 - l6)) : 0ull;
+  const uint64_t mr = m >> r6;
+  uint64_t L = (l < 64) ? ml : 0ull;
+  uint64_t H = (l < 64) ? spill : (l < 128 ? ml : 0ull);
+  if (sh < 0) {
+    L = r < 64 ? mr : 0ull;
+    H = 0;
+  }
+  lo = L;
+  hi = H;
+}
+
 // Fixed-point image of a finite double: trunc(|x| * 2^S) with x's sign, as
 // a two's-complement 128-bit integer. The caller's S keeps the magnitude
 // below 2^127.
 DFR_HD void fix_of(uint64_t bits, int S, uint64_t &lo, uint64_t &hi) {
-  int e = (int)((bits >> 52) & 0x7ff);
-  uint64_t m = bits & ((1ull << 52) - 1);
-  if (e == 0)
-    e = 1;
-  else
-    m |= 1ull << 52;
-  int sh = e - 1075 + S;  // value = m * 2^(e - 1075)
-  uint64_t l = m, h = 0;
-  if (sh >= 0)
-    shl128(l, h, sh);
-  else
-    shr128(l, h, -sh);
-  if (bits >> 63) {
-    l = ~l + 1;
-    h = ~h + (l == 0 ? 1 : 0);
-  }
-  lo = l;
-  hi = h;
+  const int eb = (int)((bits >> 52) & 0x7ff);
+  const uint64_t m = (bits & ((1ull << 52) - 1)) | (eb ? (1ull << 52) : 0ull);
+  const int e = eb ? eb : 1;
+  uint64_t l, h;
+  shift_u64_to_128(m, e - 1075 + S, l, h);  // value = m * 2^(e - 1075)
+  // two's complement when negative, without branches: with neg all ones,
+  // (x ^ neg) - neg = ~x + 1; the low word's +1 carries into the high word
+  // exactly when the low word is 0
+  const uint64_t neg = (uint64_t)0 - (bits >> 63);
+  lo = (l ^ neg) - neg;
+  hi = (h ^ neg) + ((neg & 1ull) & (l == 0 ? 1ull : 0ull));
 }
 
 // Fixed-point image of x*x: trunc(x^2 * 2^S2), non-negative.

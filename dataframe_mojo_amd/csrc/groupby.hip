@@ -488,7 +488,8 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots) {
                     : (size_t)P.lds_entries * 8 * (LW + (P.lds_mode == LDS_HASHED ? 1 : 0));
   if (P.agg_mode == AGG_PARTITION) {
     P.n_segs = C.grid;
-    P.seg_rows = chunk;
+    P.seg_waves = C.block / 64;
+    P.seg_rows = chunk / P.seg_waves;  // a wave sees chunk / waves rows
     P.seg_partials = std::max(P.lds_entries, 1);
     const int W = P.words;
     C.lds_bytes = (P.lds_mode == LDS_NONE ? 0
@@ -564,10 +565,16 @@ int launch_scan(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char 
 template <int LDSM>
 int launch_scan_cols(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
   const int nc = C.plan.n_cols;
-  if (nc <= 4 && ctx->tune.scan_unroll == 1) return launch_scan<LDSM, 4, 1>(ctx, C, dplan, error);
-  if (nc <= 4 && ctx->tune.scan_unroll == 2) return launch_scan<LDSM, 4, 2>(ctx, C, dplan, error);
-  if (nc <= 4) return launch_scan<LDSM, 4, 4>(ctx, C, dplan, error);
-  if (nc <= 8) return launch_scan<LDSM, 8, 2>(ctx, C, dplan, error);
+  const int u = ctx->tune.scan_unroll;
+  if (nc <= 4) {
+    if (u == 4) return launch_scan<LDSM, 4, 4>(ctx, C, dplan, error);
+    if (u == 2) return launch_scan<LDSM, 4, 2>(ctx, C, dplan, error);
+    return launch_scan<LDSM, 4, 1>(ctx, C, dplan, error);
+  }
+  if (nc <= 8) {
+    if (u >= 2) return launch_scan<LDSM, 8, 2>(ctx, C, dplan, error);
+    return launch_scan<LDSM, 8, 1>(ctx, C, dplan, error);
+  }
   return launch_scan<LDSM, kMaxCols, 1>(ctx, C, dplan, error);
 }
 
@@ -585,7 +592,8 @@ int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan
   HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
   hipLaunchKernelGGL(k_part_offsets, dim3(P.n_parts), dim3(kBlock), 0, ctx->stream, dplan);
   hipLaunchKernelGGL(k_part_base, dim3(1), dim3(kBlock), 0, ctx->stream, dplan);
-  const size_t cur_bytes = (size_t)2 * P.n_parts * 4;
+  // k_part_scatter keeps one 64-bit write cursor per partition and kind
+  const size_t cur_bytes = (size_t)2 * P.n_parts * sizeof(unsigned long long);
   if ((st = set_lds(k_part_scatter, cur_bytes, error))) return st;
   hipLaunchKernelGGL(k_part_scatter, dim3(P.n_segs), dim3(1024), cur_bytes, ctx->stream, dplan);
   if ((st = set_lds(k_part_aggregate, C.c_lds_bytes, error))) return st;
@@ -639,12 +647,12 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
     P.keys_tab = P.mode == MODE_HASH ? P.keys_tab : nullptr;
     if (P.agg_mode == AGG_PARTITION) {
       const uint64_t S_ = (uint64_t)P.n_segs, NP = (uint64_t)P.n_parts, W = (uint64_t)P.words;
-      const uint64_t raw_words = S_ * (uint64_t)P.seg_rows * (uint64_t)P.rw;
+      const uint64_t raw_words = S_ * (uint64_t)P.seg_waves * (uint64_t)P.seg_rows * (uint64_t)P.rw;
       const uint64_t par_words = S_ * (uint64_t)P.seg_partials * (1 + W);
       if ((st = S.get(raw_words, &P.spill, error)) || (st = S.get(raw_words, &P.part_raw, error)) ||
           (st = S.get(par_words, &P.partials, error)) || (st = S.get(par_words, &P.part_par, error)) ||
           (st = S.get(S_ * NP, &P.hist_raw, error)) || (st = S.get(S_ * NP, &P.hist_par, error)) ||
-          (st = S.get(S_, &P.cnt_raw, error)) || (st = S.get(S_, &P.cnt_par, error)) ||
+          (st = S.get(S_ * (uint64_t)P.seg_waves, &P.cnt_raw, error)) || (st = S.get(S_, &P.cnt_par, error)) ||
           (st = S.get(NP + 1, &P.base_raw, error)) || (st = S.get(NP + 1, &P.base_par, error)))
         return st;
     }

@@ -25,14 +25,15 @@ namespace dfr {
 
 constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
 
+__device__ __forceinline__ uint32_t key32(uint64_t key) { return (uint32_t)key ^ (uint32_t)(key >> 32); }
+
 __device__ __forceinline__ uint32_t part_of(PlanRef P, uint64_t key) {
-  return P.part_log2 ? (uint32_t)((key * kGolden) >> (64 - P.part_log2)) : 0u;
+  return P.part_log2 ? ((key32(key) * 0x9E3779B1u) >> (32 - P.part_log2)) : 0u;
 }
 
-// LDS index inside a partition's table: the hash bits below the partition's
+// LDS index inside a partition's table: a second multiplicative hash
 __device__ __forceinline__ int part_slot(PlanRef P, uint64_t key) {
-  uint64_t h = (key * kGolden) << P.part_log2;
-  return (int)(h >> (64 - P.c_log2));
+  return (int)((key32(key) * 0x85EBCA77u) >> (32 - P.c_log2));
 }
 
 // Folds one kept row into a full-layout state (every word, FLAGS and ZFIRST
@@ -40,9 +41,7 @@ __device__ __forceinline__ int part_slot(PlanRef P, uint64_t key) {
 template <int NC = kMaxCols, class Val, class Ok>
 __device__ __forceinline__ void fold_full(PlanRef P, uint64_t *st, int64_t row, Val val, Ok ok) {
   atomicAdd((unsigned long long *)&st[0], 1ull);
-  if (P.has_first) {
-    if ((uint64_t)row < st[1]) atomicMin((unsigned long long *)&st[1], (unsigned long long)row);
-  }
+  if (P.has_first) atomicMin((unsigned long long *)&st[1], (unsigned long long)row);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     if (c >= P.n_cols) continue;
@@ -54,12 +53,8 @@ __device__ __forceinline__ void fold_full(PlanRef P, uint64_t *st, int64_t row, 
     if (m & M_VCOUNT) atomicAdd((unsigned long long *)&b[A.o_count], 1ull);
     if (m & M_ISUM64) atomicAdd((unsigned long long *)&b[A.o_isum], (unsigned long long)v);
     if (m & M_ISUM128) add128(&b[A.o_isum], v, ((int64_t)v < 0) ? ~0ull : 0ull);
-    if (m & M_IMIN) {
-      if ((int64_t)v < (int64_t)b[A.o_min]) atomicMin((long long *)&b[A.o_min], (long long)v);
-    }
-    if (m & M_IMAX) {
-      if ((int64_t)v > (int64_t)b[A.o_max]) atomicMax((long long *)&b[A.o_max], (long long)v);
-    }
+    if (m & M_IMIN) atomicMin((long long *)&b[A.o_min], (long long)v);
+    if (m & M_IMAX) atomicMax((long long *)&b[A.o_max], (long long)v);
     if (m & (M_FSUM | M_FSUMSQ | M_FMIN | M_FMAX | M_FLAGS | M_ZFIRST)) {
       const uint64_t bits = A.int_as_float ? f64_bits((double)(int64_t)v) : v;
       const uint32_t ex = (uint32_t)(bits >> 52) & 0x7ffu;
@@ -86,19 +81,16 @@ __device__ __forceinline__ void fold_full(PlanRef P, uint64_t *st, int64_t row, 
       }
       if (!nan) {
         const uint64_t k = ord_key(bits);
-        if (m & M_FMIN) {
-          if (k < b[A.o_min]) atomicMin((unsigned long long *)&b[A.o_min], (unsigned long long)k);
-        }
-        if (m & M_FMAX) {
-          if (k > b[A.o_max]) atomicMax((unsigned long long *)&b[A.o_max], (unsigned long long)k);
-        }
+        if (m & M_FMIN) atomicMin((unsigned long long *)&b[A.o_min], (unsigned long long)k);
+        if (m & M_FMAX) atomicMax((unsigned long long *)&b[A.o_max], (unsigned long long)k);
       }
     }
   }
 }
 
 // Merges a partial state `s` (W words) into `d`, word by word.
-__device__ __forceinline__ void merge_full(PlanRef P, uint64_t *d, const uint64_t *s) {
+template <class SrcPtr>
+__device__ __forceinline__ void merge_full(PlanRef P, uint64_t *d, SrcPtr s) {
   for (int w = 0; w < P.words; ++w) {
     const uint64_t v = s[w];
     switch (P.wop[w]) {
@@ -142,29 +134,73 @@ __device__ __forceinline__ uint32_t wave_append(bool take, uint32_t *counter) {
   return base + (uint32_t)rank;
 }
 
+// Scan LDS table: buckets of 4 key slots, read together; a key goes to the
+// first free slot of its bucket (first come), and a row whose bucket is
+// full spills. One LDS round trip per lookup, hit or miss.
+template <int LDSM>
+__device__ __forceinline__ int bucket_find(PlanRef P, uint64_t *lkeys, uint64_t key) {
+  if (LDSM == LDS_DIRECT) return (int)key;
+  const int b = (int)((key32(key) * 0x9E3779B1u) >> (32 - (P.lds_log2 - 2)));
+  uint64_t *kb = lkeys + 4 * b;
+  uint64_t k[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) k[j] = kb[j];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (k[j] == key) return 4 * b + j;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (k[j] != kEmpty) continue;
+    const unsigned long long old =
+        atomicCAS((unsigned long long *)&kb[j], (unsigned long long)kEmpty, (unsigned long long)key);
+    if (old == kEmpty || old == key) return 4 * b + j;
+  }
+  return -1;
+}
+
+// Whether truth-table index m keeps its row. The truth words are uniform;
+// up to five leaves need only the first.
+__device__ __forceinline__ bool truth_keeps(PlanRef P, uint32_t m) {
+  uint32_t w = P.truth[0];
+  if (P.n_leaves > 5) {
+    const uint32_t q = m >> 5;
+#pragma unroll
+    for (int i = 1; i < 8; ++i)
+      if (q == (uint32_t)i) w = P.truth[i];
+  }
+  return ((w >> (m & 31)) & 1u) != 0;
+}
+
 // One kept-or-not row of pass A: predicate already applied (`kept`); folds
 // into the LDS table or spills a record. Every lane of the wave calls it.
 template <int LDSM, int NC>
 __device__ __forceinline__ void spill_row(PlanRef P, uint64_t *lkeys, uint64_t *lst, uint64_t *seg,
-                                          uint32_t *cnt, uint32_t *hraw, bool kept, int64_t row,
+                                          uint32_t &wcur, uint32_t *hraw, bool kept, int64_t row,
                                           const uint64_t (&vals)[NC], uint32_t vbits) {
   uint64_t packed = 0;
   int e = -1;
   if (kept) {
+    if (P.n_keys == 1) {
+      packed = (vbits & 1u) ? (vals[0] - (uint64_t)P.keys[0].minv) : (P.keys[0].radix - 1);
+    } else {
 #pragma unroll
-    for (int k = 0; k < kMaxKeys && k < NC; ++k) {
-      if (k >= P.n_keys) continue;
-      const DevKey K = cget(P.keys[k]);
-      const uint64_t digit = ((vbits >> k) & 1u) ? (vals[k] - (uint64_t)K.minv) : (K.radix - 1);
-      packed += digit * K.stride;
+      for (int k = 0; k < kMaxKeys && k < NC; ++k) {
+        if (k >= P.n_keys) continue;
+        const DevKey K = cget(P.keys[k]);
+        const uint64_t digit = ((vbits >> k) & 1u) ? (vals[k] - (uint64_t)K.minv) : (K.radix - 1);
+        packed += digit * K.stride;
+      }
     }
-    if (LDSM != LDS_NONE) e = lds_find<LDSM>(P, lkeys, packed);
+    if (LDSM != LDS_NONE) e = bucket_find<LDSM>(P, lkeys, packed);
     if (e >= 0)
       fold_full<NC>(P, lst + (int64_t)e * P.words, row, [&](int c) { return vals[c]; },
                     [&](int c) { return ((vbits >> c) & 1u) != 0; });
   }
+  // the wave appends to its own segment: cursor uniform across the wave
   const bool spill = kept && e < 0;
-  const uint32_t at = wave_append(spill, cnt);
+  const uint64_t sm = __ballot(spill);
+  const uint32_t at = wcur + (uint32_t)__popcll(sm & ((1ull << __lane_id()) - 1ull));
+  wcur += (uint32_t)__popcll(sm);
   if (spill) {
     DFR_GLOBAL uint64_t *rec = gpw(seg) + (uint64_t)at * P.rw;
     uint64_t vmask = 0;
@@ -185,7 +221,7 @@ __device__ __forceinline__ void spill_row(PlanRef P, uint64_t *lkeys, uint64_t *
 // them is processed, so each wave keeps U * NC * 16 bytes in flight.
 template <int LDSM, int NC, int U>
 __device__ __forceinline__ void spill_tile(PlanRef P, uint64_t *lkeys, uint64_t *lst, uint64_t *seg,
-                                           uint32_t *cnt, uint32_t *hraw, int64_t base, int64_t end) {
+                                           uint32_t &wcur, uint32_t *hraw, int64_t base, int64_t end) {
   uint64_t v[U][2][NC];
   uint32_t vb[U][2];
   bool any[U], two[U];
@@ -242,9 +278,9 @@ __device__ __forceinline__ void spill_tile(PlanRef P, uint64_t *lkeys, uint64_t 
           m |= (uint32_t)leaf_true(L, fl, vals[c], (vbits >> c) & 1u) << L.bit;
         }
       }
-      kept = truth_bit(P, m) != 0;
+      kept = truth_keeps(P, m);
     }
-    spill_row<LDSM, NC>(P, lkeys, lst, seg, cnt, hraw, kept, row, vals, vbits);
+    spill_row<LDSM, NC>(P, lkeys, lst, seg, wcur, hraw, kept, row, vals, vbits);
   }
 }
 
@@ -269,12 +305,17 @@ __global__ void __launch_bounds__(1024) k_scan_spill(const DevPlan *__restrict__
   for (int i = threadIdx.x; i < 2 * NP + 2; i += blockDim.x) hraw[i] = 0;
   __syncthreads();
   const int s = blockIdx.x;
-  uint64_t *seg = P.spill + (uint64_t)s * (uint64_t)P.seg_rows * P.rw;
+  // wave segment: the rows a wave sees, seg_rows records at most
+  const int wave = threadIdx.x >> 6;
+  const uint64_t wseg = (uint64_t)s * (blockDim.x >> 6) + wave;
+  uint64_t *seg = P.spill + wseg * (uint64_t)P.seg_rows * P.rw;
+  uint32_t wcur = 0;
   const int64_t b0 = (int64_t)s * chunk;
   const int64_t b1 = min(b0 + chunk, P.rows);
   const int64_t step = 2 * (int64_t)blockDim.x * U;
   for (int64_t base = b0; base < b1; base += step)
-    spill_tile<LDSM, NC, U>(P, lkeys, lst, seg, &cnt[0], hraw, base, b1);
+    spill_tile<LDSM, NC, U>(P, lkeys, lst, seg, wcur, hraw, base, b1);
+  if ((threadIdx.x & 63) == 0) P.cnt_raw[wseg] = wcur;
   __syncthreads();
   if (LDSM != LDS_NONE) {
     uint64_t *pseg = P.partials + (uint64_t)s * (uint64_t)P.seg_partials * (1 + W);
@@ -297,10 +338,7 @@ __global__ void __launch_bounds__(1024) k_scan_spill(const DevPlan *__restrict__
     P.hist_raw[(uint64_t)s * NP + p] = hraw[p];
     P.hist_par[(uint64_t)s * NP + p] = hpar[p];
   }
-  if (threadIdx.x == 0) {
-    P.cnt_raw[s] = cnt[0];
-    P.cnt_par[s] = cnt[1];
-  }
+  if (threadIdx.x == 0) P.cnt_par[s] = cnt[1];
 }
 
 // Pass B, step 1: per partition p, exclusive scan over segments of the
@@ -342,35 +380,64 @@ __global__ void k_part_base(const DevPlan *__restrict__ plan) {
   }
 }
 
-// Pass B, step 3: segment s scatters its records into partition order.
-__global__ void __launch_bounds__(1024) k_part_scatter(const DevPlan *__restrict__ plan) {
-  extern __shared__ uint32_t cur[];  // [2 * n_parts]
-  PlanRef P = plan_ref(plan);
-  const int s = blockIdx.x, NP = P.n_parts, W = P.words;
-  for (int i = threadIdx.x; i < 2 * NP; i += blockDim.x) cur[i] = 0;
-  __syncthreads();
-  const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
-  {
-    const uint32_t n = P.cnt_raw[s];
-    const DFR_GLOBAL uint64_t *seg = gp(P.spill) + (uint64_t)s * (uint64_t)P.seg_rows * P.rw;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-      const DFR_GLOBAL uint64_t *rec = seg + (uint64_t)i * P.rw;
-      const uint32_t p = part_of(P, rec[0] & keymask);
-      const uint64_t pos = P.base_raw[p] + P.hist_raw[(uint64_t)s * NP + p] + atomicAdd(&cur[p], 1u);
-      DFR_GLOBAL uint64_t *dst = gpw(P.part_raw) + pos * P.rw;
-      for (int w = 0; w < P.rw; ++w) dst[w] = rec[w];
+// Pass B, step 3: segment s scatters its records into partition order. The
+// segment's write cursor of every partition sits in LDS (start of the
+// partition + this segment's offset); each thread moves U records at once.
+template <int RW, int U>
+__device__ __forceinline__ void scatter_records(const DFR_GLOBAL uint64_t *seg, uint32_t n, int rw, uint64_t keymask,
+                                                PlanRef P, unsigned long long *cur, DFR_GLOBAL uint64_t *dst) {
+  for (uint32_t i0 = 0; i0 < n; i0 += U * blockDim.x) {
+    uint64_t w[U][RW];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u * blockDim.x + threadIdx.x;
+      ok[u] = i < n;
+#pragma unroll
+      for (int k = 0; k < RW; ++k) w[u][k] = (ok[u] && k < rw) ? seg[(uint64_t)i * rw + k] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
+      const uint32_t p = part_of(P, w[u][0] & keymask);
+      const uint64_t pos = atomicAdd(&cur[p], 1ull);
+      DFR_GLOBAL uint64_t *d = dst + pos * rw;
+#pragma unroll
+      for (int k = 0; k < RW; ++k)
+        if (k < rw) d[k] = w[u][k];
     }
   }
-  {
-    const uint32_t n = P.cnt_par[s];
-    const DFR_GLOBAL uint64_t *seg = gp(P.partials) + (uint64_t)s * (uint64_t)P.seg_partials * (1 + W);
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-      const DFR_GLOBAL uint64_t *rec = seg + (uint64_t)i * (1 + W);
-      const uint32_t p = part_of(P, rec[0]);
-      const uint64_t pos = P.base_par[p] + P.hist_par[(uint64_t)s * NP + p] + atomicAdd(&cur[NP + p], 1u);
-      DFR_GLOBAL uint64_t *dst = gpw(P.part_par) + pos * (1 + W);
-      for (int w = 0; w < 1 + W; ++w) dst[w] = rec[w];
-    }
+}
+
+__global__ void __launch_bounds__(1024) k_part_scatter(const DevPlan *__restrict__ plan) {
+  extern __shared__ unsigned long long cur[];  // [2 * n_parts]
+  PlanRef P = plan_ref(plan);
+  const int s = blockIdx.x, NP = P.n_parts, W = P.words;
+  for (int p = threadIdx.x; p < NP; p += blockDim.x) {
+    cur[p] = P.base_raw[p] + P.hist_raw[(uint64_t)s * NP + p];
+    cur[NP + p] = P.base_par[p] + P.hist_par[(uint64_t)s * NP + p];
+  }
+  __syncthreads();
+  const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
+  for (int w = 0; w < P.seg_waves; ++w) {
+    const uint64_t wseg = (uint64_t)s * P.seg_waves + w;
+    const DFR_GLOBAL uint64_t *rseg = gp(P.spill) + wseg * (uint64_t)P.seg_rows * P.rw;
+    const uint32_t n = P.cnt_raw[wseg];
+    if (P.rw <= 4)
+      scatter_records<4, 4>(rseg, n, P.rw, keymask, P, cur, gpw(P.part_raw));
+    else if (P.rw <= 8)
+      scatter_records<8, 2>(rseg, n, P.rw, keymask, P, cur, gpw(P.part_raw));
+    else
+      scatter_records<kMaxCols + 3, 1>(rseg, n, P.rw, keymask, P, cur, gpw(P.part_raw));
+  }
+  const DFR_GLOBAL uint64_t *pseg = gp(P.partials) + (uint64_t)s * (uint64_t)P.seg_partials * (1 + W);
+  const uint32_t n = P.cnt_par[s];
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const DFR_GLOBAL uint64_t *rec = pseg + (uint64_t)i * (1 + W);
+    const uint32_t p = part_of(P, rec[0]);
+    const uint64_t pos = atomicAdd(&cur[NP + p], 1ull);
+    DFR_GLOBAL uint64_t *d = gpw(P.part_par) + pos * (1 + W);
+    for (int w = 0; w < 1 + W; ++w) d[w] = rec[w];
   }
 }
 
@@ -391,6 +458,50 @@ __device__ __forceinline__ int part_find(PlanRef P, uint64_t *lkeys, uint64_t ke
   return -1;
 }
 
+// Folds partition p's raw records, U per thread at once (loads first).
+template <int RW, int U>
+__device__ __forceinline__ void part_fold_records(PlanRef P, uint64_t *lkeys, uint64_t *lst, uint64_t keymask, int p) {
+  const int W = P.words, rw = P.rw;
+  const uint64_t b0 = P.base_raw[p], b1 = P.base_raw[p + 1];
+  const DFR_GLOBAL uint64_t *recs = gp(P.part_raw);
+  for (uint64_t i0 = b0; i0 < b1; i0 += (uint64_t)U * blockDim.x) {
+    uint64_t w[U][RW];
+    bool okr[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = i0 + (uint64_t)u * blockDim.x + threadIdx.x;
+      okr[u] = i < b1;
+#pragma unroll
+      for (int k = 0; k < RW; ++k) w[u][k] = (okr[u] && k < rw) ? recs[i * rw + k] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!okr[u]) continue;
+      const uint64_t *rec = w[u];
+      // record words picked with selects (uniform indices, no scratch)
+      auto pick = [&](int k) {
+        uint64_t v = 0;
+#pragma unroll
+        for (int j = 0; j < RW; ++j)
+          if (j == k) v = rec[j];
+        return v;
+      };
+      const uint64_t key = rec[0] & keymask;
+      const uint64_t vmask = P.vmask_in_key ? (rec[0] >> P.key_bits) : (P.rec_meta >= 0 ? pick(P.rec_meta) : ~0ull);
+      const int64_t row = P.rec_row >= 0 ? (int64_t)pick(P.rec_row) : 0;
+      auto val = [&](int c) { return pick(P.rec_val0 + P.val_of[c]); };
+      auto ok = [&](int c) { return ((vmask >> P.val_of[c]) & 1ull) != 0; };
+      const int e = part_find(P, lkeys, key);
+      if (e >= 0) {
+        fold_full(P, lst + (int64_t)e * W, row, val, ok);
+      } else {
+        const uint64_t g = global_slot(P, key);
+        if (g != kEmpty) fold_full(P, P.state + g * (uint64_t)W, row, val, ok);
+      }
+    }
+  }
+}
+
 // Pass C: partition p.
 __global__ void __launch_bounds__(1024) k_part_aggregate(const DevPlan *__restrict__ plan) {
   extern __shared__ uint64_t smem[];
@@ -404,32 +515,21 @@ __global__ void __launch_bounds__(1024) k_part_aggregate(const DevPlan *__restri
   }
   __syncthreads();
   const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
-  for (uint64_t i = P.base_raw[p] + threadIdx.x; i < P.base_raw[p + 1]; i += blockDim.x) {
-    const DFR_GLOBAL uint64_t *rec = gp(P.part_raw) + i * P.rw;
-    const uint64_t key = rec[0] & keymask;
-    const uint64_t vmask = P.vmask_in_key ? (rec[0] >> P.key_bits) : (P.rec_meta >= 0 ? rec[P.rec_meta] : ~0ull);
-    const int64_t row = P.rec_row >= 0 ? (int64_t)rec[P.rec_row] : 0;
-    auto val = [&](int c) { return rec[P.rec_val0 + P.val_of[c]]; };
-    auto ok = [&](int c) { return ((vmask >> P.val_of[c]) & 1ull) != 0; };
-    const int e = part_find(P, lkeys, key);
-    if (e >= 0) {
-      fold_full(P, lst + (int64_t)e * W, row, val, ok);
-    } else {
-      const uint64_t g = global_slot(P, key);
-      if (g != kEmpty) fold_full(P, P.state + g * (uint64_t)W, row, val, ok);
-    }
-  }
+  if (P.rw <= 4)
+    part_fold_records<4, 4>(P, lkeys, lst, keymask, p);
+  else if (P.rw <= 8)
+    part_fold_records<8, 2>(P, lkeys, lst, keymask, p);
+  else
+    part_fold_records<kMaxCols + 3, 1>(P, lkeys, lst, keymask, p);
   for (uint64_t i = P.base_par[p] + threadIdx.x; i < P.base_par[p + 1]; i += blockDim.x) {
     const DFR_GLOBAL uint64_t *rec = gp(P.part_par) + i * (1 + W);
     const uint64_t key = rec[0];
-    uint64_t part[kMaxWords];
-    for (int w = 0; w < W; ++w) part[w] = rec[1 + w];
     const int e = part_find(P, lkeys, key);
     if (e >= 0) {
-      merge_full(P, lst + (int64_t)e * W, part);
+      merge_full(P, lst + (int64_t)e * W, rec + 1);
     } else {
       const uint64_t g = global_slot(P, key);
-      if (g != kEmpty) merge_full(P, P.state + g * (uint64_t)W, part);
+      if (g != kEmpty) merge_full(P, P.state + g * (uint64_t)W, rec + 1);
     }
   }
   __syncthreads();

@@ -173,14 +173,15 @@ struct DevPlan {
   int32_t pk_log2;       // direct mode: keys per partition = 1 << pk_log2
   int32_t c_entries;     // LDS entries of a partition table (hash mode)
   int32_t c_log2;
-  int32_t n_segs;        // workgroups of pass A (one segment each)
-  int64_t seg_rows;      // record capacity of a spill segment
+  int32_t n_segs;        // workgroups of pass A
+  int32_t seg_waves;     // waves per workgroup: one spill segment each
+  int64_t seg_rows;      // record capacity of a wave's spill segment
   int64_t seg_partials;  // partial-state capacity of a segment
   uint64_t *spill;       // n_segs * seg_rows * rw
   uint64_t *partials;    // n_segs * seg_partials * (1 + words)
   uint32_t *hist_raw;    // [n_segs][n_parts]
   uint32_t *hist_par;
-  uint32_t *cnt_raw;     // [n_segs]
+  uint32_t *cnt_raw;     // [n_segs * seg_waves]
   uint32_t *cnt_par;
   uint64_t *base_raw;    // [n_parts + 1] partition starts
   uint64_t *base_par;

@@ -32,7 +32,7 @@ struct Tuning {
   int64_t hash_slots = 0;      // 0: sized from the key domain and the rows
   int agg_mode = 1;            // AGG_PARTITION (1) or AGG_ATOMIC (0)
   int part_lds_bytes = 98304;  // LDS bytes of a partition table (pass C)
-  int scan_unroll = 4;         // row pairs a scan thread loads at once (1, 2, 4)
+  int scan_unroll = 1;         // row pairs a scan thread loads at once (1, 2, 4)
 };
 
 }  // namespace dfr
