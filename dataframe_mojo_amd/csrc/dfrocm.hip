@@ -282,6 +282,14 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.part_lds_bytes = (int)value;
   else if (k == "scan_unroll" && (value == 1 || value == 2 || value == 4))
     t.scan_unroll = (int)value;
+  else if (k == "two_level" && value >= -1 && value <= 1)
+    t.two_level = (int)value;
+  else if (k == "max_parts" && value >= 1 && value <= 4096 && (value & (value - 1)) == 0)
+    t.max_parts = (int)value;
+  else if (k == "sub_lds_bytes" && value >= 4096 && value <= ctx->lds_per_block - 2048)
+    t.sub_lds_bytes = (int)value;
+  else if (k == "sub_block" && value >= 64 && value <= 512 && value % 64 == 0)
+    t.sub_block = (int)value;
   else
     return dfr_fail(error, DFR_INVALID, "bad tuning %s=%lld", key, (long long)value);
   return DFR_OK;

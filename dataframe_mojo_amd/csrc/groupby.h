@@ -17,8 +17,10 @@ struct Compiled {
   int64_t chunk = 0;
   size_t lds_bytes = 0;
   size_t c_lds_bytes = 0;  // partition tables (pass C)
+  uint64_t *scan2_raw = nullptr;  // block sums of the second level's scans
+  uint64_t *scan2_par = nullptr;
 };
 
 int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **error);
-void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots);
+void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_boost);
 int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error);

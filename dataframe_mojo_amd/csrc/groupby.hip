@@ -56,6 +56,10 @@ Tuning default_tuning(dfr_context *ctx) {
   t.lds_mode = env_int("DFR_LDS_MODE", -1);
   t.agg_mode = env_int("DFR_AGG_MODE", AGG_PARTITION);
   t.part_lds_bytes = std::min(env_int("DFR_PART_LDS_BYTES", 98304), ctx->lds_per_block);
+  t.scan_unroll = env_int("DFR_SCAN_UNROLL", 1);
+  t.two_level = env_int("DFR_TWO_LEVEL", -1);
+  t.sub_lds_bytes = std::min(env_int("DFR_SUB_LDS_BYTES", 49152), ctx->lds_per_block - 2048);
+  t.sub_block = std::max(64, std::min(512, env_int("DFR_SUB_BLOCK", 256) / 64 * 64));
   return t;
 }
 
@@ -373,7 +377,7 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
   for (int c = 0; c < P.n_cols; ++c)
     if (P.cagg[c].mask) P.val_of[c] = (int8_t)P.n_vals++;
   P.key_bits = bitlen_u64(P.domain > 0 ? P.domain - 1 : 0);
-  P.vmask_in_key = (P.key_bits + P.n_vals <= 64) ? 1 : 0;
+  P.vmask_in_key = (P.key_bits < 64 && P.key_bits + P.n_vals <= 64) ? 1 : 0;
   // the row index travels with a record when the state orders by row:
   // FIRST (maintain_order) or a first-zero word (signed-zero ties)
   bool need_row = P.has_first != 0;
@@ -388,7 +392,7 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
 }
 
 // Table mode and launch shape from the tuning and the plan.
-void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots) {
+void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_boost) {
   DevPlan &P = C.plan;
   const Tuning &T = ctx->tune;
   const int64_t rows = C.rows;
@@ -440,14 +444,37 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots) {
     // partition tables: E entries of (key + W words) within part_lds_bytes
     const int W = P.words;
     uint64_t ce = 64;
-    while (ce * 2 * 8ull * (W + 1) <= (uint64_t)T.part_lds_bytes && ce < 16384) ce *= 2;
+    // (2 KiB of LDS stay free for the second level's static arrays)
+    const uint64_t c_budget = (uint64_t)std::min(T.part_lds_bytes, ctx->lds_per_block - 2048);
+    while (ce * 2 * 8ull * (W + 1) <= c_budget && ce < (uint64_t)kMaxPartEntries) ce *= 2;
     P.c_entries = (int32_t)ce;
     P.c_log2 = log2_exact(ce);
     const uint64_t groups_est = std::max<uint64_t>(1, std::min<uint64_t>(D, (uint64_t)std::max<int64_t>(rows, 1)));
+    const uint64_t max_parts = (uint64_t)T.max_parts;
     uint64_t np = next_pow2((2 * groups_est + ce - 1) / ce);
-    np = std::min<uint64_t>(std::max<uint64_t>(np, 1), 4096);
+    np = std::min<uint64_t>(std::max<uint64_t>(np, 1), max_parts);
     P.n_parts = (int32_t)np;
     P.part_log2 = log2_exact(np);
+    // a second level when one level's tables may not hold every group; its
+    // pass C emits groups into dense tables instead of a global one. The
+    // sub-partition count is set once the first level's record totals are
+    // known (second_level_layout).
+    P.two_level = 0;
+    P.n_sub = 0;
+    P.sub_chunks = 0;
+    const bool two = T.two_level > 0 || (T.two_level < 0 && 2 * groups_est > max_parts * ce);
+    if (two) {
+      P.two_level = 1;
+      P.n_parts = (int32_t)max_parts;
+      P.part_log2 = log2_exact(max_parts);
+      // smaller tables than one level's: more workgroups per CU
+      uint64_t e2 = 64;
+      while (e2 * 2 * 8ull * (W + 1) <= (uint64_t)T.sub_lds_bytes && e2 < (uint64_t)kMaxPartEntries) e2 *= 2;
+      P.c_entries = (int32_t)e2;
+      P.c_log2 = log2_exact(e2);
+      P.mode = MODE_HASH;
+      P.cap = std::max<uint64_t>(1, std::min<uint64_t>(D, (uint64_t)std::max<int64_t>(rows, 1)));
+    }
     // the scan's LDS table (full state words) next to two histograms
     const int hist_bytes = (2 * P.n_parts + 2) * 4;
     const int avail = std::max(0, std::min(T.lds_bytes, ctx->lds_per_block) - hist_bytes);
@@ -596,9 +623,62 @@ int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan
   const size_t cur_bytes = (size_t)2 * P.n_parts * sizeof(unsigned long long);
   if ((st = set_lds(k_part_scatter, cur_bytes, error))) return st;
   hipLaunchKernelGGL(k_part_scatter, dim3(P.n_segs), dim3(1024), cur_bytes, ctx->stream, dplan);
-  if ((st = set_lds(k_part_aggregate, C.c_lds_bytes, error))) return st;
-  hipLaunchKernelGGL(k_part_aggregate, dim3(P.n_parts), dim3(512), C.c_lds_bytes, ctx->stream, dplan);
-  ctx->obs.launches += 5;
+  ctx->obs.launches += 3;
+  if (!P.two_level) {
+    if ((st = set_lds(k_part_aggregate, C.c_lds_bytes, error))) return st;
+    hipLaunchKernelGGL(k_part_aggregate, dim3(P.n_parts), dim3(512), C.c_lds_bytes, ctx->stream, dplan);
+    ctx->obs.launches += 1;
+  }
+  HIP_TRY(hipGetLastError());
+  return DFR_OK;
+}
+
+// Sub-partitions for `records` first-level records (raw + partial): about
+// half a table's entries each on average, twice as many per retry (boost).
+void second_level_layout(dfr_context *ctx, Compiled &C, uint64_t records, int boost) {
+  DevPlan &P = C.plan;
+  const uint64_t NP = (uint64_t)P.n_parts;
+  const uint64_t per_sub = std::max<uint64_t>(1, (uint64_t)P.c_entries / 2);
+  uint64_t ns = (std::max<uint64_t>(records, 1) + NP * per_sub - 1) / (NP * per_sub);
+  // (the second level's LDS cursors, 8 bytes per sub-partition, stay
+  // within 64 KiB)
+  ns = std::min<uint64_t>(std::max<uint64_t>(ns << boost, 2), 8192);
+  P.n_sub = (int32_t)ns;
+  // chunks of about 8192 records, at most 64 per partition and 2^26
+  // counters in all
+  uint64_t ch = next_pow2(std::max<uint64_t>(1, records / (NP * 8192)));
+  ch = std::min<uint64_t>(ch, 64);
+  while (ch > 1 && NP * ns * ch > (1ull << 26)) ch >>= 1;
+  P.sub_chunks = (int32_t)ch;
+  (void)ctx;
+}
+
+// The second level after the first scatter: per-chunk sub-partition counts,
+// their scans, the scatter into sub-partition order, then pass C per
+// sub-partition.
+int launch_second_level(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
+  const DevPlan &P = C.plan;
+  int st;
+  const uint64_t NS = (uint64_t)P.n_sub;
+  const uint64_t N2 = (uint64_t)P.n_parts * NS * (uint64_t)P.sub_chunks;
+  const dim3 grid2(P.sub_chunks, P.n_parts, 2);
+  if ((st = set_lds(k_sub_hist, NS * 4, error))) return st;
+  hipLaunchKernelGGL(k_sub_hist, grid2, dim3(256), NS * 4, ctx->stream, dplan);
+  const uint64_t nb = (N2 + kScanChunk - 1) / kScanChunk;
+  for (int which = 0; which < 2; ++which) {
+    uint64_t *bs = which ? C.scan2_par : C.scan2_raw;
+    const uint32_t *h = which ? P.hist2_par : P.hist2_raw;
+    uint64_t *b2 = which ? P.base2_par : P.base2_raw;
+    hipLaunchKernelGGL(k_count_blocks, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, h, N2, bs);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, ctx->stream, bs, nb);
+    hipLaunchKernelGGL(k_count_apply, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, h, N2, bs, b2);
+  }
+  if ((st = set_lds(k_sub_scatter, NS * 8, error))) return st;
+  hipLaunchKernelGGL(k_sub_scatter, grid2, dim3(256), NS * 8, ctx->stream, dplan);
+  if ((st = set_lds(k_sub_aggregate, C.c_lds_bytes, error))) return st;
+  hipLaunchKernelGGL(k_sub_aggregate, dim3((unsigned)((uint64_t)P.n_parts * NS)), dim3(ctx->tune.sub_block),
+                     C.c_lds_bytes, ctx->stream, dplan);
+  ctx->obs.launches += 9;
   HIP_TRY(hipGetLastError());
   return DFR_OK;
 }
@@ -631,11 +711,16 @@ int launch_aggregate(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, 
 
 }  // namespace
 
+// the plan, then read-backs (group count, fault flags) share the staging bytes
+static_assert(sizeof(DevPlan) + 64 + 32 <= kStagingBytes, "staging holds a plan and its read-backs");
+
 int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
   DevPlan &P = C.plan;
   int64_t hash_slots = ctx->tune.hash_slots;
+  int sub_boost = 0;
   for (int attempt = 0;; ++attempt) {
-    choose_layout(ctx, C, hash_slots);
+    choose_layout(ctx, C, hash_slots, sub_boost);
+    const bool dense = P.two_level != 0;
     Scratch S(ctx);
     int st;
     DevPlan *dplan;
@@ -655,19 +740,28 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
           (st = S.get(S_ * (uint64_t)P.seg_waves, &P.cnt_raw, error)) || (st = S.get(S_, &P.cnt_par, error)) ||
           (st = S.get(NP + 1, &P.base_raw, error)) || (st = S.get(NP + 1, &P.base_par, error)))
         return st;
+      if (dense) {
+        // second level: the spill segments are dead after the first
+        // scatter and take the sub-partition order
+        if ((st = S.get(1, &P.gcount, error))) return st;
+        P.part2_raw = P.spill;
+        P.part2_par = P.partials;
+      }
     }
     P.fault = fault;
     const uint64_t occ_words = (P.cap + 63) / 64;
-    uint64_t *occ, *occ_prefix, *order;
+    uint64_t *occ = nullptr, *occ_prefix = nullptr, *order = nullptr;
     const uint64_t max_groups = std::min<uint64_t>(P.cap, (uint64_t)std::max<int64_t>(C.rows, 1));
-    if ((st = S.get(occ_words, &occ, error)) || (st = S.get(occ_words, &occ_prefix, error)) ||
-        (st = S.get(max_groups + 1, &order, error)))
+    if (!dense && ((st = S.get(occ_words, &occ, error)) || (st = S.get(occ_words, &occ_prefix, error)) ||
+                   (st = S.get(max_groups + 1, &order, error))))
       return st;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     memcpy(ctx->staging, &P, sizeof P);
     HIP_TRY(hipMemcpyAsync(dplan, ctx->staging, sizeof P, hipMemcpyHostToDevice, ctx->stream));
     fill(ctx, (uint64_t *)fault, 2, 0);
-    {
+    if (dense) {
+      fill(ctx, P.gcount, 1, 0);  // dense tables need no set-up: pass C writes every word
+    } else {
       int blocks = (int)std::min<uint64_t>((P.cap + kBlock - 1) / kBlock, (uint64_t)ctx->cus * 8);
       hipLaunchKernelGGL(k_init_state, dim3(std::max(blocks, 1)), dim3(kBlock), 0, ctx->stream, dplan);
       ctx->obs.launches++;
@@ -676,21 +770,41 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
     if (P.agg_mode == AGG_PARTITION) {
       if (C.rows > 0 && (st = launch_partitioned(ctx, C, dplan, error))) return st;
       if (C.rows == 0) HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
+      if (dense && C.rows > 0) {
+        // the first level's record totals size the second level
+        uint64_t *tot = (uint64_t *)ctx->staging;
+        HIP_TRY(hipMemcpyAsync(tot, P.base_raw + P.n_parts, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(tot + 1, P.base_par + P.n_parts, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        ctx->obs.waits++;
+        second_level_layout(ctx, C, tot[0] + tot[1], sub_boost);
+        const uint64_t N2 = (uint64_t)P.n_parts * (uint64_t)P.n_sub * (uint64_t)P.sub_chunks;
+        const uint64_t nb = (N2 + kScanChunk - 1) / kScanChunk;
+        if ((st = S.get(N2, &P.hist2_raw, error)) || (st = S.get(N2, &P.hist2_par, error)) ||
+            (st = S.get(N2 + 1, &P.base2_raw, error)) || (st = S.get(N2 + 1, &P.base2_par, error)) ||
+            (st = S.get(nb + 1, &C.scan2_raw, error)) || (st = S.get(nb + 1, &C.scan2_par, error)))
+          return st;
+        memcpy(ctx->staging, &P, sizeof P);
+        HIP_TRY(hipMemcpyAsync(dplan, ctx->staging, sizeof P, hipMemcpyHostToDevice, ctx->stream));
+        if ((st = launch_second_level(ctx, C, dplan, error))) return st;
+      }
     } else {
       if (C.rows > 0 && (st = launch_aggregate(ctx, C, dplan, error))) return st;
       HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
     }
-    {
+    uint64_t *total;
+    if (dense) {
+      total = P.gcount;  // slots 0..G-1, in slot order
+    } else {
       uint64_t threads = occ_words * 64;
       hipLaunchKernelGGL(k_occupancy, dim3((unsigned)((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                          ctx->stream, dplan, occ, occ_words);
       ctx->obs.launches++;
+      if ((st = scan_bits(ctx, S, occ, occ_words, occ_prefix, &total, error))) return st;
+      hipLaunchKernelGGL(k_compact, dim3((unsigned)((P.cap + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                         ctx->stream, dplan, occ, occ_prefix, order);
+      ctx->obs.launches++;
     }
-    uint64_t *total;
-    if ((st = scan_bits(ctx, S, occ, occ_words, occ_prefix, &total, error))) return st;
-    hipLaunchKernelGGL(k_compact, dim3((unsigned)((P.cap + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                       ctx->stream, dplan, occ, occ_prefix, order);
-    ctx->obs.launches++;
     uint64_t *hs = (uint64_t *)ctx->staging;
     HIP_TRY(hipMemcpyAsync(hs, total, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(hs + 1, fault, 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -699,6 +813,13 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
     ctx->obs.completed = ctx->obs.launches;
     const uint64_t G = hs[0];
     const uint32_t *hf = (const uint32_t *)(hs + 1);
+    if (hf[3]) {
+      // a sub-partition outgrew its LDS table: twice the sub-partitions
+      if (sub_boost >= 6 || P.n_sub >= 8192)
+        return dfr_fail(error, DFR_CAPACITY, "group table capacity exceeded");
+      ++sub_boost;
+      continue;
+    }
     if (hf[0]) {
       // a full hash table: retry once at load <= 1/2 of every possible group
       uint64_t need = next_pow2(std::max<uint64_t>(64, 2 * std::min<uint64_t>(P.domain, (uint64_t)C.rows)));
@@ -708,7 +829,7 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
       continue;
     }
     P.groups = total;
-    P.order = order;
+    P.order = dense ? nullptr : order;
     if (C.maintain_order && G > 0) {
       const uint64_t rwords = ((uint64_t)C.rows + 63) / 64;
       uint64_t *rowbits, *rowprefix, *order2;
@@ -717,10 +838,10 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
         return st;
       fill(ctx, rowbits, rwords, 0);
       unsigned gb = (unsigned)((G + kBlock - 1) / kBlock);
-      hipLaunchKernelGGL(k_mark_first, dim3(gb), dim3(kBlock), 0, ctx->stream, dplan, order, G, rowbits);
+      hipLaunchKernelGGL(k_mark_first, dim3(gb), dim3(kBlock), 0, ctx->stream, dplan, P.order, G, rowbits);
       uint64_t *rtotal;
       if ((st = scan_bits(ctx, S, rowbits, rwords, rowprefix, &rtotal, error))) return st;
-      hipLaunchKernelGGL(k_rank_first, dim3(gb), dim3(kBlock), 0, ctx->stream, dplan, order, G, rowbits,
+      hipLaunchKernelGGL(k_rank_first, dim3(gb), dim3(kBlock), 0, ctx->stream, dplan, P.order, G, rowbits,
                          rowprefix, order2);
       ctx->obs.launches += 2;
       P.order = order2;

@@ -513,7 +513,7 @@ __global__ void k_mark_first(const DevPlan *__restrict__ plan, const uint64_t *o
   PlanRef P = plan_ref(plan);
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= groups) return;
-  uint64_t r = P.state[order[i] * (uint64_t)P.words + 1];
+  uint64_t r = P.state[(order ? order[i] : i) * (uint64_t)P.words + 1];
   atomicOr((unsigned long long *)&rowbits[r >> 6], (unsigned long long)(1ull << (r & 63)));
 }
 
@@ -523,7 +523,7 @@ __global__ void k_rank_first(const DevPlan *__restrict__ plan, const uint64_t *o
   PlanRef P = plan_ref(plan);
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= groups) return;
-  uint64_t s = order_in[i];
+  uint64_t s = order_in ? order_in[i] : i;
   uint64_t r = P.state[s * (uint64_t)P.words + 1];
   order_out[rank_of(rowbits, rowprefix, r)] = s;
 }
@@ -660,7 +660,7 @@ __global__ void k_finalize(const DevPlan *__restrict__ plan) {
   const bool active = i < G;
   const uint64_t nwords = (G + 63) >> 6;
   const bool writer = (threadIdx.x & 63) == 0 && (i >> 6) < nwords;
-  uint64_t s = active ? P.order[i] : 0;
+  uint64_t s = active ? (P.order ? P.order[i] : i) : 0;
   const uint64_t *slot = P.state + s * (uint64_t)P.words;
   uint64_t packed = 0;
   if (active) packed = (P.mode == MODE_DIRECT) ? s : P.keys_tab[s];

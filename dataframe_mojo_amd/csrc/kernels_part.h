@@ -24,6 +24,7 @@
 namespace dfr {
 
 constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+constexpr int kMaxPartEntries = 16384;  // entries of a pass C LDS table, at most
 
 __device__ __forceinline__ uint32_t key32(uint64_t key) { return (uint32_t)key ^ (uint32_t)(key >> 32); }
 
@@ -441,11 +442,12 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const DevPlan *__restrict
   }
 }
 
-// LDS probe of a partition table (first-come, linear probing); -1 if full.
+// LDS probe of a partition table (first-come, linear probing over the whole
+// table); -1 only when every entry holds another key.
 __device__ __forceinline__ int part_find(PlanRef P, uint64_t *lkeys, uint64_t key) {
   const int E = P.c_entries;
   int h = part_slot(P, key);
-  for (int p = 0; p < P.lds_probe * 4; ++p) {
+  for (int p = 0; p < E; ++p) {
     const uint64_t k = lkeys[h];
     if (k == key) return h;
     if (k == kEmpty) {
@@ -458,12 +460,13 @@ __device__ __forceinline__ int part_find(PlanRef P, uint64_t *lkeys, uint64_t ke
   return -1;
 }
 
-// Folds partition p's raw records, U per thread at once (loads first).
-template <int RW, int U>
-__device__ __forceinline__ void part_fold_records(PlanRef P, uint64_t *lkeys, uint64_t *lst, uint64_t keymask, int p) {
+// Folds the raw records [b0, b1) of one partition, U per thread at once
+// (loads first). A key the LDS table cannot place goes to the HBM table,
+// or, with DENSE (no HBM table), raises the sub-partition capacity fault.
+template <int RW, int U, bool DENSE>
+__device__ __forceinline__ void part_fold_records(PlanRef P, uint64_t *lkeys, uint64_t *lst, uint64_t keymask,
+                                                  const DFR_GLOBAL uint64_t *recs, uint64_t b0, uint64_t b1) {
   const int W = P.words, rw = P.rw;
-  const uint64_t b0 = P.base_raw[p], b1 = P.base_raw[p + 1];
-  const DFR_GLOBAL uint64_t *recs = gp(P.part_raw);
   for (uint64_t i0 = b0; i0 < b1; i0 += (uint64_t)U * blockDim.x) {
     uint64_t w[U][RW];
     bool okr[U];
@@ -494,6 +497,8 @@ __device__ __forceinline__ void part_fold_records(PlanRef P, uint64_t *lkeys, ui
       const int e = part_find(P, lkeys, key);
       if (e >= 0) {
         fold_full(P, lst + (int64_t)e * W, row, val, ok);
+      } else if (DENSE) {
+        atomicOr(&P.fault[3], 1u);
       } else {
         const uint64_t g = global_slot(P, key);
         if (g != kEmpty) fold_full(P, P.state + g * (uint64_t)W, row, val, ok);
@@ -515,12 +520,14 @@ __global__ void __launch_bounds__(1024) k_part_aggregate(const DevPlan *__restri
   }
   __syncthreads();
   const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
+  const DFR_GLOBAL uint64_t *recs = gp(P.part_raw);
+  const uint64_t b0 = P.base_raw[p], b1 = P.base_raw[p + 1];
   if (P.rw <= 4)
-    part_fold_records<4, 4>(P, lkeys, lst, keymask, p);
+    part_fold_records<4, 4, false>(P, lkeys, lst, keymask, recs, b0, b1);
   else if (P.rw <= 8)
-    part_fold_records<8, 2>(P, lkeys, lst, keymask, p);
+    part_fold_records<8, 2, false>(P, lkeys, lst, keymask, recs, b0, b1);
   else
-    part_fold_records<kMaxCols + 3, 1>(P, lkeys, lst, keymask, p);
+    part_fold_records<kMaxCols + 3, 1, false>(P, lkeys, lst, keymask, recs, b0, b1);
   for (uint64_t i = P.base_par[p] + threadIdx.x; i < P.base_par[p + 1]; i += blockDim.x) {
     const DFR_GLOBAL uint64_t *rec = gp(P.part_par) + i * (1 + W);
     const uint64_t key = rec[0];
@@ -540,6 +547,168 @@ __global__ void __launch_bounds__(1024) k_part_aggregate(const DevPlan *__restri
     const uint64_t g = global_slot(P, key);
     if (g == kEmpty) continue;
     uint64_t *d = P.state + g * (uint64_t)W;
+    const uint64_t *s = lst + (int64_t)e * W;
+    for (int w = 0; w < W; ++w) d[w] = s[w];
+  }
+}
+
+// ------------------------------------------------ second partition level
+
+// Sub-partition of a key: the hash bits below the partition's, scaled to
+// [0, n_sub) by a multiply-shift.
+__device__ __forceinline__ uint32_t sub_of(PlanRef P, uint64_t key) {
+  const uint32_t h = (key32(key) * 0x9E3779B1u) << P.part_log2;
+  return (uint32_t)(((uint64_t)h * (uint32_t)P.n_sub) >> 32);
+}
+
+// Chunk j (of nch) of partition p's records [base[p], base[p + 1]).
+__device__ __forceinline__ void chunk_range(const uint64_t *base, int p, int j, int nch, uint64_t &lo,
+                                            uint64_t &hi) {
+  const uint64_t b0 = base[p], n = base[p + 1] - b0;
+  lo = b0 + n * (uint64_t)j / (uint64_t)nch;
+  hi = b0 + n * (uint64_t)(j + 1) / (uint64_t)nch;
+}
+
+// Grid (sub_chunks, n_parts, 2): chunk x of partition y counts its records
+// per sub-partition; z = 0 raw records, z = 1 partial states.
+__global__ void __launch_bounds__(256) k_sub_hist(const DevPlan *__restrict__ plan) {
+  extern __shared__ uint32_t h2[];  // [n_sub]
+  PlanRef P = plan_ref(plan);
+  const int j = blockIdx.x, p = blockIdx.y, which = blockIdx.z;
+  const int NS = P.n_sub;
+  for (int i = threadIdx.x; i < NS; i += blockDim.x) h2[i] = 0;
+  __syncthreads();
+  const DFR_GLOBAL uint64_t *recs = gp(which ? P.part_par : P.part_raw);
+  const uint64_t rw = which ? 1 + P.words : P.rw;
+  const uint64_t keymask = (!which && P.vmask_in_key) ? ((1ull << P.key_bits) - 1) : ~0ull;
+  uint64_t lo, hi;
+  chunk_range(which ? P.base_par : P.base_raw, p, j, P.sub_chunks, lo, hi);
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+    atomicAdd(&h2[sub_of(P, recs[i * rw] & keymask)], 1u);
+  __syncthreads();
+  uint32_t *out = which ? P.hist2_par : P.hist2_raw;
+  for (int i = threadIdx.x; i < NS; i += blockDim.x)
+    out[((uint64_t)p * NS + i) * P.sub_chunks + j] = h2[i];
+}
+
+// Exclusive scan of n 32-bit counts into 64-bit offsets, out[n] = total:
+// per-block sums, k_scan_top over them, then the offsets.
+__global__ void k_count_blocks(const uint32_t *c, uint64_t n, uint64_t *block_sums) {
+  const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
+  uint64_t t = 0;
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) t += c[base + i];
+  uint64_t tot;
+  block_exclusive_scan(t, &tot);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+__global__ void k_count_apply(const uint32_t *c, uint64_t n, const uint64_t *block_sums, uint64_t *out) {
+  const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
+  uint64_t t = 0;
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) t += c[base + i];
+  uint64_t tot;
+  uint64_t run = block_sums[blockIdx.x] + block_exclusive_scan(t, &tot);
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) {
+      out[base + i] = run;
+      run += c[base + i];
+    }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = block_sums[gridDim.x];
+}
+
+// Same grid as k_sub_hist: chunk x of partition y moves its records into
+// sub-partition order, write cursors in LDS.
+__global__ void __launch_bounds__(256) k_sub_scatter(const DevPlan *__restrict__ plan) {
+  extern __shared__ unsigned long long cur2[];  // [n_sub]
+  PlanRef P = plan_ref(plan);
+  const int j = blockIdx.x, p = blockIdx.y, which = blockIdx.z;
+  const int NS = P.n_sub;
+  const uint64_t *b2 = which ? P.base2_par : P.base2_raw;
+  for (int i = threadIdx.x; i < NS; i += blockDim.x) cur2[i] = b2[((uint64_t)p * NS + i) * P.sub_chunks + j];
+  __syncthreads();
+  const DFR_GLOBAL uint64_t *src = gp(which ? P.part_par : P.part_raw);
+  DFR_GLOBAL uint64_t *dst = gpw(which ? P.part2_par : P.part2_raw);
+  const uint64_t rw = which ? 1 + P.words : P.rw;
+  const uint64_t keymask = (!which && P.vmask_in_key) ? ((1ull << P.key_bits) - 1) : ~0ull;
+  uint64_t lo, hi;
+  chunk_range(which ? P.base_par : P.base_raw, p, j, P.sub_chunks, lo, hi);
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const DFR_GLOBAL uint64_t *r = src + i * rw;
+    const uint64_t w0 = r[0];
+    const uint64_t pos = atomicAdd(&cur2[sub_of(P, w0 & keymask)], 1ull);
+    DFR_GLOBAL uint64_t *d = dst + pos * rw;
+    d[0] = w0;
+    for (uint64_t k = 1; k < rw; ++k) d[k] = r[k];
+  }
+}
+
+// Pass C over sub-partition x = p * n_sub + sub: folds its raw
+// records and merges its partial states in an LDS table, then gives each
+// group the next free slot of the dense HBM tables (one device atomic per
+// workgroup) and stores its key and state there.
+__global__ void __launch_bounds__(512) k_sub_aggregate(const DevPlan *__restrict__ plan) {
+  extern __shared__ uint64_t smem[];
+  __shared__ uint32_t wsum[kMaxPartEntries / 64];
+  __shared__ uint64_t gbase;
+  PlanRef P = plan_ref(plan);
+  const uint64_t sp = blockIdx.x;
+  const int E = P.c_entries, W = P.words, CH = P.sub_chunks;
+  uint64_t *lkeys = smem;
+  uint64_t *lst = smem + E;
+  for (int i = threadIdx.x; i < E; i += blockDim.x) {
+    lkeys[i] = kEmpty;
+    for (int w = 0; w < W; ++w) lst[i * W + w] = P.ident[w];
+  }
+  __syncthreads();
+  const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
+  const DFR_GLOBAL uint64_t *recs = gp(P.part2_raw);
+  const uint64_t b0 = P.base2_raw[sp * CH], b1 = P.base2_raw[(sp + 1) * CH];
+  if (P.rw <= 4)
+    part_fold_records<4, 4, true>(P, lkeys, lst, keymask, recs, b0, b1);
+  else if (P.rw <= 8)
+    part_fold_records<8, 2, true>(P, lkeys, lst, keymask, recs, b0, b1);
+  else
+    part_fold_records<kMaxCols + 3, 1, true>(P, lkeys, lst, keymask, recs, b0, b1);
+  const uint64_t q0 = P.base2_par[sp * CH], q1 = P.base2_par[(sp + 1) * CH];
+  for (uint64_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
+    const DFR_GLOBAL uint64_t *rec = gp(P.part2_par) + i * (1 + W);
+    const int e = part_find(P, lkeys, rec[0]);
+    if (e >= 0)
+      merge_full(P, lst + (int64_t)e * W, rec + 1);
+    else
+      atomicOr(&P.fault[3], 1u);
+  }
+  __syncthreads();
+  // dense slots: per 64-entry chunk its occupied count, a prefix over the
+  // chunks, one reservation for the workgroup
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int nchunks = E >> 6;
+  for (int c = wave; c < nchunks; c += nwaves) {
+    const uint64_t ball = __ballot(lkeys[c * 64 + lane] != kEmpty);
+    if (lane == 0) wsum[c] = (uint32_t)__popcll(ball);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int c = 0; c < nchunks; ++c) {
+      const uint32_t x = wsum[c];
+      wsum[c] = run;
+      run += x;
+    }
+    gbase = run ? atomicAdd((unsigned long long *)P.gcount, (unsigned long long)run) : 0ull;
+  }
+  __syncthreads();
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int c = wave; c < nchunks; c += nwaves) {
+    const int e = c * 64 + lane;
+    const uint64_t key = lkeys[e];
+    const uint64_t ball = __ballot(key != kEmpty);
+    if (key == kEmpty) continue;
+    const uint64_t slot = gbase + wsum[c] + (uint64_t)__popcll(ball & lt);
+    P.keys_tab[slot] = key;
+    uint64_t *d = P.state + slot * (uint64_t)W;
     const uint64_t *s = lst + (int64_t)e * W;
     for (int w = 0; w < W; ++w) d[w] = s[w];
   }

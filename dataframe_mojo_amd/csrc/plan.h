@@ -148,7 +148,8 @@ struct DevPlan {
   uint64_t domain;      // product of key radices
   uint64_t *keys_tab;   // MODE_HASH: packed key per slot
   uint64_t *state;      // cap * words
-  uint32_t *fault;      // [0] capacity, [1] Int64 sum overflow, [2] int32 sum overflow
+  uint32_t *fault;      // [0] capacity, [1] Int64 sum overflow, [2] int32 sum overflow,
+                        // [3] a sub-partition outgrew its LDS table
   uint64_t ident[kMaxWords];  // identity value of each HBM word
   int8_t lds_to_g[kMaxWords];  // HBM word of each LDS word
   DevCol cols[kMaxCols];
@@ -187,10 +188,27 @@ struct DevPlan {
   uint64_t *base_par;
   uint64_t *part_raw;    // records in partition order
   uint64_t *part_par;
+  // ---- second partition level (high cardinality; two_level != 0)
+  // Partition p splits into n_sub sub-partitions by the hash bits below the
+  // partition's. Its records are cut into sub_chunks equal chunks; hist2_*
+  // holds [p][sub][chunk] counts, base2_* their exclusive scan (+ total),
+  // and part2_* the records in sub-partition order. Pass C then runs per
+  // sub-partition and emits groups densely: slots 0..G-1 of the state and
+  // key tables, G counted in *gcount (no global hash table).
+  int32_t two_level;
+  int32_t n_sub;
+  int32_t sub_chunks;
+  uint32_t *hist2_raw;
+  uint32_t *hist2_par;
+  uint64_t *base2_raw;
+  uint64_t *base2_par;
+  uint64_t *part2_raw;
+  uint64_t *part2_par;
+  uint64_t *gcount;
   // outputs of finalize
   void *out_keys[kMaxKeys];
   uint64_t *out_key_valid[kMaxKeys];
-  uint64_t *order;      // output index -> slot
+  uint64_t *order;      // output index -> slot (null: the identity, dense tables)
   uint64_t *out_first;  // first kept row per output group (when has_first)
   uint64_t *groups;     // device scalar: number of groups
 };

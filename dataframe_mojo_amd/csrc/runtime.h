@@ -33,6 +33,10 @@ struct Tuning {
   int agg_mode = 1;            // AGG_PARTITION (1) or AGG_ATOMIC (0)
   int part_lds_bytes = 98304;  // LDS bytes of a partition table (pass C)
   int scan_unroll = 1;         // row pairs a scan thread loads at once (1, 2, 4)
+  int max_parts = 4096;        // first-level key partitions, at most (a power of two)
+  int sub_lds_bytes = 49152;   // LDS bytes of a second-level pass C table
+  int sub_block = 256;         // threads of a second-level pass C workgroup
+  int two_level = -1;          // second partition level: -1 when groups may outgrow one level, 0 never, 1 always
 };
 
 }  // namespace dfr

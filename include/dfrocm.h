@@ -110,7 +110,12 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * 2 hashed), "direct_max_slots", "hash_probe", "hash_slots", "agg_mode"
  * (1 partitioned: rows that miss the LDS table are spilled and folded per
  * key partition; 0 atomic: they fold into HBM with device atomics),
- * "part_lds_bytes" (LDS of a partition table). */
+ * "part_lds_bytes" (LDS of a partition table), "scan_unroll" (row pairs a
+ * scan thread loads at once: 1, 2 or 4), "two_level" (-1 auto: a second
+ * partition level when the groups may outgrow one level's LDS tables; 0
+ * never; 1 always), "max_parts" (first-level key partitions at most, a
+ * power of two up to 4096), "sub_lds_bytes" and "sub_block" (LDS bytes and
+ * threads of a second-level partition table's workgroup). */
 int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
 
 /* ---- device columns (the column hand-off) ------------------------------ */

@@ -174,10 +174,20 @@ def test_random_queries(seed, fi, ki):
         both(lf, exact_float=("x_min", "x_max"))
 
 
+TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26, "hash_slots": 0, "block": 512,
+                 "wg_per_cu": 2, "lds_probe": 4, "scan_unroll": 1, "agg_mode": 1, "two_level": -1,
+                 "max_parts": 4096, "part_lds_bytes": 98304, "sub_lds_bytes": 49152, "sub_block": 256}
+
+
 @pytest.mark.parametrize("setting", [
     {"lds_mode": 0}, {"lds_mode": 2}, {"lds_mode": 2, "lds_bytes": 8192}, {"direct_max_slots": 0},
     {"direct_max_slots": 0, "lds_mode": 0}, {"direct_max_slots": 0, "hash_slots": 64},
     {"block": 256, "wg_per_cu": 1}, {"block": 1024, "wg_per_cu": 4}, {"lds_probe": 1},
+    {"scan_unroll": 2}, {"scan_unroll": 4}, {"agg_mode": 0},
+    {"two_level": 1}, {"two_level": 1, "lds_mode": 0}, {"two_level": 1, "scan_unroll": 4},
+    # one first-level partition, many small second-level tables
+    {"two_level": 1, "max_parts": 1, "sub_lds_bytes": 4096, "sub_block": 64},
+    {"two_level": 1, "sub_lds_bytes": 98304, "sub_block": 512},
 ])
 def test_table_modes_give_identical_results(setting, gpu_ctx):
     df = random_frame(30000, 5, key_card=400)
@@ -188,8 +198,7 @@ def test_table_modes_give_identical_results(setting, gpu_ctx):
             gpu_ctx.tune(k, v)
         got = lf.collect()
     finally:
-        for k, v in {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26, "hash_slots": 0,
-                     "block": 512, "wg_per_cu": 2, "lds_probe": 4}.items():
+        for k, v in TUNE_DEFAULTS.items():
             gpu_ctx.tune(k, v)
     # bitwise identical: every state is order independent
     for name in base.columns:
@@ -250,3 +259,90 @@ def test_zipf_workload_prefix_matches_oracle(gpu_ctx):
     exp, first = oracle.collect(lf)
     assert_same(got, exp, True, ("x_min",))
     assert np.array_equal(res.first_rows(), first)
+
+
+@pytest.mark.parametrize("ki", range(len(KEYS)))
+def test_two_level_random_queries(ki, gpu_ctx):
+    """The second partition level (dense group tables) against the oracle,
+    ordered and unordered, on every key kind."""
+    df = random_frame(40000, 7, key_card=5000)
+    try:
+        gpu_ctx.tune("two_level", 1)
+        for mo in (True, False):
+            lf = df.lazy().filter(col("a") > -800000).group_by(*KEYS[ki], maintain_order=mo).agg(ALL_AGGS)
+            both(lf, exact_float=("x_min", "x_max"))
+    finally:
+        gpu_ctx.tune("two_level", -1)
+
+
+def _dict_frame_from_device(cols, n):
+    return DataFrame([
+        Series("k1", Column.dictionary(cols["k1"].numpy(n).astype(np.int64), datagen.dict_labels())),
+        Series("k2", Column.int64(cols["k2"].numpy(n))),
+        Series("v", Column.float64(cols["v"].numpy(n), datagen.valid_from_words(cols["v.valid"], n))),
+    ])
+
+
+@pytest.mark.parametrize("two_level", [-1, 1])
+def test_dict_workload_prefix_matches_oracle(two_level, gpu_ctx):
+    """The 500M-row dictionary workload's generator and query (two keys, a
+    modulo filter, nearly one row per group), checked against the oracle on
+    a 2M-row prefix wrapped in place, with and without the second partition
+    level."""
+    from dataframe_mojo_amd.plan import compile_plan
+    from dataframe_mojo_amd.runtime import result_frame
+
+    n = 2_000_000
+    cols = datagen.dict_500m_device(n, seed=0)
+    host = _dict_frame_from_device(cols, n)
+    aggs = [col("v").count().alias("v_count"), col("v").sum().alias("v_sum"), col("v").mean().alias("v_mean")]
+    lf = host.filter(col("k2") % 3 != 0).group_by("k1", "k2", maintain_order=True).agg(aggs)
+    dplan = compile_plan(lf)
+    dev = {
+        "k1": gpu_ctx.wrap_array(cols["k1"], labels=datagen.dict_labels(), dict_codes=True, length=n),
+        "k2": gpu_ctx.wrap_array(cols["k2"], length=n),
+        "v": gpu_ctx.wrap_array(cols["v"], cols["v.valid"], length=n),
+    }
+    try:
+        gpu_ctx.tune("two_level", two_level)
+        res = gpu_ctx.groupby(dplan, [dev[k] for k in dplan.keys], [dev[c] for c in dplan.columns])
+    finally:
+        gpu_ctx.tune("two_level", -1)
+    got = result_frame(res, dplan, [dev[k] for k in dplan.keys])
+    exp, first = oracle.collect(lf)
+    assert_same(got, exp, True, (), 2)
+    assert np.array_equal(res.first_rows(), first)
+
+
+def test_dict_workload_size_independent_properties(gpu_ctx):
+    """At 60M rows (past one partition level's capacity, so the second level
+    runs by itself): the group count equals the number of distinct kept
+    keys, the lengths sum to the kept rows, the counts to the kept valid
+    rows, and each group's key satisfies the filter."""
+    from dataframe_mojo_amd.plan import compile_plan
+
+    n = 60_000_000
+    cols = datagen.dict_500m_device(n, seed=3)
+    schema = DataFrame([Series("k1", Column.dictionary([], [])), Series("k2", Column.int64([])),
+                        Series("v", Column.float64([]))])
+    lf = schema.filter(col("k2") % 3 != 0).group_by("k1", "k2").agg(len_(), col("v").count().alias("n"))
+    dplan = compile_plan(lf)
+    dev = {
+        "k1": gpu_ctx.wrap_array(cols["k1"], labels=datagen.dict_labels(), dict_codes=True, length=n),
+        "k2": gpu_ctx.wrap_array(cols["k2"], length=n),
+        "v": gpu_ctx.wrap_array(cols["v"], cols["v.valid"], length=n),
+    }
+    res = gpu_ctx.groupby(dplan, [dev[k] for k in dplan.keys], [dev[c] for c in dplan.columns])
+    k1, k2 = cols["k1"].numpy(n).astype(np.int64), cols["k2"].numpy(n)
+    valid = datagen.valid_from_words(cols["v.valid"], n)
+    kept = (k2 % 3) != 0
+    packed = np.unique(k1[kept] * 10_000_000 + k2[kept])
+    assert res.groups == len(packed)
+    _, g1, _ = res.key(0)
+    _, g2, _ = res.key(1)
+    assert np.array_equal(np.sort(g1.astype(np.int64) * 10_000_000 + g2), packed)
+    assert (g2 % 3 != 0).all()
+    _, lens, _ = res.agg(0)
+    _, cnts, _ = res.agg(1)
+    assert int(lens.sum()) == int(kept.sum())
+    assert int(cnts.sum()) == int((kept & valid).sum())
