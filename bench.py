@@ -8,6 +8,12 @@ Workloads (columns generated on the device, resident in HBM):
   dict_500m: 500,000,000 rows — k1 dictionary String of 1000 labels, k2
     Int64 uniform [0, 1e7), v Float64 lognormal(0, 2) with 2% nulls;
     filter(k2 % 3 != 0).group_by(k1, k2).agg(count/sum/mean(v)).
+  batch_4096: 4096 independent queries, rows log-uniform in [1e3, 1e7]
+    (about 4.45e9 rows in all), key Int64 of cardinality log-uniform in
+    [1, 1e5], value Int64 uniform [-1e6, 1e6], x Float64 N(0,1) with 5%
+    nulls; filter(value > t_q).group_by(key).agg(len, sum/max(value),
+    sum/mean(x)); one dfr_groupby_batch call per step, against one
+    dfr_groupby call per query.
 A step is one whole query through the C ABI (dfr_groupby: table set-up,
 fused scan, partition passes, compaction, finish) with its inputs already on
 the device and its results left on the device. Prints one JSON line.
@@ -120,6 +126,139 @@ def cpu_baseline(spec, cols, sample: int):
     return n / min(times), min(times)
 
 
+def batch_query(threshold: int):
+    from dataframe_mojo_amd import Column, DataFrame, Series, col, len_
+
+    schema = DataFrame([
+        Series("key", Column.int64([])),
+        Series("value", Column.int64([])),
+        Series("x", Column.float64([])),
+    ])
+    return schema.filter(col("value") > threshold).group_by("key").agg(
+        len_(), col("value").sum().alias("v_sum"), col("value").max().alias("v_max"),
+        col("x").sum().alias("x_sum"), col("x").mean().alias("x_mean"))
+
+
+def bench_batch(args) -> int:
+    """batch_4096: 4096 independent filtered group-by queries (rows
+    log-uniform in [1e3, 1e7], key cardinality log-uniform in [1, 1e5],
+    threshold uniform) as one dfr_groupby_batch call per step, against the
+    same queries one dfr_groupby call each."""
+    from dataframe_mojo_amd import datagen, runtime
+    from dataframe_mojo_amd.plan import compile_plan
+    from dataframe_mojo_amd.runtime import PreparedBatch
+
+    params = datagen.batch_4096_params(args.queries, args.seed)
+    total_rows = sum(p["rows"] for p in params)
+    log(f"generating {len(params)} queries' columns on the device: {total_rows} rows")
+    cols = datagen.batch_4096_device(params, args.seed)
+    ctx = runtime.context(0)
+    if args.batch_graph:
+        ctx.tune("batch_graph", 1)
+    jobs, prepared = [], []
+    for p, c in zip(params, cols):
+        n = p["rows"]
+        dev = {"key": ctx.wrap_array(c["key"], length=n), "value": ctx.wrap_array(c["value"], length=n),
+               "x": ctx.wrap_array(c["x"], c["x.valid"], length=n)}
+        plan = compile_plan(batch_query(p["threshold"]))
+        job = (plan, [dev[k] for k in plan.keys], [dev[k] for k in plan.columns])
+        jobs.append(job)
+        prepared.append(ctx.prepare(*job))
+    batch = PreparedBatch(jobs)
+
+    def release(rs):
+        for r in rs:
+            if not isinstance(r, Exception):
+                r.release()
+
+    log("warm-up")
+    for _ in range(args.warmup):
+        release(batch.run(ctx))
+    log(f"timing {args.steps} batched steps")
+    times, dev_ms, phases = [], [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        out = batch.run(ctx)
+        times.append(time.perf_counter() - t0)
+        first = next(r for r in out if not isinstance(r, Exception))
+        dev_ms.append(first.device_ms)
+        phases.append(first.phase_ms)
+        groups = sum(r.groups for r in out if not isinstance(r, Exception))
+        release(out)
+    batch_ms = statistics.mean(times) * 1e3
+    log(f"batched: {batch_ms:.2f} ms/step (device {statistics.mean(dev_ms):.2f} ms), {groups} groups")
+
+    # plain: one launch sequence per query, the same queries, warm
+    for q in prepared[:64]:
+        q.run().release()
+    log("timing the plain one-launch-per-query pass")
+    plain = []
+    t0 = time.perf_counter()
+    for q in prepared:
+        plain.append(q.run())
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    log(f"plain: {plain_ms:.1f} ms for {len(prepared)} queries")
+
+    # identical results: every 16th query, batched vs plain, bit for bit
+    out = batch.run(ctx)
+    checked = 0
+    for i in range(0, len(jobs), 16):
+        a, b = out[i], plain[i]
+        if isinstance(a, Exception):
+            continue
+        assert a.groups == b.groups, (i, a.groups, b.groups)
+        for k in range(len(jobs[i][1])):
+            ka, kb = a.key(k)[1], b.key(k)[1]
+            oa, ob = np.argsort(ka, kind="stable"), np.argsort(kb, kind="stable")
+            assert np.array_equal(ka[oa], kb[ob]), i
+        for j in range(len(jobs[i][0].aggs)):
+            _, va, ma = a.agg(j)
+            _, vb, mb = b.agg(j)
+            assert np.array_equal(ma[oa], mb[ob]) and np.array_equal(va[oa].view(np.uint8), vb[ob].view(np.uint8)), (i, j)
+        checked += 1
+    release(out)
+    release(plain)
+    log(f"batched == plain on {checked} queries")
+
+    bytes_per_row = 8 + 8 + 8 + 1 / 8  # key, value, x, x validity
+    out_bytes = groups * (8 + 5 * 8 + 3 / 8)
+    agg_ms = statistics.mean(p[1] for p in phases) if not args.batch_graph else None
+    kernel_ms = agg_ms if agg_ms else statistics.mean(dev_ms)
+    achieved = total_rows * bytes_per_row / (kernel_ms / 1e3)
+    line = {
+        "metric": "filtered group-by rows/s (batched)",
+        "value": total_rows / (batch_ms / 1e3),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": batch_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64+f64",
+        "data": "synthetic (seeded counter-based generator, device resident)",
+        "config": {"workload": "batch_4096", "queries": len(params), "rows": total_rows, "groups": groups,
+                   "rows_per_query": "log-uniform [1e3, 1e7]", "key_cardinality": "log-uniform [1, 1e5]",
+                   "filter": "value > t, t uniform in [-1e6, 1e6]", "aggs": "len, sum/max(value), sum/mean(x)",
+                   "graph": bool(args.batch_graph)},
+        "batched_vs_plain": plain_ms / batch_ms,
+        "plain_ms": plain_ms,
+        "identical_results_checked": checked,
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_batch_aggregate" if agg_ms else "whole batch", "kernel_ms": kernel_ms,
+                     "bytes_per_row": bytes_per_row},
+        "query_hbm": {"alg_bytes": total_rows * bytes_per_row + out_bytes,
+                      "GB/s": (total_rows * bytes_per_row + out_bytes) / (batch_ms / 1e3) / 1e9},
+        "phases_ms": dict(zip(["init", "aggregate", "compact", "finish"],
+                              [statistics.mean(p[i] for p in phases) for i in range(4)])),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 WORKLOADS = {
     "zipf_200m": {
         "rows": 200_000_000, "query": zipf_query, "wrap": wrap_zipf, "host": host_zipf,
@@ -143,7 +282,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="zipf_200m", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="zipf_200m", choices=sorted([*WORKLOADS, "batch_4096"]))
+    ap.add_argument("--queries", type=int, default=4096, help="batch_4096: queries per batch")
+    ap.add_argument("--batch-graph", type=int, default=0, help="batch_4096: run the batch as a captured HIP graph")
     ap.add_argument("--rows", type=int, default=0, help="rows (default: the workload's own)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=20_000_000)
@@ -155,6 +296,9 @@ def main() -> int:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         return 2
+
+    if args.workload == "batch_4096":
+        return bench_batch(args)
 
     from dataframe_mojo_amd import datagen, runtime
     from dataframe_mojo_amd.plan import compile_plan

@@ -48,6 +48,7 @@ EXPORTS = [
     "dfr_column_length",
     "dfr_column_null_count",
     "dfr_groupby",
+    "dfr_groupby_batch",
     "dfr_result_groups",
     "dfr_result_key_dtype",
     "dfr_result_agg_dtype",
@@ -169,6 +170,8 @@ def load(required: bool = True):
     lib.dfr_column_null_count.argtypes = [vp]
     lib.dfr_column_null_count.restype = i64
     lib.dfr_groupby.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(vp), errp]
+    lib.dfr_groupby_batch.argtypes = [vp, ctypes.POINTER(Query), i32, ctypes.POINTER(vp), ctypes.POINTER(i32),
+                                      ctypes.POINTER(vp), errp]
     lib.dfr_result_groups.argtypes = [vp]
     lib.dfr_result_groups.restype = i64
     lib.dfr_result_key_dtype.argtypes = [vp, i32]
@@ -204,12 +207,17 @@ class Unsupported(DfrError):
     """Work the device path does not take; raised before submission."""
 
 
-def check(status: int, err: ctypes.c_void_p) -> None:
-    if status == DFR_OK:
-        return
+def error_of(status: int, err: ctypes.c_void_p) -> DfrError:
+    """The exception for a failed status; frees the library's message."""
     lib = load()
     msg = ctypes.string_at(err.value).decode() if err.value else f"libdfrocm status {status}"
     if err.value:
         lib.dfr_free(err)
     cls = {DFR_DATA_FAULT: DataFault, DFR_UNSUPPORTED: Unsupported}.get(status, DfrError)
-    raise cls(status, msg)
+    return cls(status, msg)
+
+
+def check(status: int, err: ctypes.c_void_p) -> None:
+    if status == DFR_OK:
+        return
+    raise error_of(status, err)

@@ -241,6 +241,7 @@ extern "C" void dfr_context_destroy(dfr_context *ctx) {
   ctx->pool.live.clear();
   ctx->pool.trim();
   if (ctx->staging) hipHostFree(ctx->staging);
+  if (ctx->bstaging) hipHostFree(ctx->bstaging);
   hipEventDestroy(ctx->ev0);
   hipEventDestroy(ctx->ev1);
   for (hipEvent_t ev : ctx->evp) hipEventDestroy(ev);
@@ -290,6 +291,10 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.sub_lds_bytes = (int)value;
   else if (k == "sub_block" && value >= 64 && value <= 512 && value % 64 == 0)
     t.sub_block = (int)value;
+  else if (k == "batch_tile" && value >= 2048 && value <= (1 << 30))
+    t.batch_tile = (int)value;
+  else if (k == "batch_graph" && (value == 0 || value == 1))
+    t.batch_graph = (int)value;
   else
     return dfr_fail(error, DFR_INVALID, "bad tuning %s=%lld", key, (long long)value);
   return DFR_OK;

@@ -683,27 +683,30 @@ int launch_second_level(dfr_context *ctx, const Compiled &C, const DevPlan *dpla
   return DFR_OK;
 }
 
+template <int LDSM, int NC>
+int launch_aggregate_nc(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
+  int st;
+  const size_t lds = LDSM == LDS_NONE ? 0 : C.lds_bytes;
+  if ((st = set_lds(k_aggregate<LDSM, NC>, lds, error))) return st;
+  hipLaunchKernelGGL((k_aggregate<LDSM, NC>), dim3(C.grid), dim3(C.block), lds, ctx->stream, dplan, (int64_t)0,
+                     C.rows, C.chunk);
+  return DFR_OK;
+}
+
+template <int LDSM>
+int launch_aggregate_mode(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
+  if (C.plan.n_cols <= 4) return launch_aggregate_nc<LDSM, 4>(ctx, C, dplan, error);
+  return launch_aggregate_nc<LDSM, kMaxCols>(ctx, C, dplan, error);
+}
+
 int launch_aggregate(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
-  const DevPlan &P = C.plan;
-  dim3 grid(C.grid), block(C.block);
-  size_t lds = C.lds_bytes;
-  switch (P.lds_mode) {
-    case LDS_DIRECT:
-      if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_aggregate<LDS_DIRECT>,
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL(k_aggregate<LDS_DIRECT>, grid, block, lds, ctx->stream, dplan, (int64_t)0,
-                         C.rows, C.chunk);
-      break;
-    case LDS_HASHED:
-      if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_aggregate<LDS_HASHED>,
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL(k_aggregate<LDS_HASHED>, grid, block, lds, ctx->stream, dplan, (int64_t)0,
-                         C.rows, C.chunk);
-      break;
-    default:
-      hipLaunchKernelGGL(k_aggregate<LDS_NONE>, grid, block, 0, ctx->stream, dplan, (int64_t)0, C.rows,
-                         C.chunk);
+  int st;
+  switch (C.plan.lds_mode) {
+    case LDS_DIRECT: st = launch_aggregate_mode<LDS_DIRECT>(ctx, C, dplan, error); break;
+    case LDS_HASHED: st = launch_aggregate_mode<LDS_HASHED>(ctx, C, dplan, error); break;
+    default: st = launch_aggregate_mode<LDS_NONE>(ctx, C, dplan, error);
   }
+  if (st) return st;
   ctx->obs.launches++;
   HIP_TRY(hipGetLastError());
   return DFR_OK;
@@ -947,3 +950,5 @@ extern "C" int dfr_groupby(dfr_context *ctx, const dfr_query *query, dfr_result 
   if (st) return st;
   return run_query(ctx, C, out, error);
 }
+
+#include "batch.h"

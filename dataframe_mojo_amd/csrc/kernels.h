@@ -206,16 +206,16 @@ __device__ __noinline__ void rare_update(PlanRef P, uint64_t packed, int gword,
 }
 
 // Folds one kept row into a group's words. `LDS` selects the ds_* form.
-template <bool LDS>
+template <bool LDS, int NC>
 __device__ __forceinline__ void fold_row(PlanRef P, uint64_t *st, int64_t row,
-                                         const uint64_t *vals, uint32_t validbits,
+                                         const uint64_t (&vals)[NC], uint32_t validbits,
                                          uint64_t packed) {
   atomicAdd((unsigned long long *)&st[0], 1ull);
   if (P.has_first) {
     if ((uint64_t)row < st[1]) atomicMin((unsigned long long *)&st[1], (unsigned long long)row);
   }
 #pragma unroll
-  for (int c = 0; c < kMaxCols; ++c) {
+  for (int c = 0; c < NC; ++c) {
     if (c >= P.n_cols) continue;
     const DevColAgg A = cget(P.cagg[c]);
     uint32_t m = A.mask;
@@ -284,9 +284,9 @@ __device__ __forceinline__ int lds_find(PlanRef P, uint64_t *lkeys, uint64_t key
   return -1;
 }
 
-template <int LDSM>
+template <int LDSM, int NC>
 __device__ __forceinline__ void fold_kept(PlanRef P, uint64_t *lkeys, uint64_t *lst,
-                                          int64_t row, const uint64_t *vals,
+                                          int64_t row, const uint64_t (&vals)[NC],
                                           uint32_t validbits) {
   uint64_t packed = 0;
 #pragma unroll
@@ -299,31 +299,33 @@ __device__ __forceinline__ void fold_kept(PlanRef P, uint64_t *lkeys, uint64_t *
   if (LDSM != LDS_NONE) {
     int e = lds_find<LDSM>(P, lkeys, packed);
     if (e >= 0) {
-      fold_row<true>(P, lst + (int64_t)e * P.lds_words, row, vals, validbits, packed);
+      fold_row<true, NC>(P, lst + (int64_t)e * P.lds_words, row, vals, validbits, packed);
       return;
     }
   }
   uint64_t g = global_slot(P, packed);
   if (g == kEmpty) return;
-  fold_row<false>(P, P.state + g * (uint64_t)P.words, row, vals, validbits, packed);
+  fold_row<false, NC>(P, P.state + g * (uint64_t)P.words, row, vals, validbits, packed);
 }
 
-template <int LDSM>
+// Rows r and r + 1 of a range ending at `end`; the plan has at most NC
+// columns.
+template <int LDSM, int NC>
 __device__ __forceinline__ void process_pair(PlanRef P, uint64_t *lkeys, uint64_t *lst,
                                              int64_t r, int64_t end) {
   if (r >= end) return;
   const bool two = r + 1 < end;
-  uint64_t v0[kMaxCols], v1[kMaxCols];
+  uint64_t v0[NC], v1[NC];
   uint32_t valid = 0;
 #pragma unroll
-  for (int c = 0; c < kMaxCols; ++c) {
+  for (int c = 0; c < NC; ++c) {
     if (c >= P.n_cols) continue;
     load_pair(cget(P.cols[c]), r, two, P.vec_ok, v0[c], v1[c]);
     valid |= load_valid_pair(cget(P.cols[c]), r, two) << (2 * c);
   }
   uint32_t va = 0, vb = 0;  // validity of row r / r+1, bit c
 #pragma unroll
-  for (int c = 0; c < kMaxCols; ++c) {
+  for (int c = 0; c < NC; ++c) {
     va |= ((valid >> (2 * c)) & 1u) << c;
     vb |= ((valid >> (2 * c + 1)) & 1u) << c;
   }
@@ -331,7 +333,7 @@ __device__ __forceinline__ void process_pair(PlanRef P, uint64_t *lkeys, uint64_
   if (P.n_leaves) {
     uint32_t m0 = 0, m1 = 0;
 #pragma unroll
-    for (int c = 0; c < kMaxCols; ++c) {
+    for (int c = 0; c < NC; ++c) {
       if (c >= P.n_cols || !((P.leaf_cols >> c) & 1)) continue;
       const bool fl = P.cols[c].is_float;
       for (int l = P.leaf_begin[c]; l < P.leaf_begin[c + 1]; ++l) {
@@ -342,8 +344,8 @@ __device__ __forceinline__ void process_pair(PlanRef P, uint64_t *lkeys, uint64_
     }
     keep &= truth_bit(P, m0) | (truth_bit(P, m1) << 1);
   }
-  if (keep & 1u) fold_kept<LDSM>(P, lkeys, lst, r, v0, va);
-  if (keep & 2u) fold_kept<LDSM>(P, lkeys, lst, r + 1, v1, vb);
+  if (keep & 1u) fold_kept<LDSM, NC>(P, lkeys, lst, r, v0, va);
+  if (keep & 2u) fold_kept<LDSM, NC>(P, lkeys, lst, r + 1, v1, vb);
 }
 
 // Merges one LDS entry into its HBM slot.
@@ -373,14 +375,10 @@ __device__ __forceinline__ void merge_entry(PlanRef P, const uint64_t *s, uint64
 
 // ------------------------------------------------------------- the scan
 
-// Each workgroup folds rows [row_begin + blockIdx.x * chunk, +chunk). chunk
-// is a multiple of 2 * blockDim.x.
-template <int LDSM>
-__global__ void __launch_bounds__(1024) k_aggregate(const DevPlan *__restrict__ plan,
-                                                    int64_t row_begin, int64_t row_end,
-                                                    int64_t chunk) {
-  extern __shared__ uint64_t smem[];
-  PlanRef P = plan_ref(plan);
+// One workgroup folds rows [b0, b1) into its LDS table, then merges the
+// table into the HBM table. The caller syncs before reusing `smem`.
+template <int LDSM, int NC>
+__device__ __forceinline__ void aggregate_tile(PlanRef P, uint64_t *smem, int64_t b0, int64_t b1) {
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + (LDSM == LDS_HASHED ? P.lds_entries : 0);
   if (LDSM != LDS_NONE) {
@@ -391,11 +389,9 @@ __global__ void __launch_bounds__(1024) k_aggregate(const DevPlan *__restrict__ 
     }
     __syncthreads();
   }
-  const int64_t b0 = row_begin + (int64_t)blockIdx.x * chunk;
-  const int64_t b1 = min(b0 + chunk, row_end);
   const int64_t step = 2 * (int64_t)blockDim.x;
   for (int64_t base = b0; base < b1; base += step)
-    process_pair<LDSM>(P, lkeys, lst, base + 2 * (int64_t)threadIdx.x, b1);
+    process_pair<LDSM, NC>(P, lkeys, lst, base + 2 * (int64_t)threadIdx.x, b1);
   if (LDSM != LDS_NONE) {
     __syncthreads();
     const int E = P.lds_entries, LW = P.lds_words;
@@ -405,6 +401,18 @@ __global__ void __launch_bounds__(1024) k_aggregate(const DevPlan *__restrict__ 
       merge_entry(P, s, LDSM == LDS_DIRECT ? (uint64_t)e : lkeys[e]);
     }
   }
+}
+
+// Each workgroup folds rows [row_begin + blockIdx.x * chunk, +chunk). chunk
+// is a multiple of 2 * blockDim.x.
+template <int LDSM, int NC>
+__global__ void __launch_bounds__(1024) k_aggregate(const DevPlan *__restrict__ plan,
+                                                    int64_t row_begin, int64_t row_end,
+                                                    int64_t chunk) {
+  extern __shared__ uint64_t smem[];
+  PlanRef P = plan_ref(plan);
+  const int64_t b0 = row_begin + (int64_t)blockIdx.x * chunk;
+  aggregate_tile<LDSM, NC>(P, smem, b0, min(b0 + chunk, row_end));
 }
 
 // ------------------------------------------------------ table set-up
@@ -653,14 +661,12 @@ __device__ __forceinline__ void store_value(void *dst, int dtype, uint64_t i, ui
     ((uint64_t *)dst)[i] = bits;
 }
 
-__global__ void k_finalize(const DevPlan *__restrict__ plan) {
-  PlanRef P = plan_ref(plan);
-  const uint64_t G = *P.groups;
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i < G;
+// Output group i (of G) from table slot s. The 64 lanes of a wave hold
+// groups 64k .. 64k + 63 of one result (inactive lanes past G), so lane 0
+// writes each validity word whole.
+__device__ __forceinline__ void finalize_group(PlanRef P, uint64_t i, uint64_t s, bool active, uint64_t G) {
   const uint64_t nwords = (G + 63) >> 6;
   const bool writer = (threadIdx.x & 63) == 0 && (i >> 6) < nwords;
-  uint64_t s = active ? (P.order ? P.order[i] : i) : 0;
   const uint64_t *slot = P.state + s * (uint64_t)P.words;
   uint64_t packed = 0;
   if (active) packed = (P.mode == MODE_DIRECT) ? s : P.keys_tab[s];
@@ -682,6 +688,15 @@ __global__ void k_finalize(const DevPlan *__restrict__ plan) {
     uint64_t ball = __ballot(active && o.valid);
     if (writer) Ga.out_valid[i >> 6] = ball;
   }
+}
+
+__global__ void k_finalize(const DevPlan *__restrict__ plan) {
+  PlanRef P = plan_ref(plan);
+  const uint64_t G = *P.groups;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < G;
+  const uint64_t s = active ? (P.order ? P.order[i] : i) : 0;
+  finalize_group(P, i, s, active, G);
 }
 
 // ------------------------------------------------------- column stats

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <memory>
 #include <vector>
 
 #include "../../include/dfrocm.h"
@@ -36,7 +37,9 @@ struct Tuning {
   int max_parts = 4096;        // first-level key partitions, at most (a power of two)
   int sub_lds_bytes = 49152;   // LDS bytes of a second-level pass C table
   int sub_block = 256;         // threads of a second-level pass C workgroup
-  int two_level = -1;          // second partition level: -1 when groups may outgrow one level, 0 never, 1 always
+  int two_level = -1;
+  int batch_tile = 131072;     // rows of a batch work item
+  int batch_graph = 0;         // 1: a batch's launch sequence runs as one captured HIP graph          // second partition level: -1 when groups may outgrow one level, 0 never, 1 always
 };
 
 }  // namespace dfr
@@ -63,6 +66,8 @@ struct dfr_context {
   hipStream_t stream = nullptr;
   Pool pool;
   char *staging = nullptr;  // pinned host bytes for plans and small read-backs
+  char *bstaging = nullptr;  // pinned host bytes of batch uploads (grown on demand)
+  size_t bstaging_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evp[4] = {nullptr, nullptr, nullptr, nullptr};  // phase boundaries
   dfr_observation obs = {};
@@ -100,6 +105,7 @@ struct dfr_result {
   std::vector<uint64_t *> key_valid, agg_valid;
   uint64_t *first_rows = nullptr;
   void *buffer = nullptr;  // one pool block holding every output
+  std::shared_ptr<void> shared;  // or a block shared by a batch's results
   size_t bytes = 0;
   double device_ms = 0.0;
   double phase_ms[4] = {0, 0, 0, 0};

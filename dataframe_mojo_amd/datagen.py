@@ -271,6 +271,33 @@ def dict_500m_device(n: int = 500_000_000, seed: int = 0) -> dict:
     }
 
 
+def batch_4096_params(n_queries: int = 4096, seed: int = 0, rows_lo: float = 1e3, rows_hi: float = 1e7) -> list:
+    """Per-query parameters of the batched workload: rows log-uniform in
+    [rows_lo, rows_hi], key cardinality log-uniform in [1, 1e5], filter
+    threshold uniform in [-1e6, 1e6] (from a seeded host generator)."""
+    rng = np.random.default_rng(seed)
+    rows = np.exp(rng.uniform(np.log(rows_lo), np.log(rows_hi), n_queries)).astype(np.int64)
+    card = np.exp(rng.uniform(0.0, np.log(1e5), n_queries)).astype(np.int64).clip(1)
+    thr = rng.integers(-1_000_000, 1_000_001, n_queries)
+    return [{"rows": int(r), "card": int(c), "threshold": int(t)} for r, c, t in zip(rows, card, thr)]
+
+
+def batch_4096_device(params: list, seed: int = 0) -> list:
+    """Device columns of every batched query (query q draws from streams
+    100 + 4q ..): key Int64 uniform in [0, card), value Int64 uniform in
+    [-1e6, 1e6], x Float64 N(0, 1) with 5% nulls."""
+    out = []
+    for q, p in enumerate(params):
+        n, base = p["rows"], 100 + 4 * q
+        out.append({
+            "key": device_uniform(n, seed, base, 0, p["card"] - 1),
+            "value": device_uniform(n, seed, base + 1, -1_000_000, 1_000_000),
+            "x": device_normal(n, seed, base + 2, 0.0, 1.0),
+            "x.valid": device_validity(n, seed, base + 3, 0.05),
+        })
+    return out
+
+
 def dict_labels(m: int = 1000) -> list[str]:
     """The 1000 category labels of the dictionary workload (code i -> label)."""
     return [f"cat-{i:04d}" for i in range(m)]

@@ -250,6 +250,69 @@ class Context:
         return Result(self, h, len(keys), len(plan.aggs))
 
 
+    def groupby_batch(self, jobs) -> list:
+        """Runs [(plan, keys, columns), ...] (or a PreparedBatch) as one
+        dfr_groupby_batch call. Returns one entry per job: a Result, or the
+        DfrError that query raised (the batch itself raises only when it
+        could not run)."""
+        prepared = jobs if isinstance(jobs, PreparedBatch) else PreparedBatch(jobs)
+        return prepared.run(self)
+
+    def prepare(self, plan: DevicePlan, keys: list, columns: list) -> "PreparedQuery":
+        """A query whose C descriptor is built once, for repeated runs."""
+        return PreparedQuery(self, plan, keys, columns)
+
+
+class PreparedQuery:
+    """One query's dfr_query, built once."""
+
+    def __init__(self, ctx: "Context", plan: DevicePlan, keys: list, columns: list):
+        self.ctx, self.plan, self.keys = ctx, plan, keys
+        self.query, self._keep = build_query(plan, keys, columns)
+        self._keep = (self._keep, keys, columns)
+
+    def run(self) -> Result:
+        h = ctypes.c_void_p()
+        e = _err()
+        with self.ctx.lock:
+            _lib.check(self.ctx.lib.dfr_groupby(self.ctx.handle, ctypes.byref(self.query), ctypes.byref(h),
+                                                ctypes.byref(e)), e)
+        return Result(self.ctx, h, len(self.keys), len(self.plan.aggs))
+
+
+class PreparedBatch:
+    """The dfr_query array of a batch, built once."""
+
+    def __init__(self, jobs: list):
+        self.jobs = list(jobs)
+        n = len(self.jobs)
+        self.queries = (_lib.Query * max(n, 1))()
+        self._keep = []
+        for i, (plan, keys, columns) in enumerate(self.jobs):
+            q, k = build_query(plan, keys, columns)
+            self.queries[i] = q
+            self._keep.append((k, keys, columns))
+
+    def run(self, ctx: "Context") -> list:
+        n = len(self.jobs)
+        if n == 0:
+            return []
+        handles = (ctypes.c_void_p * n)()
+        statuses = (ctypes.c_int32 * n)()
+        errors = (ctypes.c_void_p * n)()
+        e = _err()
+        with ctx.lock:
+            _lib.check(ctx.lib.dfr_groupby_batch(ctx.handle, self.queries, n, handles, statuses, errors,
+                                                 ctypes.byref(e)), e)
+        out = []
+        for i, (plan, keys, _) in enumerate(self.jobs):
+            if statuses[i] != _lib.DFR_OK:
+                out.append(_lib.error_of(statuses[i], ctypes.c_void_p(errors[i])))
+            else:
+                out.append(Result(ctx, ctypes.c_void_p(handles[i]), len(keys), len(plan.aggs)))
+        return out
+
+
 _contexts: dict = {}
 _ctx_lock = threading.Lock()
 

@@ -115,7 +115,9 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * partition level when the groups may outgrow one level's LDS tables; 0
  * never; 1 always), "max_parts" (first-level key partitions at most, a
  * power of two up to 4096), "sub_lds_bytes" and "sub_block" (LDS bytes and
- * threads of a second-level partition table's workgroup). */
+ * threads of a second-level partition table's workgroup), "batch_tile"
+ * (rows of a batch work item), "batch_graph" (1: a batch's launch sequence
+ * is captured and run as one HIP graph). */
 int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
 
 /* ---- device columns (the column hand-off) ------------------------------ */
@@ -229,6 +231,21 @@ typedef struct dfr_result dfr_result;
  * the device (storage/residence.mojo:33-80) until copied out. */
 int dfr_groupby(dfr_context *ctx, const dfr_query *query, dfr_result **out,
                 char **error);
+
+/* Runs n independent queries as one batch (the reference has no batch
+ * entry point; this serves callers with many small queries, for which one
+ * launch sequence per query costs more than the work). One persistent
+ * kernel drains a device work queue of (query, row tile) items, and the
+ * batch is compacted and finished together, with one host synchronisation.
+ * results[i] holds what dfr_groupby(queries[i]) returns: the same groups and
+ * values, bit for bit (group order may differ: without maintain_order it is
+ * unspecified). statuses[i] is that query's status: a query that fails to
+ * compile (DFR_UNSUPPORTED, DFR_INVALID, maintain_order queries included)
+ * or faults (DFR_DATA_FAULT) gets no result and, when `errors` is non-null,
+ * a message in errors[i] to free with dfr_free. The call's own status is
+ * DFR_OK when the batch ran. */
+int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int32_t n, dfr_result **results,
+                      int32_t *statuses, char **errors, char **error);
 
 int64_t dfr_result_groups(const dfr_result *result);
 /* DFR_INT32 / DFR_INT64 / DFR_FLOAT64 / DFR_DICT of key k or aggregate a. */
