@@ -14,6 +14,11 @@ Workloads (columns generated on the device, resident in HBM):
     nulls; filter(value > t_q).group_by(key).agg(len, sum/max(value),
     sum/mean(x)); one dfr_groupby_batch call per step, against one
     dfr_groupby call per query.
+  serve: Poisson arrivals at 2000 queries/s of that query distribution over
+    row windows of 64 device-resident tables (1e6 to 1e8 rows; Int64 and
+    dictionary String keys) through the serving loop (dfr_server_*); p50 and
+    p99 latency and queries/s, against serial one-call-at-a-time execution
+    of the same arrivals.
 A step is one whole query through the C ABI (dfr_groupby: table set-up,
 fused scan, partition passes, compaction, finish) with its inputs already on
 the device and its results left on the device. Prints one JSON line.
@@ -259,6 +264,156 @@ def bench_batch(args) -> int:
     return 0
 
 
+def bench_serve(args) -> int:
+    """serve: Poisson arrivals (args.rate queries/s for args.duration s) of
+    the batched workload's query distribution over 64 device-resident
+    tables, through the serving loop, against the same arrivals served one
+    dfr_groupby call at a time."""
+    import threading
+
+    from dataframe_mojo_amd import datagen, runtime
+    from dataframe_mojo_amd.plan import compile_plan
+    from dataframe_mojo_amd.runtime import Server
+
+    rng = np.random.default_rng(args.seed)
+    n_tables = args.tables
+    rows = np.exp(rng.uniform(np.log(1e6), np.log(args.table_rows_max), n_tables)).astype(np.int64)
+    card = np.exp(rng.uniform(0.0, np.log(1e5), n_tables)).astype(np.int64).clip(1)
+    log(f"generating {n_tables} tables on the device: {int(rows.sum())} rows")
+    ctx = runtime.context(0)
+    labels = datagen.dict_labels()
+    tabs = []
+    for t in range(n_tables):
+        n, base = int(rows[t]), 5000 + 8 * t
+        dict_key = t % 2 == 1
+        hi = min(int(card[t]), 1000) if dict_key else int(card[t])
+        key = datagen.device_uniform(n, args.seed, base, 0, hi - 1, dtype=np.uint32 if dict_key else np.int64)
+        value = datagen.device_uniform(n, args.seed, base + 1, -1_000_000, 1_000_000)
+        x = datagen.device_normal(n, args.seed, base + 2, 0.0, 1.0)
+        xv = datagen.device_validity(n, args.seed, base + 3, 0.05)
+        dev = {"key": ctx.wrap_array(key, labels=labels if dict_key else None, dict_codes=dict_key),
+               "value": ctx.wrap_array(value), "x": ctx.wrap_array(x, xv)}
+        tabs.append((n, dev, (key, value, x, xv)))
+    # the query stream: arrival times and (table, window, threshold)
+    n_q = int(args.rate * args.duration)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n_q))
+    jobs = []
+    total_rows = 0
+    for i in range(n_q):
+        t = int(rng.integers(0, n_tables))
+        n, dev, _ = tabs[t]
+        length = int(np.exp(rng.uniform(np.log(1e3), np.log(min(1e7, n)))))
+        off = int(rng.integers(0, n - length + 1))
+        thr = int(rng.integers(-1_000_000, 1_000_001))
+        plan = compile_plan(batch_query(thr))
+        sl = {c: dev[c].slice(off, length) for c in ("key", "value", "x")}
+        jobs.append(ctx.prepare(plan, [sl[k] for k in plan.keys], [sl[c] for c in plan.columns]))
+        total_rows += length
+    log(f"{n_q} queries over {total_rows} rows, {args.rate:.0f}/s for {args.duration:.1f} s")
+
+    # warm both paths
+    for j in jobs[:32]:
+        j.run().release()
+
+    def percentiles(lat):
+        return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+    # served: a submitter thread keeps the arrival schedule; waiters collect
+    lat = np.zeros(n_q)
+    with Server(ctx, lanes=args.lanes, max_batch=args.max_batch, budget=args.serve_budget) as srv:
+        tickets = [0] * n_q
+        ready = [threading.Event() for _ in range(n_q)]
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i in range(n_q):
+                d = t0 + arrivals[i] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = srv.submit(jobs[i])
+                ready[i].set()
+
+        def waiter(k, m):
+            for i in range(k, n_q, m):
+                ready[i].wait()
+                r = srv.wait(tickets[i])
+                lat[i] = r.latency_ms
+                r.release()
+
+        th = [threading.Thread(target=submitter)] + [threading.Thread(target=waiter, args=(k, 4)) for k in range(4)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        served_s = time.perf_counter() - t0
+        stats = srv.stats()
+    s50, s99 = percentiles(lat)
+    log(f"served: p50 {s50:.2f} ms p99 {s99:.2f} ms, {n_q / served_s:.0f} q/s, batches {stats['batches']}, "
+        f"max batch {stats['max_batch']}, lanes at once {stats['concurrent_max']}")
+
+    # serial plain: one dfr_groupby at a time, same arrivals (queueing included)
+    plat = np.zeros(n_q)
+    t0 = time.perf_counter()
+    busy_until = 0.0
+    for i in range(n_q):
+        now = time.perf_counter() - t0
+        if now < arrivals[i]:
+            time.sleep(arrivals[i] - now)
+        jobs[i].run().release()
+        plat[i] = (time.perf_counter() - t0 - arrivals[i]) * 1e3
+    plain_s = time.perf_counter() - t0
+    p50, p99 = percentiles(plat)
+    log(f"serial plain: p50 {p50:.2f} ms p99 {p99:.2f} ms, {n_q / plain_s:.0f} q/s")
+
+    # the same result alone and in the mix: a sample, served again in a mix
+    sample = list(range(0, n_q, max(1, n_q // 64)))
+    checked = 0
+    with Server(ctx, lanes=args.lanes, max_batch=args.max_batch) as srv:
+        ts = [srv.submit(jobs[i]) for i in sample]
+        for i, t in zip(sample, ts):
+            a = srv.wait(t)
+            b = jobs[i].run()
+            ka, kb = a.key(0)[1], b.key(0)[1]
+            oa, ob = np.argsort(ka, kind="stable"), np.argsort(kb, kind="stable")
+            assert a.groups == b.groups and np.array_equal(ka[oa], kb[ob]), i
+            for j in range(len(jobs[i].plan.aggs)):
+                _, va, ma = a.agg(j)
+                _, vb, mb = b.agg(j)
+                assert np.array_equal(ma[oa], mb[ob]) and np.array_equal(va[oa].view(np.uint8), vb[ob].view(np.uint8))
+            a.release()
+            b.release()
+            checked += 1
+    log(f"served == alone on {checked} queries")
+    line = {
+        "metric": "served filtered group-by p99 latency",
+        "value": s99,
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": n_q,
+        "warmup": 32,
+        "ms_per_step": served_s * 1e3 / n_q,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64+f64",
+        "data": "synthetic (seeded counter-based generator, device resident)",
+        "config": {"workload": "serve", "tables": n_tables, "table_rows": "log-uniform [1e6, %.0e]" % args.table_rows_max,
+                   "keys": "Int64 (cardinality log-uniform [1, 1e5]) and dictionary String, alternating",
+                   "arrivals": f"Poisson {args.rate:.0f}/s for {args.duration} s", "queries": n_q,
+                   "query": "window rows log-uniform [1e3, 1e7], value > t, len/sum/max(value), sum/mean(x)",
+                   "lanes": args.lanes, "max_batch": args.max_batch, "budget_bytes": args.serve_budget},
+        "served": {"p50_ms": s50, "p99_ms": s99, "qps": n_q / served_s, "rows_per_s": total_rows / served_s,
+                   "batches": stats["batches"], "max_batch": stats["max_batch"],
+                   "concurrent_lanes_max": stats["concurrent_max"]},
+        "serial_plain": {"p50_ms": p50, "p99_ms": p99, "qps": n_q / plain_s},
+        "identical_results_checked": checked,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 WORKLOADS = {
     "zipf_200m": {
         "rows": 200_000_000, "query": zipf_query, "wrap": wrap_zipf, "host": host_zipf,
@@ -282,7 +437,14 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="zipf_200m", choices=sorted([*WORKLOADS, "batch_4096"]))
+    ap.add_argument("--workload", default="zipf_200m", choices=sorted([*WORKLOADS, "batch_4096", "serve"]))
+    ap.add_argument("--rate", type=float, default=2000.0, help="serve: arrivals per second")
+    ap.add_argument("--duration", type=float, default=5.0, help="serve: seconds of arrivals")
+    ap.add_argument("--tables", type=int, default=64, help="serve: device-resident tables")
+    ap.add_argument("--table-rows-max", type=float, default=1e8, help="serve: largest table")
+    ap.add_argument("--lanes", type=int, default=2, help="serve: server lanes")
+    ap.add_argument("--max-batch", type=int, default=256, help="serve: queries per batch at most")
+    ap.add_argument("--serve-budget", type=int, default=0, help="serve: admission budget in bytes (0: none)")
     ap.add_argument("--queries", type=int, default=4096, help="batch_4096: queries per batch")
     ap.add_argument("--batch-graph", type=int, default=0, help="batch_4096: run the batch as a captured HIP graph")
     ap.add_argument("--rows", type=int, default=0, help="rows (default: the workload's own)")
@@ -299,6 +461,8 @@ def main() -> int:
 
     if args.workload == "batch_4096":
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     from dataframe_mojo_amd import datagen, runtime
     from dataframe_mojo_amd.plan import compile_plan

@@ -19,6 +19,7 @@ DFR_DATA_FAULT = 2
 DFR_CAPACITY = 3
 DFR_DEVICE = 4
 DFR_INVALID = 5
+DFR_PENDING = 6
 
 DFR_INT32 = 1
 DFR_INT64 = 2
@@ -49,6 +50,12 @@ EXPORTS = [
     "dfr_column_null_count",
     "dfr_groupby",
     "dfr_groupby_batch",
+    "dfr_column_slice",
+    "dfr_server_create",
+    "dfr_server_submit",
+    "dfr_server_wait",
+    "dfr_server_stats_get",
+    "dfr_server_destroy",
     "dfr_result_groups",
     "dfr_result_key_dtype",
     "dfr_result_agg_dtype",
@@ -170,6 +177,12 @@ def load(required: bool = True):
     lib.dfr_column_null_count.argtypes = [vp]
     lib.dfr_column_null_count.restype = i64
     lib.dfr_groupby.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(vp), errp]
+    lib.dfr_column_slice.argtypes = [vp, i64, i64, ctypes.POINTER(vp), errp]
+    lib.dfr_server_create.argtypes = [vp, i32, i32, i64, ctypes.POINTER(vp), errp]
+    lib.dfr_server_submit.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(i64), errp]
+    lib.dfr_server_wait.argtypes = [vp, i64, i64, ctypes.POINTER(vp), ctypes.POINTER(i64), errp]
+    lib.dfr_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    lib.dfr_server_destroy.argtypes = [vp]
     lib.dfr_groupby_batch.argtypes = [vp, ctypes.POINTER(Query), i32, ctypes.POINTER(vp), ctypes.POINTER(i32),
                                       ctypes.POINTER(vp), errp]
     lib.dfr_result_groups.argtypes = [vp]
@@ -188,6 +201,11 @@ def load(required: bool = True):
     lib.dfr_result_release.argtypes = [vp]
     _lib = lib
     return lib
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int64) for f in
+                ("submitted", "completed", "batches", "max_batch", "concurrent_max", "queued", "in_flight_bytes")]
 
 
 class DfrError(RuntimeError):

@@ -1,6 +1,7 @@
 """Builds the package's native libraries in-tree for gfx950.
 
-libdfrocm.so  the C-ABI provider library (csrc/dfrocm.hip, csrc/groupby.hip)
+libdfrocm.so  the C-ABI provider library (csrc/dfrocm.hip, csrc/groupby.hip,
+              csrc/server.hip)
 libdfrgen.so  seeded synthetic column generators (csrc/datagen.hip), used by
               the benchmark and the tests to create device-resident inputs
 
@@ -37,7 +38,7 @@ COMMON = [
 
 LIBS = {
     "libdfrocm.so": {
-        "sources": ["dfrocm.hip", "groupby.hip", "batch.hip"],
+        "sources": ["dfrocm.hip", "groupby.hip", "server.hip"],
         "extra": [f"-Wl,--version-script={os.path.join(CSRC, 'libdfrocm.map')}"],
     },
     "libdfrgen.so": {"sources": ["datagen.hip"], "extra": []},

@@ -184,8 +184,8 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
   // compile every query for the atomic table path (one pass, no spill)
   std::vector<Compiled> C(n);
   std::vector<int> live;
-  const int saved_mode = ctx->tune.agg_mode;
-  ctx->tune.agg_mode = AGG_ATOMIC;
+  Tuning atomic_tune = ctx->tune;
+  atomic_tune.agg_mode = AGG_ATOMIC;
   for (int i = 0; i < n; ++i) {
     char *msg = nullptr;
     int st = compile_query(ctx, &queries[i], C[i], &msg);
@@ -197,10 +197,9 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
       else free(msg);
       continue;
     }
-    choose_layout(ctx, C[i], ctx->tune.hash_slots, 0);
+    choose_layout(ctx, C[i], ctx->tune.hash_slots, 0, &atomic_tune);
     live.push_back(i);
   }
-  ctx->tune.agg_mode = saved_mode;
   const int m = (int)live.size();
   if (m == 0) return DFR_OK;
 

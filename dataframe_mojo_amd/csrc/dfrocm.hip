@@ -134,6 +134,7 @@ extern "C" int dfr_device_info_get(int32_t ordinal, dfr_device_info *info, char 
 // ------------------------------------------------------------ context
 
 int Pool::alloc(size_t bytes, void **out, char **error) {
+  std::lock_guard<std::mutex> g(mu);
   size_t size = round_size(bytes);
   auto it = free_.find(size);
   if (it != free_.end() && !it->second.empty()) {
@@ -165,6 +166,7 @@ int Pool::alloc(size_t bytes, void **out, char **error) {
 
 void Pool::release(void *p) {
   if (!p) return;
+  std::lock_guard<std::mutex> g(mu);
   auto it = live.find(p);
   if (it == live.end()) return;
   size_t size = it->second;
@@ -390,6 +392,28 @@ extern "C" int dfr_column_wrap(dfr_context *ctx, int32_t dtype, const void *devi
     delete col;
     return st;
   }
+  *out = col;
+  return DFR_OK;
+}
+
+extern "C" int dfr_column_slice(const dfr_column *parent, int64_t offset, int64_t length, dfr_column **out,
+                                char **error) {
+  if (!parent || !out) return dfr_fail(error, DFR_INVALID, "null argument");
+  if (offset < 0 || length < 0 || offset + length > parent->length)
+    return dfr_fail(error, DFR_INVALID, "slice [%lld, %lld) outside a column of %lld rows", (long long)offset,
+                    (long long)(offset + length), (long long)parent->length);
+  dfr_column *col = new dfr_column();
+  col->ctx = parent->ctx;
+  col->dtype = parent->dtype;
+  col->length = length;
+  col->dict_size = parent->dict_size;
+  col->values = (const char *)parent->values + offset * dtype_width(parent->dtype);
+  col->valid = parent->valid;
+  col->vbit = parent->vbit + offset;
+  // the parent's statistics bound the slice's (range, exponents); its null
+  // count is an upper bound, so a slice of a nullable column stays nullable
+  col->stats = parent->stats;
+  col->nulls = parent->nulls;
   *out = col;
   return DFR_OK;
 }

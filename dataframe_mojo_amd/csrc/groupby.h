@@ -22,5 +22,7 @@ struct Compiled {
 };
 
 int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **error);
-void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_boost);
+// `tune` overrides the context's tuning (null: the context's).
+void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_boost,
+                   const dfr::Tuning *tune = nullptr);
 int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error);

@@ -138,7 +138,7 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
   const int64_t rows = C.cols[0]->length;
   for (dfr_column *c : C.cols) {
     if (c->length != rows) return dfr_fail(error, DFR_INVALID, "columns differ in length");
-    if (c->ctx != ctx) return dfr_fail(error, DFR_INVALID, "a column belongs to another context");
+    if (c->ctx->device != ctx->device) return dfr_fail(error, DFR_INVALID, "a column lives on another device");
   }
   C.rows = rows;
   P.rows = rows;
@@ -392,9 +392,9 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
 }
 
 // Table mode and launch shape from the tuning and the plan.
-void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_boost) {
+void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_boost, const Tuning *tune) {
   DevPlan &P = C.plan;
-  const Tuning &T = ctx->tune;
+  const Tuning &T = tune ? *tune : ctx->tune;
   const int64_t rows = C.rows;
   const uint64_t D = P.domain;
   const size_t slot_bytes = (size_t)P.words * 8;

@@ -4,6 +4,7 @@
 
 #include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/dfrocm.h"
@@ -45,6 +46,7 @@ struct Tuning {
 }  // namespace dfr
 
 struct Pool {
+  std::mutex mu;  // results may be released from another thread than the context's
   int64_t budget = 0;
   size_t reserved = 0;  // bytes held from hipMalloc (live + cached)
   size_t cached = 0;

@@ -72,6 +72,14 @@ class DeviceColumn:
     def null_count(self) -> int:
         return int(self.ctx.lib.dfr_column_null_count(self.handle))
 
+    def slice(self, offset: int, length: int) -> "DeviceColumn":
+        """Rows [offset, offset + length) as a column of their own, borrowing
+        this one's buffers and statistics (no device work)."""
+        h = ctypes.c_void_p()
+        e = _err()
+        _lib.check(self.ctx.lib.dfr_column_slice(self.handle, offset, length, ctypes.byref(h), ctypes.byref(e)), e)
+        return DeviceColumn(self.ctx, h, self.dtype, length, self.labels, keepalive=self)
+
     def release(self):
         if self.handle:
             self.ctx.lib.dfr_column_release(self.handle)
@@ -311,6 +319,64 @@ class PreparedBatch:
             else:
                 out.append(Result(ctx, ctypes.c_void_p(handles[i]), len(keys), len(plan.aggs)))
         return out
+
+
+class Server:
+    """The serving loop (dfr_server_*): `lanes` worker threads with their own
+    streams take waiting queries as batches, admitted by `budget` device
+    bytes (0: no limit); submit() returns a ticket, wait() its Result."""
+
+    def __init__(self, ctx: "Context", lanes: int = 2, max_batch: int = 256, budget: int = 0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = ctypes.c_void_p()
+        e = _err()
+        _lib.check(self.lib.dfr_server_create(ctx.handle, lanes, max_batch, budget, ctypes.byref(h), ctypes.byref(e)), e)
+        self.handle = h
+        self._jobs: dict = {}
+
+    def submit(self, job) -> int:
+        """Queues a PreparedQuery (or (plan, keys, columns)); returns its ticket."""
+        pq = job if isinstance(job, PreparedQuery) else PreparedQuery(self.ctx, *job)
+        t = ctypes.c_int64()
+        e = _err()
+        _lib.check(self.lib.dfr_server_submit(self.handle, ctypes.byref(pq.query), ctypes.byref(t), ctypes.byref(e)), e)
+        self._jobs[t.value] = pq
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> Result:
+        """The ticket's Result; its `latency_ms` is the server's time from
+        submission to the end of the query's batch."""
+        h = ctypes.c_void_p()
+        lat = ctypes.c_int64()
+        e = _err()
+        st = self.lib.dfr_server_wait(self.handle, ticket, timeout_ms, ctypes.byref(h), ctypes.byref(lat),
+                                      ctypes.byref(e))
+        if st == _lib.DFR_PENDING:
+            if e.value:
+                self.lib.dfr_free(e)
+            raise TimeoutError(f"ticket {ticket} is not done")
+        pq = self._jobs.pop(ticket, None)
+        _lib.check(st, e)
+        r = Result(self.ctx, h, len(pq.keys), len(pq.plan.aggs))
+        r.latency_ms = lat.value / 1e6
+        return r
+
+    def stats(self) -> dict:
+        out = _lib.ServerStats()
+        self.lib.dfr_server_stats_get(self.handle, ctypes.byref(out))
+        return {f: getattr(out, f) for f, _ in out._fields_}
+
+    def close(self):
+        if self.handle:
+            self.lib.dfr_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 _contexts: dict = {}

@@ -48,6 +48,8 @@ extern "C" {
 #define DFR_DEVICE 4
 /* Invalid arguments (null pointers, mismatched lengths, bad indexes). */
 #define DFR_INVALID 5
+/* dfr_server_wait: the ticket's query has not finished within the timeout. */
+#define DFR_PENDING 6
 
 /* ---- column types ------------------------------------------------------ */
 #define DFR_INT32 1
@@ -143,6 +145,13 @@ int dfr_column_wrap(dfr_context *ctx, int32_t dtype, const void *device_values,
                     const uint64_t *device_validity, int64_t bit_offset,
                     int64_t dict_size, dfr_column **out, char **error);
 
+/* A window [offset, offset + length) of a column, without device work: it
+ * borrows the parent's buffers (the parent must outlive it) and takes the
+ * parent's statistics as bounds, so it costs no launch; the reference's
+ * columns are windows the same way (storage/column.mojo:18-30). A query
+ * result over a slice is the same whatever other slices run with it. */
+int dfr_column_slice(const dfr_column *parent, int64_t offset, int64_t length, dfr_column **out,
+                     char **error);
 void dfr_column_release(dfr_column *column);
 int64_t dfr_column_length(const dfr_column *column);
 int64_t dfr_column_null_count(const dfr_column *column);
@@ -246,6 +255,44 @@ int dfr_groupby(dfr_context *ctx, const dfr_query *query, dfr_result **out,
  * DFR_OK when the batch ran. */
 int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int32_t n, dfr_result **results,
                       int32_t *statuses, char **errors, char **error);
+
+/* ---- serving ------------------------------------------------------------
+ * A request loop over one device (the reference has no server; this is for
+ * callers that answer many concurrent queries over device-resident
+ * columns). `lanes` worker threads, each with its own context and stream,
+ * take waiting queries in arrival order, as many as fit `max_batch` and the
+ * device bytes (`budget_bytes`, 0: no limit) not held by queries in flight,
+ * and run them as one batch (dfr_groupby_batch). Queries that arrive while
+ * batches run wait only until a lane frees, so with two or more lanes their
+ * work overlaps on the device. A result equals what dfr_groupby gives for
+ * the query alone. The query's columns may belong to any context on the
+ * same device and must outlive the wait; results come from the lanes'
+ * contexts and must be released before dfr_server_destroy. */
+typedef struct dfr_server dfr_server;
+
+typedef struct dfr_server_stats {
+    int64_t submitted;
+    int64_t completed;
+    int64_t batches;
+    int64_t max_batch;       /* largest batch run */
+    int64_t concurrent_max;  /* most lanes running at once */
+    int64_t queued;          /* waiting for a lane now */
+    int64_t in_flight_bytes; /* admission estimate held by running batches */
+} dfr_server_stats;
+
+int dfr_server_create(dfr_context *ctx, int32_t lanes, int32_t max_batch, int64_t budget_bytes,
+                      dfr_server **out, char **error);
+/* Queues a query (copied; its columns are not) and returns its ticket. */
+int dfr_server_submit(dfr_server *server, const dfr_query *query, int64_t *ticket, char **error);
+/* Waits up to timeout_ms (< 0: no limit) for a ticket; returns the query's
+ * status (DFR_PENDING on time-out) and hands over its result and, when
+ * `latency_ns` is non-null, the time from its submission to the end of its
+ * batch. A ticket is answered once. */
+int dfr_server_wait(dfr_server *server, int64_t ticket, int64_t timeout_ms, dfr_result **out,
+                    int64_t *latency_ns, char **error);
+int dfr_server_stats_get(dfr_server *server, dfr_server_stats *out);
+/* Runs every queued query, then stops the lanes. */
+void dfr_server_destroy(dfr_server *server);
 
 int64_t dfr_result_groups(const dfr_result *result);
 /* DFR_INT32 / DFR_INT64 / DFR_FLOAT64 / DFR_DICT of key k or aggregate a. */
