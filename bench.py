@@ -318,52 +318,69 @@ def bench_serve(args) -> int:
     def percentiles(lat):
         return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
-    # served: a submitter thread keeps the arrival schedule; waiters collect
-    lat = np.zeros(n_q)
-    with Server(ctx, lanes=args.lanes, max_batch=args.max_batch, budget=args.serve_budget) as srv:
-        tickets = [0] * n_q
-        ready = [threading.Event() for _ in range(n_q)]
+    def served(arr, js):
+        """The serving loop under the arrival times `arr`: a submitter thread
+        keeps the schedule, waiter threads collect the server's latencies."""
+        nq = len(js)
+        lat = np.zeros(nq)
+        with Server(ctx, lanes=args.lanes, max_batch=args.max_batch, budget=args.serve_budget) as srv:
+            tickets = [0] * nq
+            ready = [threading.Event() for _ in range(nq)]
+            t0 = time.perf_counter()
+
+            def submitter():
+                for i in range(nq):
+                    d = t0 + arr[i] - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                    tickets[i] = srv.submit(js[i])
+                    ready[i].set()
+
+            def waiter(k, m):
+                for i in range(k, nq, m):
+                    ready[i].wait()
+                    r = srv.wait(tickets[i])
+                    lat[i] = r.latency_ms
+                    r.release()
+
+            th = [threading.Thread(target=submitter)] + [threading.Thread(target=waiter, args=(k, 4)) for k in range(4)]
+            for x in th:
+                x.start()
+            for x in th:
+                x.join()
+            secs = time.perf_counter() - t0
+            stats = srv.stats()
+        p = percentiles(lat)
+        return {"p50_ms": p[0], "p99_ms": p[1], "qps": nq / secs, "batches": stats["batches"],
+                "max_batch": stats["max_batch"], "concurrent_lanes_max": stats["concurrent_max"]}
+
+    def serial(arr, js):
+        """One dfr_groupby at a time under the same arrivals (queueing
+        included)."""
+        nq = len(js)
+        lat = np.zeros(nq)
         t0 = time.perf_counter()
+        for i in range(nq):
+            now = time.perf_counter() - t0
+            if now < arr[i]:
+                time.sleep(arr[i] - now)
+            js[i].run().release()
+            lat[i] = (time.perf_counter() - t0 - arr[i]) * 1e3
+        p = percentiles(lat)
+        return {"p50_ms": p[0], "p99_ms": p[1], "qps": nq / (time.perf_counter() - t0)}
 
-        def submitter():
-            for i in range(n_q):
-                d = t0 + arrivals[i] - time.perf_counter()
-                if d > 0:
-                    time.sleep(d)
-                tickets[i] = srv.submit(jobs[i])
-                ready[i].set()
-
-        def waiter(k, m):
-            for i in range(k, n_q, m):
-                ready[i].wait()
-                r = srv.wait(tickets[i])
-                lat[i] = r.latency_ms
-                r.release()
-
-        th = [threading.Thread(target=submitter)] + [threading.Thread(target=waiter, args=(k, 4)) for k in range(4)]
-        for x in th:
-            x.start()
-        for x in th:
-            x.join()
-        served_s = time.perf_counter() - t0
-        stats = srv.stats()
-    s50, s99 = percentiles(lat)
-    log(f"served: p50 {s50:.2f} ms p99 {s99:.2f} ms, {n_q / served_s:.0f} q/s, batches {stats['batches']}, "
-        f"max batch {stats['max_batch']}, lanes at once {stats['concurrent_max']}")
-
-    # serial plain: one dfr_groupby at a time, same arrivals (queueing included)
-    plat = np.zeros(n_q)
-    t0 = time.perf_counter()
-    busy_until = 0.0
-    for i in range(n_q):
-        now = time.perf_counter() - t0
-        if now < arrivals[i]:
-            time.sleep(arrivals[i] - now)
-        jobs[i].run().release()
-        plat[i] = (time.perf_counter() - t0 - arrivals[i]) * 1e3
-    plain_s = time.perf_counter() - t0
-    p50, p99 = percentiles(plat)
-    log(f"serial plain: p50 {p50:.2f} ms p99 {p99:.2f} ms, {n_q / plain_s:.0f} q/s")
+    srv_res = served(arrivals, jobs)
+    s50, s99 = srv_res["p50_ms"], srv_res["p99_ms"]
+    served_s = n_q / srv_res["qps"]
+    log(f"served: {srv_res}")
+    plain_res = serial(arrivals, jobs)
+    log(f"serial plain: {plain_res}")
+    # overload: the same queries arriving `overload` times faster
+    over = {}
+    if args.overload > 1:
+        arr2 = arrivals / args.overload
+        over = {"rate": args.rate * args.overload, "served": served(arr2, jobs), "serial_plain": serial(arr2, jobs)}
+        log(f"overload x{args.overload}: {over}")
 
     # the same result alone and in the mix: a sample, served again in a mix
     sample = list(range(0, n_q, max(1, n_q // 64)))
@@ -402,10 +419,9 @@ def bench_serve(args) -> int:
                    "arrivals": f"Poisson {args.rate:.0f}/s for {args.duration} s", "queries": n_q,
                    "query": "window rows log-uniform [1e3, 1e7], value > t, len/sum/max(value), sum/mean(x)",
                    "lanes": args.lanes, "max_batch": args.max_batch, "budget_bytes": args.serve_budget},
-        "served": {"p50_ms": s50, "p99_ms": s99, "qps": n_q / served_s, "rows_per_s": total_rows / served_s,
-                   "batches": stats["batches"], "max_batch": stats["max_batch"],
-                   "concurrent_lanes_max": stats["concurrent_max"]},
-        "serial_plain": {"p50_ms": p50, "p99_ms": p99, "qps": n_q / plain_s},
+        "served": {**srv_res, "rows_per_s": total_rows / served_s},
+        "serial_plain": plain_res,
+        "overload": over,
         "identical_results_checked": checked,
         "roofline": None,
         "cpu_baseline": None,
@@ -445,6 +461,7 @@ def main() -> int:
     ap.add_argument("--lanes", type=int, default=2, help="serve: server lanes")
     ap.add_argument("--max-batch", type=int, default=256, help="serve: queries per batch at most")
     ap.add_argument("--serve-budget", type=int, default=0, help="serve: admission budget in bytes (0: none)")
+    ap.add_argument("--overload", type=float, default=4.0, help="serve: also run the arrivals this many times faster")
     ap.add_argument("--queries", type=int, default=4096, help="batch_4096: queries per batch")
     ap.add_argument("--batch-graph", type=int, default=0, help="batch_4096: run the batch as a captured HIP graph")
     ap.add_argument("--rows", type=int, default=0, help="rows (default: the workload's own)")

@@ -91,6 +91,8 @@ class Observation(ctypes.Structure):
             "peak_reserved_bytes",
             "waits",
             "queries",
+            "jit_compiles",
+            "jit_launches",
         )
     ]
 

@@ -1,7 +1,8 @@
 """Builds the package's native libraries in-tree for gfx950.
 
 libdfrocm.so  the C-ABI provider library (csrc/dfrocm.hip, csrc/groupby.hip,
-              csrc/server.hip)
+              csrc/server.hip, csrc/jit.hip; links hiprtc, which compiles pass A
+              for a query's shape at run time from the headers in csrc/)
 libdfrgen.so  seeded synthetic column generators (csrc/datagen.hip), used by
               the benchmark and the tests to create device-resident inputs
 
@@ -38,8 +39,8 @@ COMMON = [
 
 LIBS = {
     "libdfrocm.so": {
-        "sources": ["dfrocm.hip", "groupby.hip", "server.hip"],
-        "extra": [f"-Wl,--version-script={os.path.join(CSRC, 'libdfrocm.map')}"],
+        "sources": ["dfrocm.hip", "groupby.hip", "server.hip", "jit.hip"],
+        "extra": [f"-Wl,--version-script={os.path.join(CSRC, 'libdfrocm.map')}", "-lhiprtc", "-ldl"],
     },
     "libdfrgen.so": {"sources": ["datagen.hip"], "extra": []},
 }

@@ -1,8 +1,20 @@
 // device_util.h — integer, fixed-point and float-ordering helpers shared by
 // the kernels (and, where marked __host__, by the host finish code).
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#else
+// hiprtc declares the fixed-width integers in its own namespace
+using __hip_internal::int8_t;
+using __hip_internal::int16_t;
+using __hip_internal::int32_t;
+using __hip_internal::int64_t;
+using __hip_internal::uint8_t;
+using __hip_internal::uint16_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
+#endif
 
 namespace dfr {
 

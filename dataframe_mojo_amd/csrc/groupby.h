@@ -26,3 +26,6 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
 void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_boost,
                    const dfr::Tuning *tune = nullptr);
 int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error);
+
+// Pass A compiled at run time for the plan's shape (jit.hip), or null.
+hipFunction_t jit_scan_function(dfr_context *ctx, const dfr::DevPlan &P, int LDSM, int NC, int U);

@@ -60,6 +60,7 @@ Tuning default_tuning(dfr_context *ctx) {
   t.two_level = env_int("DFR_TWO_LEVEL", -1);
   t.sub_lds_bytes = std::min(env_int("DFR_SUB_LDS_BYTES", 49152), ctx->lds_per_block - 2048);
   t.sub_block = std::max(64, std::min(512, env_int("DFR_SUB_BLOCK", 256) / 64 * 64));
+  t.jit = env_int("DFR_JIT", 1);
   return t;
 }
 
@@ -583,6 +584,20 @@ int set_lds(K kernel, size_t bytes, char **error) {
 template <int LDSM, int NC, int U>
 int launch_scan(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
   int st;
+  const Tuning &T = ctx->tune;
+  if ((T.jit == 2 || (T.jit == 1 && C.rows >= T.jit_rows)) && C.lds_bytes <= 65536) {
+    // the query's shape compiled at run time; the precompiled kernel
+    // otherwise
+    hipFunction_t fn = jit_scan_function(ctx, C.plan, LDSM, NC, U);
+    if (fn) {
+      int64_t chunk = C.chunk;
+      void *args[] = {(void *)&dplan, (void *)&chunk};
+      HIP_TRY(hipModuleLaunchKernel(fn, C.grid, 1, 1, C.block, 1, 1, (unsigned)C.lds_bytes, ctx->stream, args,
+                                    nullptr));
+      ctx->obs.jit_launches++;
+      return DFR_OK;
+    }
+  }
   if ((st = set_lds(k_scan_spill<LDSM, NC, U>, C.lds_bytes, error))) return st;
   hipLaunchKernelGGL((k_scan_spill<LDSM, NC, U>), dim3(C.grid), dim3(C.block), C.lds_bytes, ctx->stream, dplan,
                      C.chunk);

@@ -1,0 +1,162 @@
+// jit.hip — pass A compiled at run time (hiprtc) for one query's shape.
+//
+// The precompiled pass A reads every plan field from constant memory: the
+// column types, which aggregates each column feeds, the state-word offsets
+// and the predicate's structure are all run-time values there, and the
+// kernel carries the branches and registers for every case. This file
+// generates a kernel that builds a register copy of the plan in which those
+// shape fields are literals, so the compiler folds them away; the fields
+// that vary with the data or the query's constants (column pointers, key
+// ranges, literals, fixed-point scales, table sizes) stay run-time values,
+// so one compiled kernel serves every query of the same shape. Kernels are
+// cached per device and shape for the life of the process.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <dlfcn.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/dfrocm.h"
+#include "groupby.h"
+#include "runtime.h"
+
+using namespace dfr;
+
+namespace {
+
+struct JitEntry {
+  hipModule_t module = nullptr;
+  hipFunction_t fn = nullptr;
+  bool failed = false;
+};
+
+std::mutex g_mu;
+std::unordered_map<std::string, JitEntry> g_cache;
+
+// The directory of the kernel headers: csrc/ beside this library.
+std::string header_dir() {
+  Dl_info info;
+  if (!dladdr((const void *)&header_dir, &info) || !info.dli_fname) return "";
+  std::string path = info.dli_fname;
+  const size_t slash = path.rfind('/');
+  return (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/csrc";
+}
+
+// The kernel source for this plan's shape.
+std::string shape_source(const DevPlan &P, int LDSM, int NC, int U) {
+  std::ostringstream o;
+  o << "#define DFR_JIT 1\n#include \"kernels_part.h\"\nusing namespace dfr;\n"
+    << "extern \"C\" __global__ void __launch_bounds__(1024) dfr_scan_jit(const DevPlan *__restrict__ plan, "
+       "int64_t chunk) {\n"
+    << "  PlanRef R = plan_ref(plan);\n"
+    << "  DevPlan L{};\n";
+  // values of the data and of the query's constants: from the plan
+  o << "  L.rows = R.rows; L.lds_entries = R.lds_entries; L.lds_log2 = R.lds_log2;\n"
+    << "  L.n_parts = R.n_parts; L.part_log2 = R.part_log2; L.key_bits = R.key_bits;\n"
+    << "  L.spill = R.spill; L.partials = R.partials; L.seg_rows = R.seg_rows; L.seg_partials = R.seg_partials;\n"
+    << "  L.hist_raw = R.hist_raw; L.hist_par = R.hist_par; L.cnt_raw = R.cnt_raw; L.cnt_par = R.cnt_par;\n";
+  for (int c = 0; c < P.n_cols; ++c)
+    o << "  L.cols[" << c << "].values = R.cols[" << c << "].values; L.cols[" << c << "].valid = R.cols[" << c
+      << "].valid; L.cols[" << c << "].vbit = R.cols[" << c << "].vbit;\n"
+      << "  L.cagg[" << c << "].fshift = R.cagg[" << c << "].fshift; L.cagg[" << c << "].fshift2 = R.cagg[" << c
+      << "].fshift2;\n";
+  for (int k = 0; k < P.n_keys; ++k)
+    o << "  L.keys[" << k << "].minv = R.keys[" << k << "].minv; L.keys[" << k << "].radix = R.keys[" << k
+      << "].radix; L.keys[" << k << "].stride = R.keys[" << k << "].stride;\n";
+  for (int l = 0; l < P.n_leaves; ++l)
+    o << "  L.leaves[" << l << "].lit = R.leaves[" << l << "].lit; L.leaves[" << l << "].mod = R.leaves[" << l
+      << "].mod;\n";
+  // the query's shape: literals
+  o << "  L.n_cols = " << P.n_cols << "; L.n_keys = " << P.n_keys << "; L.n_leaves = " << P.n_leaves
+    << "; L.words = " << P.words << "; L.lds_words = " << P.lds_words << "; L.lds_mode = " << P.lds_mode
+    << "; L.has_first = " << P.has_first << "; L.vec_ok = " << P.vec_ok << "; L.leaf_cols = " << P.leaf_cols
+    << ";\n";
+  o << "  L.rw = " << P.rw << "; L.n_vals = " << P.n_vals << "; L.rec_row = " << P.rec_row
+    << "; L.rec_meta = " << P.rec_meta << "; L.rec_val0 = " << P.rec_val0 << "; L.vmask_in_key = " << P.vmask_in_key
+    << ";\n";
+  for (int i = 0; i < 8; ++i) o << "  L.truth[" << i << "] = " << P.truth[i] << "u;\n";
+  for (int c = 0; c < P.n_cols; ++c) {
+    const DevColAgg &A = P.cagg[c];
+    o << "  L.cols[" << c << "].dtype = " << P.cols[c].dtype << "; L.cols[" << c << "].is_float = " << P.cols[c].is_float
+      << "; L.val_of[" << c << "] = " << (int)P.val_of[c] << ";\n"
+      << "  L.cagg[" << c << "].mask = " << A.mask << "u; L.cagg[" << c << "].gbase = " << A.gbase << "; L.cagg[" << c
+      << "].lbase = " << A.lbase << "; L.cagg[" << c << "].int_as_float = " << A.int_as_float << ";\n"
+      << "  L.cagg[" << c << "].o_count = " << (int)A.o_count << "; L.cagg[" << c << "].o_isum = " << (int)A.o_isum
+      << "; L.cagg[" << c << "].o_fsum = " << (int)A.o_fsum << "; L.cagg[" << c << "].o_sq = " << (int)A.o_sq
+      << "; L.cagg[" << c << "].o_min = " << (int)A.o_min << "; L.cagg[" << c << "].o_max = " << (int)A.o_max
+      << "; L.cagg[" << c << "].o_flags = " << (int)A.o_flags << "; L.cagg[" << c << "].o_zfirst = " << (int)A.o_zfirst
+      << ";\n";
+  }
+  for (int c = 0; c <= P.n_cols; ++c) o << "  L.leaf_begin[" << c << "] = " << (int)P.leaf_begin[c] << ";\n";
+  for (int k = 0; k < P.n_keys; ++k) o << "  L.keys[" << k << "].nullable = " << P.keys[k].nullable << ";\n";
+  for (int l = 0; l < P.n_leaves; ++l)
+    o << "  L.leaves[" << l << "].op = " << P.leaves[l].op << "; L.leaves[" << l << "].flags = " << P.leaves[l].flags
+      << "; L.leaves[" << l << "].bit = " << P.leaves[l].bit << ";\n";
+  for (int w = 0; w < P.words; ++w) o << "  L.ident[" << w << "] = " << P.ident[w] << "ull;\n";
+  o << "  scan_spill_body<" << LDSM << ", " << NC << ", " << U << ">(L, chunk);\n}\n";
+  return o.str();
+}
+
+JitEntry compile(const std::string &src, int device) {
+  JitEntry e;
+  e.failed = true;
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "dfr_scan_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) return e;
+  hipDeviceProp_t prop;
+  std::string arch = "--offload-arch=gfx950";
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    std::string name = prop.gcnArchName;
+    const size_t colon = name.find(':');
+    arch = "--offload-arch=" + name.substr(0, colon);
+  }
+  const std::string inc = "-I" + header_dir();
+  const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", inc.c_str(), "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops"};
+  const hiprtcResult r = hiprtcCompileProgram(prog, 8, opts);
+  size_t code_size = 0;
+  if (r == HIPRTC_SUCCESS && hiprtcGetCodeSize(prog, &code_size) == HIPRTC_SUCCESS && code_size > 0) {
+    std::vector<char> code(code_size);
+    if (hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS &&
+        hipModuleLoadData(&e.module, code.data()) == hipSuccess &&
+        hipModuleGetFunction(&e.fn, e.module, "dfr_scan_jit") == hipSuccess)
+      e.failed = false;
+  }
+  hiprtcDestroyProgram(&prog);
+  return e;
+}
+
+}  // namespace
+
+// The runtime-compiled pass A for this plan's shape on the context's device,
+// or null when it cannot be built here (then the precompiled kernel runs).
+hipFunction_t jit_scan_function(dfr_context *ctx, const DevPlan &P, int LDSM, int NC, int U) {
+  const std::string src = shape_source(P, LDSM, NC, U);
+  const std::string key = std::to_string(ctx->device) + "\n" + src;
+  std::lock_guard<std::mutex> g(g_mu);
+  auto it = g_cache.find(key);
+  if (it == g_cache.end()) {
+    hipSetDevice(ctx->device);
+    if (const char *dir = getenv("DFR_JIT_DUMP")) {
+      // the generated source, for inspection
+      const std::string path = std::string(dir) + "/scan_" + std::to_string(std::hash<std::string>()(src)) + ".hip";
+      if (FILE *f = fopen(path.c_str(), "w")) {
+        fputs(src.c_str(), f);
+        fclose(f);
+      }
+    }
+    it = g_cache.emplace(key, compile(src, ctx->device)).first;
+    ctx->obs.jit_compiles++;
+  }
+  return it->second.failed ? nullptr : it->second.fn;
+}
+
+// The generated source of a plan's pass A (for tests and inspection).
+std::string jit_scan_source(const DevPlan &P, int LDSM, int NC, int U) { return shape_source(P, LDSM, NC, U); }

@@ -17,11 +17,20 @@
 //                    ranking first rows in a row bitmap (no sort)
 //   k_finalize       keys and aggregate results, validity as packed bits
 #pragma once
+// DFR_JIT: this header is being compiled at run time (hiprtc) for one
+// query's pass A, which needs only the row helpers below.
+#ifndef DFR_JIT
+#define DFR_JIT 0
+#endif
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 
 #include "device_util.h"
 #include "plan.h"
+#if !DFR_JIT
 #include "runtime.h"
+#endif
 
 namespace dfr {
 
@@ -43,6 +52,12 @@ __device__ __forceinline__ T cget(const DFR_CONSTANT T &x) {
 #pragma unroll
   for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
   return out;
+}
+
+// A plan record of a register copy of the plan (runtime-compiled pass A).
+template <class T>
+__device__ __forceinline__ T cget(const T &x) {
+  return x;
 }
 
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
@@ -163,6 +178,7 @@ __device__ __forceinline__ uint32_t truth_bit(PlanRef P, uint32_t m) {
 }
 
 // HBM slot of a packed key (inserting it in the hash table when new).
+#if !DFR_JIT
 __device__ __noinline__ uint64_t global_slot_hash(PlanRef P, uint64_t key) {
   uint64_t mask = P.cap - 1;
   uint64_t i = hash_index(key, P.cap_log2);
@@ -186,6 +202,8 @@ __device__ __forceinline__ uint64_t global_slot(PlanRef P, uint64_t key) {
   return global_slot_hash(P, key);
 }
 
+#endif  // !DFR_JIT
+
 __device__ __forceinline__ void add128(uint64_t *p, uint64_t lo, uint64_t hi) {
   unsigned long long old = atomicAdd((unsigned long long *)p, (unsigned long long)lo);
   uint64_t carry = ((uint64_t)old + lo) < (uint64_t)old ? 1 : 0;
@@ -193,6 +211,7 @@ __device__ __forceinline__ void add128(uint64_t *p, uint64_t lo, uint64_t hi) {
   if (h) atomicAdd((unsigned long long *)(p + 1), (unsigned long long)h);
 }
 
+#if !DFR_JIT
 // The rare-event words (FLAGS, ZFIRST) live in HBM only.
 __device__ __noinline__ void rare_update(PlanRef P, uint64_t packed, int gword,
                                          uint64_t value, bool is_min) {
@@ -754,5 +773,7 @@ __global__ void k_col_stats(DevCol C, int64_t n, ColStats *out) {
     atomicAdd((unsigned long long *)&out->valid, (unsigned long long)cnt);
   }
 }
+
+#endif  // !DFR_JIT
 
 }  // namespace dfr

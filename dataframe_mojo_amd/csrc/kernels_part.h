@@ -28,7 +28,20 @@ constexpr int kMaxPartEntries = 16384;  // entries of a pass C LDS table, at mos
 
 __device__ __forceinline__ uint32_t key32(uint64_t key) { return (uint32_t)key ^ (uint32_t)(key >> 32); }
 
-__device__ __forceinline__ uint32_t part_of(PlanRef P, uint64_t key) {
+// A register copy of the plan (the runtime-compiled pass A) is indexed only
+// at compile-time positions, so it stays in registers: loops over its arrays
+// are unrolled to their maximum length there.
+template <class PV>
+struct RegPlan {
+  static constexpr bool value = false;
+};
+template <>
+struct RegPlan<DevPlan> {
+  static constexpr bool value = true;
+};
+
+template <class PV>
+__device__ __forceinline__ uint32_t part_of(const PV &P, uint64_t key) {
   return P.part_log2 ? ((key32(key) * 0x9E3779B1u) >> (32 - P.part_log2)) : 0u;
 }
 
@@ -39,8 +52,8 @@ __device__ __forceinline__ int part_slot(PlanRef P, uint64_t key) {
 
 // Folds one kept row into a full-layout state (every word, FLAGS and ZFIRST
 // included). `val(c)` and `ok(c)` give column c's value and validity.
-template <int NC = kMaxCols, class Val, class Ok>
-__device__ __forceinline__ void fold_full(PlanRef P, uint64_t *st, int64_t row, Val val, Ok ok) {
+template <int NC = kMaxCols, class Val, class Ok, class PV>
+__device__ __forceinline__ void fold_full(const PV &P, uint64_t *st, int64_t row, Val val, Ok ok) {
   atomicAdd((unsigned long long *)&st[0], 1ull);
   if (P.has_first) atomicMin((unsigned long long *)&st[1], (unsigned long long)row);
 #pragma unroll
@@ -138,8 +151,8 @@ __device__ __forceinline__ uint32_t wave_append(bool take, uint32_t *counter) {
 // Scan LDS table: buckets of 4 key slots, read together; a key goes to the
 // first free slot of its bucket (first come), and a row whose bucket is
 // full spills. One LDS round trip per lookup, hit or miss.
-template <int LDSM>
-__device__ __forceinline__ int bucket_find(PlanRef P, uint64_t *lkeys, uint64_t key) {
+template <int LDSM, class PV>
+__device__ __forceinline__ int bucket_find(const PV &P, uint64_t *lkeys, uint64_t key) {
   if (LDSM == LDS_DIRECT) return (int)key;
   const int b = (int)((key32(key) * 0x9E3779B1u) >> (32 - (P.lds_log2 - 2)));
   uint64_t *kb = lkeys + 4 * b;
@@ -161,7 +174,8 @@ __device__ __forceinline__ int bucket_find(PlanRef P, uint64_t *lkeys, uint64_t 
 
 // Whether truth-table index m keeps its row. The truth words are uniform;
 // up to five leaves need only the first.
-__device__ __forceinline__ bool truth_keeps(PlanRef P, uint32_t m) {
+template <class PV>
+__device__ __forceinline__ bool truth_keeps(const PV &P, uint32_t m) {
   uint32_t w = P.truth[0];
   if (P.n_leaves > 5) {
     const uint32_t q = m >> 5;
@@ -174,8 +188,8 @@ __device__ __forceinline__ bool truth_keeps(PlanRef P, uint32_t m) {
 
 // One kept-or-not row of pass A: predicate already applied (`kept`); folds
 // into the LDS table or spills a record. Every lane of the wave calls it.
-template <int LDSM, int NC>
-__device__ __forceinline__ void spill_row(PlanRef P, uint64_t *lkeys, uint64_t *lst, uint64_t *seg,
+template <int LDSM, int NC, class PV>
+__device__ __forceinline__ void spill_row(const PV &P, uint64_t *lkeys, uint64_t *lst, uint64_t *seg,
                                           uint32_t &wcur, uint32_t *hraw, bool kept, int64_t row,
                                           const uint64_t (&vals)[NC], uint32_t vbits) {
   uint64_t packed = 0;
@@ -220,8 +234,8 @@ __device__ __forceinline__ void spill_row(PlanRef P, uint64_t *lkeys, uint64_t *
 
 // U row pairs per thread: every load of the U pairs is issued before any of
 // them is processed, so each wave keeps U * NC * 16 bytes in flight.
-template <int LDSM, int NC, int U>
-__device__ __forceinline__ void spill_tile(PlanRef P, uint64_t *lkeys, uint64_t *lst, uint64_t *seg,
+template <int LDSM, int NC, int U, class PV>
+__device__ __forceinline__ void spill_tile(const PV &P, uint64_t *lkeys, uint64_t *lst, uint64_t *seg,
                                            uint32_t &wcur, uint32_t *hraw, int64_t base, int64_t end) {
   uint64_t v[U][2][NC];
   uint32_t vb[U][2];
@@ -274,9 +288,18 @@ __device__ __forceinline__ void spill_tile(PlanRef P, uint64_t *lkeys, uint64_t 
       for (int c = 0; c < NC; ++c) {
         if (c >= P.n_cols || !((P.leaf_cols >> c) & 1)) continue;
         const bool fl = P.cols[c].is_float;
-        for (int l = P.leaf_begin[c]; l < P.leaf_begin[c + 1]; ++l) {
-          const DevLeaf L = cget(P.leaves[l]);
-          m |= (uint32_t)leaf_true(L, fl, vals[c], (vbits >> c) & 1u) << L.bit;
+        if constexpr (RegPlan<PV>::value) {
+#pragma unroll
+          for (int l = 0; l < kMaxLeaves; ++l) {
+            if (l < P.leaf_begin[c] || l >= P.leaf_begin[c + 1]) continue;
+            const DevLeaf L = P.leaves[l];
+            m |= (uint32_t)leaf_true(L, fl, vals[c], (vbits >> c) & 1u) << L.bit;
+          }
+        } else {
+          for (int l = P.leaf_begin[c]; l < P.leaf_begin[c + 1]; ++l) {
+            const DevLeaf L = cget(P.leaves[l]);
+            m |= (uint32_t)leaf_true(L, fl, vals[c], (vbits >> c) & 1u) << L.bit;
+          }
         }
       }
       kept = truth_keeps(P, m);
@@ -286,11 +309,12 @@ __device__ __forceinline__ void spill_tile(PlanRef P, uint64_t *lkeys, uint64_t 
 }
 
 // Pass A. Workgroup s folds rows [s * chunk, (s + 1) * chunk); NC bounds
-// the number of input columns, U the row pairs a thread loads at once.
-template <int LDSM, int NC, int U>
-__global__ void __launch_bounds__(1024) k_scan_spill(const DevPlan *__restrict__ plan, int64_t chunk) {
+// the number of input columns, U the row pairs a thread loads at once. PV is
+// the plan's type: the plan in constant memory, or (runtime-compiled) a
+// register copy whose fields of the query's shape are compile-time values.
+template <int LDSM, int NC, int U, class PV>
+__device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
   extern __shared__ uint64_t smem[];
-  PlanRef P = plan_ref(plan);
   const int E = P.lds_entries, W = P.words, NP = P.n_parts;
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + (LDSM == LDS_HASHED ? E : 0);
@@ -300,7 +324,13 @@ __global__ void __launch_bounds__(1024) k_scan_spill(const DevPlan *__restrict__
   if (LDSM != LDS_NONE) {
     for (int i = threadIdx.x; i < E; i += blockDim.x) {
       if (LDSM == LDS_HASHED) lkeys[i] = kEmpty;
-      for (int w = 0; w < W; ++w) lst[i * W + w] = P.ident[w];
+      if constexpr (RegPlan<PV>::value) {
+#pragma unroll
+        for (int w = 0; w < kMaxWords; ++w)
+          if (w < W) lst[i * W + w] = P.ident[w];
+      } else {
+        for (int w = 0; w < W; ++w) lst[i * W + w] = P.ident[w];
+      }
     }
   }
   for (int i = threadIdx.x; i < 2 * NP + 2; i += blockDim.x) hraw[i] = 0;
@@ -342,6 +372,12 @@ __global__ void __launch_bounds__(1024) k_scan_spill(const DevPlan *__restrict__
   if (threadIdx.x == 0) P.cnt_par[s] = cnt[1];
 }
 
+template <int LDSM, int NC, int U>
+__global__ void __launch_bounds__(1024) k_scan_spill(const DevPlan *__restrict__ plan, int64_t chunk) {
+  scan_spill_body<LDSM, NC, U>(plan_ref(plan), chunk);
+}
+
+#if !DFR_JIT
 // Pass B, step 1: per partition p, exclusive scan over segments of the
 // histograms (in place), partition totals into base_*[p].
 __global__ void k_part_offsets(const DevPlan *__restrict__ plan) {
@@ -713,5 +749,7 @@ __global__ void __launch_bounds__(512) k_sub_aggregate(const DevPlan *__restrict
     for (int w = 0; w < W; ++w) d[w] = s[w];
   }
 }
+
+#endif  // !DFR_JIT
 
 }  // namespace dfr

@@ -23,7 +23,9 @@
 // range), so every state is order independent and results are bitwise
 // reproducible whatever the launch geometry or batching.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <stdint.h>
+#endif
 
 namespace dfr {
 

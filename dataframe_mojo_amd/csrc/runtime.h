@@ -40,7 +40,9 @@ struct Tuning {
   int sub_block = 256;         // threads of a second-level pass C workgroup
   int two_level = -1;
   int batch_tile = 131072;     // rows of a batch work item
-  int batch_graph = 0;         // 1: a batch's launch sequence runs as one captured HIP graph          // second partition level: -1 when groups may outgrow one level, 0 never, 1 always
+  int batch_graph = 0;         // 1: a batch's launch sequence runs as one captured HIP graph
+  int jit = 1;                 // pass A compiled for the query's shape: 0 never, 1 from jit_rows rows, 2 always
+  int64_t jit_rows = 1 << 24;          // second partition level: -1 when groups may outgrow one level, 0 never, 1 always
 };
 
 }  // namespace dfr

@@ -127,27 +127,44 @@ void dfr_server::worker(int lane) {
       stats.concurrent_max = std::max<int64_t>(stats.concurrent_max, busy_lanes);
     }
     const int n = (int)batch.size();
-    std::vector<dfr_query> qs(n);
-    for (int i = 0; i < n; ++i) qs[i] = batch[i]->q;
     std::vector<dfr_result *> res(n, nullptr);
     std::vector<int32_t> st(n, DFR_OK);
     std::vector<char *> errs(n, nullptr);
     char *berr = nullptr;
-    const int bst = dfr_groupby_batch(ctx, qs.data(), n, res.data(), st.data(), errs.data(), &berr);
+    int bst = DFR_OK;
+    // ordered queries, and a batch of one, take the one-query path; the
+    // rest run as one batch
+    std::vector<dfr_query> qs;
+    std::vector<int> at;
+    for (int i = 0; i < n; ++i) {
+      if (n == 1 || batch[i]->q.maintain_order) {
+        st[i] = dfr_groupby(ctx, &batch[i]->q, &res[i], &errs[i]);
+      } else {
+        qs.push_back(batch[i]->q);
+        at.push_back(i);
+      }
+    }
+    if (!qs.empty()) {
+      const int m = (int)qs.size();
+      std::vector<dfr_result *> r2(m, nullptr);
+      std::vector<int32_t> s2(m, DFR_OK);
+      std::vector<char *> e2(m, nullptr);
+      bst = dfr_groupby_batch(ctx, qs.data(), m, r2.data(), s2.data(), e2.data(), &berr);
+      for (int k = 0; k < m; ++k) {
+        res[at[k]] = r2[k];
+        st[at[k]] = bst == DFR_OK ? s2[k] : bst;
+        errs[at[k]] = bst == DFR_OK ? e2[k] : (berr ? strdup(berr) : nullptr);
+      }
+    }
     const int64_t t_done = now_ns();
     {
       std::lock_guard<std::mutex> lk(mu);
       for (int i = 0; i < n; ++i) {
         Done d;
         d.latency_ns = t_done - batch[i]->t_submit;
-        if (bst != DFR_OK) {
-          d.status = bst;
-          d.error = berr ? strdup(berr) : nullptr;
-        } else {
-          d.status = st[i];
-          d.result = res[i];
-          d.error = errs[i];
-        }
+        d.status = st[i];
+        d.result = res[i];
+        d.error = errs[i];
         done[batch[i]->ticket] = d;
         delete batch[i];
       }

@@ -103,6 +103,8 @@ typedef struct dfr_observation {
     int64_t peak_reserved_bytes;
     int64_t waits;             /* host waits on the stream */
     int64_t queries;           /* group-by queries executed */
+    int64_t jit_compiles;      /* pass A kernels compiled at run time for a query shape */
+    int64_t jit_launches;      /* pass A launches of such kernels */
 } dfr_observation;
 int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
 
@@ -119,7 +121,10 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * power of two up to 4096), "sub_lds_bytes" and "sub_block" (LDS bytes and
  * threads of a second-level partition table's workgroup), "batch_tile"
  * (rows of a batch work item), "batch_graph" (1: a batch's launch sequence
- * is captured and run as one HIP graph). */
+ * is captured and run as one HIP graph), "jit" (pass A compiled at run time
+ * for the query's shape: 0 never, 1 when the input has at least "jit_rows"
+ * rows, 2 always; the precompiled kernel runs when it cannot be built),
+ * "jit_rows". */
 int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
 
 /* ---- device columns (the column hand-off) ------------------------------ */

@@ -176,7 +176,7 @@ def test_random_queries(seed, fi, ki):
 
 TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26, "hash_slots": 0, "block": 512,
                  "wg_per_cu": 2, "lds_probe": 4, "scan_unroll": 1, "agg_mode": 1, "two_level": -1,
-                 "max_parts": 4096, "part_lds_bytes": 98304, "sub_lds_bytes": 49152, "sub_block": 256}
+                 "max_parts": 4096, "part_lds_bytes": 98304, "sub_lds_bytes": 49152, "sub_block": 256, "jit": 1}
 
 
 @pytest.mark.parametrize("setting", [
@@ -188,6 +188,8 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
     # one first-level partition, many small second-level tables
     {"two_level": 1, "max_parts": 1, "sub_lds_bytes": 4096, "sub_block": 64},
     {"two_level": 1, "sub_lds_bytes": 98304, "sub_block": 512},
+    # pass A compiled at run time for the query's shape
+    {"jit": 2}, {"jit": 2, "lds_mode": 0}, {"jit": 2, "two_level": 1}, {"jit": 2, "scan_unroll": 4},
 ])
 def test_table_modes_give_identical_results(setting, gpu_ctx):
     df = random_frame(30000, 5, key_card=400)
@@ -346,3 +348,30 @@ def test_dict_workload_size_independent_properties(gpu_ctx):
     _, cnts, _ = res.agg(1)
     assert int(lens.sum()) == int(kept.sum())
     assert int(cnts.sum()) == int((kept & valid).sum())
+
+
+@pytest.mark.parametrize("ki", range(len(KEYS)))
+@pytest.mark.parametrize("fi", [1, 2, 3])
+def test_jit_scan_matches_precompiled(ki, fi, gpu_ctx):
+    """Pass A compiled at run time for the query's shape gives the
+    precompiled kernel's results bit for bit, and the oracle's."""
+    df = random_frame(25000, 11 + fi, key_card=300)
+    lf = df.lazy()
+    for f in FILTERS[fi]:
+        lf = lf.filter(f)
+    lf = lf.group_by(*KEYS[ki], maintain_order=True).agg(ALL_AGGS)
+    try:
+        gpu_ctx.tune("jit", 0)
+        base = lf.collect()
+        before = gpu_ctx.observe()["jit_launches"]
+        gpu_ctx.tune("jit", 2)
+        got = lf.collect()
+        assert gpu_ctx.observe()["jit_launches"] > before  # the runtime-compiled kernel ran
+    finally:
+        gpu_ctx.tune("jit", 1)
+    for name in base.columns:
+        a, b = base.column(name).column, got.column(name).column
+        assert np.array_equal(a.values.view(np.uint8), b.values.view(np.uint8)), name
+        assert np.array_equal(a.is_valid(), b.is_valid()), name
+    exp, _ = oracle.collect(lf)
+    assert_same(got, exp, True, ("x_min", "x_max"), len(KEYS[ki]))
