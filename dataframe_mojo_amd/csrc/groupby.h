@@ -27,5 +27,7 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
                    const dfr::Tuning *tune = nullptr);
 int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error);
 
-// Pass A compiled at run time for the plan's shape (jit.hip), or null.
-hipFunction_t jit_scan_function(dfr_context *ctx, const dfr::DevPlan &P, int LDSM, int NC, int U);
+// Kernels compiled at run time for the plan's shape (jit.hip): pass A, and
+// pass C at one or two partition levels; null when they cannot be built.
+enum { JIT_SCAN = 0, JIT_PART = 1, JIT_SUB = 2 };
+hipFunction_t jit_function(dfr_context *ctx, const dfr::DevPlan &P, int LDSM, int NC, int U, int which);

@@ -581,22 +581,46 @@ int set_lds(K kernel, size_t bytes, char **error) {
   return DFR_OK;
 }
 
+// The template shape pass A runs with: LDS table mode, column bound, row
+// pairs per thread.
+void scan_shape(const dfr_context *ctx, const Compiled &C, int *ldsm, int *nc, int *u) {
+  const int n = C.plan.n_cols, t = ctx->tune.scan_unroll;
+  *ldsm = C.plan.lds_mode;
+  *nc = n <= 4 ? 4 : (n <= 8 ? 8 : kMaxCols);
+  *u = n <= 4 ? (t == 4 ? 4 : (t == 2 ? 2 : 1)) : (n <= 8 ? (t >= 2 ? 2 : 1) : 1);
+}
+
+// Launches the kernel `which` compiled at run time for the query's shape;
+// false when the tuning leaves it off or it cannot run here (the caller then
+// launches the precompiled kernel).
+bool jit_launch(dfr_context *ctx, const Compiled &C, int which, unsigned grid, unsigned block, size_t lds,
+                void **args) {
+  const Tuning &T = ctx->tune;
+  if (!(T.jit == 2 || (T.jit == 1 && C.rows >= T.jit_rows))) return false;
+  int ldsm, nc, u;
+  scan_shape(ctx, C, &ldsm, &nc, &u);
+  hipFunction_t fn = jit_function(ctx, C.plan, ldsm, nc, u, which);
+  if (!fn) return false;
+  if (lds > 65536 &&
+      hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, (unsigned)lds, ctx->stream, args, nullptr) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  ctx->obs.jit_launches++;
+  return true;
+}
+
 template <int LDSM, int NC, int U>
 int launch_scan(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
   int st;
-  const Tuning &T = ctx->tune;
-  if ((T.jit == 2 || (T.jit == 1 && C.rows >= T.jit_rows)) && C.lds_bytes <= 65536) {
-    // the query's shape compiled at run time; the precompiled kernel
-    // otherwise
-    hipFunction_t fn = jit_scan_function(ctx, C.plan, LDSM, NC, U);
-    if (fn) {
-      int64_t chunk = C.chunk;
-      void *args[] = {(void *)&dplan, (void *)&chunk};
-      HIP_TRY(hipModuleLaunchKernel(fn, C.grid, 1, 1, C.block, 1, 1, (unsigned)C.lds_bytes, ctx->stream, args,
-                                    nullptr));
-      ctx->obs.jit_launches++;
-      return DFR_OK;
-    }
+  {
+    int64_t chunk = C.chunk;
+    void *args[] = {(void *)&dplan, (void *)&chunk};
+    if (jit_launch(ctx, C, JIT_SCAN, C.grid, C.block, C.lds_bytes, args)) return DFR_OK;
   }
   if ((st = set_lds(k_scan_spill<LDSM, NC, U>, C.lds_bytes, error))) return st;
   hipLaunchKernelGGL((k_scan_spill<LDSM, NC, U>), dim3(C.grid), dim3(C.block), C.lds_bytes, ctx->stream, dplan,
@@ -606,15 +630,15 @@ int launch_scan(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char 
 
 template <int LDSM>
 int launch_scan_cols(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
-  const int nc = C.plan.n_cols;
-  const int u = ctx->tune.scan_unroll;
-  if (nc <= 4) {
+  int ldsm, nc, u;
+  scan_shape(ctx, C, &ldsm, &nc, &u);
+  if (nc == 4) {
     if (u == 4) return launch_scan<LDSM, 4, 4>(ctx, C, dplan, error);
     if (u == 2) return launch_scan<LDSM, 4, 2>(ctx, C, dplan, error);
     return launch_scan<LDSM, 4, 1>(ctx, C, dplan, error);
   }
-  if (nc <= 8) {
-    if (u >= 2) return launch_scan<LDSM, 8, 2>(ctx, C, dplan, error);
+  if (nc == 8) {
+    if (u == 2) return launch_scan<LDSM, 8, 2>(ctx, C, dplan, error);
     return launch_scan<LDSM, 8, 1>(ctx, C, dplan, error);
   }
   return launch_scan<LDSM, kMaxCols, 1>(ctx, C, dplan, error);
@@ -640,8 +664,11 @@ int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan
   hipLaunchKernelGGL(k_part_scatter, dim3(P.n_segs), dim3(1024), cur_bytes, ctx->stream, dplan);
   ctx->obs.launches += 3;
   if (!P.two_level) {
-    if ((st = set_lds(k_part_aggregate, C.c_lds_bytes, error))) return st;
-    hipLaunchKernelGGL(k_part_aggregate, dim3(P.n_parts), dim3(512), C.c_lds_bytes, ctx->stream, dplan);
+    void *args[] = {(void *)&dplan};
+    if (!jit_launch(ctx, C, JIT_PART, P.n_parts, 512, C.c_lds_bytes, args)) {
+      if ((st = set_lds(k_part_aggregate, C.c_lds_bytes, error))) return st;
+      hipLaunchKernelGGL(k_part_aggregate, dim3(P.n_parts), dim3(512), C.c_lds_bytes, ctx->stream, dplan);
+    }
     ctx->obs.launches += 1;
   }
   HIP_TRY(hipGetLastError());
@@ -690,9 +717,13 @@ int launch_second_level(dfr_context *ctx, const Compiled &C, const DevPlan *dpla
   }
   if ((st = set_lds(k_sub_scatter, NS * 8, error))) return st;
   hipLaunchKernelGGL(k_sub_scatter, grid2, dim3(256), NS * 8, ctx->stream, dplan);
-  if ((st = set_lds(k_sub_aggregate, C.c_lds_bytes, error))) return st;
-  hipLaunchKernelGGL(k_sub_aggregate, dim3((unsigned)((uint64_t)P.n_parts * NS)), dim3(ctx->tune.sub_block),
-                     C.c_lds_bytes, ctx->stream, dplan);
+  void *args[] = {(void *)&dplan};
+  const unsigned n_sub_parts = (unsigned)((uint64_t)P.n_parts * NS);
+  if (!jit_launch(ctx, C, JIT_SUB, n_sub_parts, ctx->tune.sub_block, C.c_lds_bytes, args)) {
+    if ((st = set_lds(k_sub_aggregate, C.c_lds_bytes, error))) return st;
+    hipLaunchKernelGGL(k_sub_aggregate, dim3(n_sub_parts), dim3(ctx->tune.sub_block), C.c_lds_bytes, ctx->stream,
+                       dplan);
+  }
   ctx->obs.launches += 9;
   HIP_TRY(hipGetLastError());
   return DFR_OK;

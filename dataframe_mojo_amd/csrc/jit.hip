@@ -34,7 +34,7 @@ namespace {
 
 struct JitEntry {
   hipModule_t module = nullptr;
-  hipFunction_t fn = nullptr;
+  hipFunction_t fn[3] = {nullptr, nullptr, nullptr};  // JIT_SCAN, JIT_PART, JIT_SUB
   bool failed = false;
 };
 
@@ -54,15 +54,19 @@ std::string header_dir() {
 std::string shape_source(const DevPlan &P, int LDSM, int NC, int U) {
   std::ostringstream o;
   o << "#define DFR_JIT 1\n#include \"kernels_part.h\"\nusing namespace dfr;\n"
-    << "extern \"C\" __global__ void __launch_bounds__(1024) dfr_scan_jit(const DevPlan *__restrict__ plan, "
-       "int64_t chunk) {\n"
+    << "__device__ __forceinline__ DevPlan make_plan(const DevPlan *plan) {\n"
     << "  PlanRef R = plan_ref(plan);\n"
     << "  DevPlan L{};\n";
   // values of the data and of the query's constants: from the plan
   o << "  L.rows = R.rows; L.lds_entries = R.lds_entries; L.lds_log2 = R.lds_log2;\n"
     << "  L.n_parts = R.n_parts; L.part_log2 = R.part_log2; L.key_bits = R.key_bits;\n"
     << "  L.spill = R.spill; L.partials = R.partials; L.seg_rows = R.seg_rows; L.seg_partials = R.seg_partials;\n"
-    << "  L.hist_raw = R.hist_raw; L.hist_par = R.hist_par; L.cnt_raw = R.cnt_raw; L.cnt_par = R.cnt_par;\n";
+    << "  L.hist_raw = R.hist_raw; L.hist_par = R.hist_par; L.cnt_raw = R.cnt_raw; L.cnt_par = R.cnt_par;\n"
+    << "  L.c_entries = R.c_entries; L.c_log2 = R.c_log2; L.base_raw = R.base_raw; L.base_par = R.base_par;\n"
+    << "  L.part_raw = R.part_raw; L.part_par = R.part_par; L.state = R.state; L.keys_tab = R.keys_tab;\n"
+    << "  L.mode = R.mode; L.cap = R.cap; L.cap_log2 = R.cap_log2; L.max_probe = R.max_probe; L.fault = R.fault;\n"
+    << "  L.n_sub = R.n_sub; L.sub_chunks = R.sub_chunks; L.base2_raw = R.base2_raw; L.base2_par = R.base2_par;\n"
+    << "  L.part2_raw = R.part2_raw; L.part2_par = R.part2_par; L.gcount = R.gcount;\n";
   for (int c = 0; c < P.n_cols; ++c)
     o << "  L.cols[" << c << "].values = R.cols[" << c << "].values; L.cols[" << c << "].valid = R.cols[" << c
       << "].valid; L.cols[" << c << "].vbit = R.cols[" << c << "].vbit;\n"
@@ -100,8 +104,18 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U) {
   for (int l = 0; l < P.n_leaves; ++l)
     o << "  L.leaves[" << l << "].op = " << P.leaves[l].op << "; L.leaves[" << l << "].flags = " << P.leaves[l].flags
       << "; L.leaves[" << l << "].bit = " << P.leaves[l].bit << ";\n";
-  for (int w = 0; w < P.words; ++w) o << "  L.ident[" << w << "] = " << P.ident[w] << "ull;\n";
-  o << "  scan_spill_body<" << LDSM << ", " << NC << ", " << U << ">(L, chunk);\n}\n";
+  for (int w = 0; w < P.words; ++w)
+    o << "  L.ident[" << w << "] = " << P.ident[w] << "ull; L.wop[" << w << "] = " << (int)P.wop[w] << ";\n";
+  o << "  return L;\n}\n";
+  // pass A, pass C (one level), pass C (second level)
+  o << "extern \"C\" __global__ void __launch_bounds__(1024) dfr_scan_jit(const DevPlan *__restrict__ plan, "
+       "int64_t chunk) {\n"
+    << "  const DevPlan L = make_plan(plan);\n"
+    << "  scan_spill_body<" << LDSM << ", " << NC << ", " << U << ">(L, chunk);\n}\n";
+  o << "extern \"C\" __global__ void __launch_bounds__(1024) dfr_part_jit(const DevPlan *__restrict__ plan) {\n"
+    << "  const DevPlan L = make_plan(plan);\n  part_aggregate_body(L);\n}\n";
+  o << "extern \"C\" __global__ void __launch_bounds__(512) dfr_sub_jit(const DevPlan *__restrict__ plan) {\n"
+    << "  const DevPlan L = make_plan(plan);\n  sub_aggregate_body(L);\n}\n";
   return o.str();
 }
 
@@ -126,7 +140,9 @@ JitEntry compile(const std::string &src, int device) {
     std::vector<char> code(code_size);
     if (hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS &&
         hipModuleLoadData(&e.module, code.data()) == hipSuccess &&
-        hipModuleGetFunction(&e.fn, e.module, "dfr_scan_jit") == hipSuccess)
+        hipModuleGetFunction(&e.fn[JIT_SCAN], e.module, "dfr_scan_jit") == hipSuccess &&
+        hipModuleGetFunction(&e.fn[JIT_PART], e.module, "dfr_part_jit") == hipSuccess &&
+        hipModuleGetFunction(&e.fn[JIT_SUB], e.module, "dfr_sub_jit") == hipSuccess)
       e.failed = false;
   }
   hiprtcDestroyProgram(&prog);
@@ -135,9 +151,10 @@ JitEntry compile(const std::string &src, int device) {
 
 }  // namespace
 
-// The runtime-compiled pass A for this plan's shape on the context's device,
-// or null when it cannot be built here (then the precompiled kernel runs).
-hipFunction_t jit_scan_function(dfr_context *ctx, const DevPlan &P, int LDSM, int NC, int U) {
+// A runtime-compiled kernel (JIT_SCAN, JIT_PART, JIT_SUB) for this plan's
+// shape on the context's device, or null when it cannot be built here (then
+// the precompiled kernel runs).
+hipFunction_t jit_function(dfr_context *ctx, const DevPlan &P, int LDSM, int NC, int U, int which) {
   const std::string src = shape_source(P, LDSM, NC, U);
   const std::string key = std::to_string(ctx->device) + "\n" + src;
   std::lock_guard<std::mutex> g(g_mu);
@@ -155,7 +172,7 @@ hipFunction_t jit_scan_function(dfr_context *ctx, const DevPlan &P, int LDSM, in
     it = g_cache.emplace(key, compile(src, ctx->device)).first;
     ctx->obs.jit_compiles++;
   }
-  return it->second.failed ? nullptr : it->second.fn;
+  return it->second.failed ? nullptr : it->second.fn[which];
 }
 
 // The generated source of a plan's pass A (for tests and inspection).

@@ -178,31 +178,31 @@ __device__ __forceinline__ uint32_t truth_bit(PlanRef P, uint32_t m) {
 }
 
 // HBM slot of a packed key (inserting it in the hash table when new).
-#if !DFR_JIT
-__device__ __noinline__ uint64_t global_slot_hash(PlanRef P, uint64_t key) {
-  uint64_t mask = P.cap - 1;
-  uint64_t i = hash_index(key, P.cap_log2);
-  for (int p = 0; p < P.max_probe; ++p) {
-    uint64_t k = atomic_load_u64(&P.keys_tab[i]);
+__device__ __noinline__ uint64_t global_slot_hash(uint64_t *keys_tab, uint64_t cap, int cap_log2, int max_probe,
+                                                  uint32_t *fault, uint64_t key) {
+  uint64_t mask = cap - 1;
+  uint64_t i = hash_index(key, cap_log2);
+  for (int p = 0; p < max_probe; ++p) {
+    uint64_t k = atomic_load_u64(&keys_tab[i]);
     if (k == key) return i;
     if (k == kEmpty) {
       unsigned long long old =
-          atomicCAS((unsigned long long *)&P.keys_tab[i], (unsigned long long)kEmpty,
-                    (unsigned long long)key);
+          atomicCAS((unsigned long long *)&keys_tab[i], (unsigned long long)kEmpty, (unsigned long long)key);
       if (old == kEmpty || old == key) return i;
     }
     i = (i + 1) & mask;
   }
-  atomicOr(&P.fault[0], 1u);
+  atomicOr(&fault[0], 1u);
   return kEmpty;
 }
 
-__device__ __forceinline__ uint64_t global_slot(PlanRef P, uint64_t key) {
+// The HBM slot of a packed key: the key itself (direct table) or its slot
+// in the open-addressing table; kEmpty when that table is full.
+template <class PV>
+__device__ __forceinline__ uint64_t global_slot(const PV &P, uint64_t key) {
   if (P.mode == MODE_DIRECT) return key;
-  return global_slot_hash(P, key);
+  return global_slot_hash(P.keys_tab, P.cap, P.cap_log2, P.max_probe, P.fault, key);
 }
-
-#endif  // !DFR_JIT
 
 __device__ __forceinline__ void add128(uint64_t *p, uint64_t lo, uint64_t hi) {
   unsigned long long old = atomicAdd((unsigned long long *)p, (unsigned long long)lo);

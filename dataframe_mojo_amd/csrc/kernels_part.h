@@ -46,7 +46,8 @@ __device__ __forceinline__ uint32_t part_of(const PV &P, uint64_t key) {
 }
 
 // LDS index inside a partition's table: a second multiplicative hash
-__device__ __forceinline__ int part_slot(PlanRef P, uint64_t key) {
+template <class PV>
+__device__ __forceinline__ int part_slot(const PV &P, uint64_t key) {
   return (int)((key32(key) * 0x85EBCA77u) >> (32 - P.c_log2));
 }
 
@@ -103,9 +104,9 @@ __device__ __forceinline__ void fold_full(const PV &P, uint64_t *st, int64_t row
 }
 
 // Merges a partial state `s` (W words) into `d`, word by word.
-template <class SrcPtr>
-__device__ __forceinline__ void merge_full(PlanRef P, uint64_t *d, SrcPtr s) {
-  for (int w = 0; w < P.words; ++w) {
+template <class SrcPtr, class PV>
+__device__ __forceinline__ void merge_word(const PV &P, uint64_t *d, SrcPtr s, int w) {
+  {
     const uint64_t v = s[w];
     switch (P.wop[w]) {
       case W_ADD:
@@ -131,6 +132,18 @@ __device__ __forceinline__ void merge_full(PlanRef P, uint64_t *d, SrcPtr s) {
         break;
       default:  // W_HI: merged with its low word
         break;
+    }
+  }
+}
+
+template <class SrcPtr, class PV>
+__device__ __forceinline__ void merge_full(const PV &P, uint64_t *d, SrcPtr s) {
+#pragma unroll
+  for (int w = 0; w < (RegPlan<PV>::value ? kMaxWords : 1); ++w) {
+    if constexpr (RegPlan<PV>::value) {
+      if (w < P.words) merge_word(P, d, s, w);
+    } else {
+      for (int x = 0; x < P.words; ++x) merge_word(P, d, s, x);
     }
   }
 }
@@ -477,10 +490,12 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const DevPlan *__restrict
     for (int w = 0; w < 1 + W; ++w) d[w] = rec[w];
   }
 }
+#endif  // !DFR_JIT
 
 // LDS probe of a partition table (first-come, linear probing over the whole
 // table); -1 only when every entry holds another key.
-__device__ __forceinline__ int part_find(PlanRef P, uint64_t *lkeys, uint64_t key) {
+template <class PV>
+__device__ __forceinline__ int part_find(const PV &P, uint64_t *lkeys, uint64_t key) {
   const int E = P.c_entries;
   int h = part_slot(P, key);
   for (int p = 0; p < E; ++p) {
@@ -499,8 +514,8 @@ __device__ __forceinline__ int part_find(PlanRef P, uint64_t *lkeys, uint64_t ke
 // Folds the raw records [b0, b1) of one partition, U per thread at once
 // (loads first). A key the LDS table cannot place goes to the HBM table,
 // or, with DENSE (no HBM table), raises the sub-partition capacity fault.
-template <int RW, int U, bool DENSE>
-__device__ __forceinline__ void part_fold_records(PlanRef P, uint64_t *lkeys, uint64_t *lst, uint64_t keymask,
+template <int RW, int U, bool DENSE, class PV>
+__device__ __forceinline__ void part_fold_records(const PV &P, uint64_t *lkeys, uint64_t *lst, uint64_t keymask,
                                                   const DFR_GLOBAL uint64_t *recs, uint64_t b0, uint64_t b1) {
   const int W = P.words, rw = P.rw;
   for (uint64_t i0 = b0; i0 < b1; i0 += (uint64_t)U * blockDim.x) {
@@ -544,15 +559,21 @@ __device__ __forceinline__ void part_fold_records(PlanRef P, uint64_t *lkeys, ui
 }
 
 // Pass C: partition p.
-__global__ void __launch_bounds__(1024) k_part_aggregate(const DevPlan *__restrict__ plan) {
+template <class PV>
+__device__ __forceinline__ void part_aggregate_body(const PV &P) {
   extern __shared__ uint64_t smem[];
-  PlanRef P = plan_ref(plan);
   const int p = blockIdx.x, E = P.c_entries, W = P.words;
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + E;
   for (int i = threadIdx.x; i < E; i += blockDim.x) {
     lkeys[i] = kEmpty;
-    for (int w = 0; w < W; ++w) lst[i * W + w] = P.ident[w];
+    if constexpr (RegPlan<PV>::value) {
+#pragma unroll
+      for (int w = 0; w < kMaxWords; ++w)
+        if (w < W) lst[i * W + w] = P.ident[w];
+    } else {
+      for (int w = 0; w < W; ++w) lst[i * W + w] = P.ident[w];
+    }
   }
   __syncthreads();
   const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
@@ -588,6 +609,13 @@ __global__ void __launch_bounds__(1024) k_part_aggregate(const DevPlan *__restri
   }
 }
 
+#if !DFR_JIT
+__global__ void __launch_bounds__(1024) k_part_aggregate(const DevPlan *__restrict__ plan) {
+  part_aggregate_body(plan_ref(plan));
+}
+#endif
+
+#if !DFR_JIT
 // ------------------------------------------------ second partition level
 
 // Sub-partition of a key: the hash bits below the partition's, scaled to
@@ -679,23 +707,30 @@ __global__ void __launch_bounds__(256) k_sub_scatter(const DevPlan *__restrict__
     for (uint64_t k = 1; k < rw; ++k) d[k] = r[k];
   }
 }
+#endif  // !DFR_JIT
 
 // Pass C over sub-partition x = p * n_sub + sub: folds its raw
 // records and merges its partial states in an LDS table, then gives each
 // group the next free slot of the dense HBM tables (one device atomic per
 // workgroup) and stores its key and state there.
-__global__ void __launch_bounds__(512) k_sub_aggregate(const DevPlan *__restrict__ plan) {
+template <class PV>
+__device__ __forceinline__ void sub_aggregate_body(const PV &P) {
   extern __shared__ uint64_t smem[];
   __shared__ uint32_t wsum[kMaxPartEntries / 64];
   __shared__ uint64_t gbase;
-  PlanRef P = plan_ref(plan);
   const uint64_t sp = blockIdx.x;
   const int E = P.c_entries, W = P.words, CH = P.sub_chunks;
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + E;
   for (int i = threadIdx.x; i < E; i += blockDim.x) {
     lkeys[i] = kEmpty;
-    for (int w = 0; w < W; ++w) lst[i * W + w] = P.ident[w];
+    if constexpr (RegPlan<PV>::value) {
+#pragma unroll
+      for (int w = 0; w < kMaxWords; ++w)
+        if (w < W) lst[i * W + w] = P.ident[w];
+    } else {
+      for (int w = 0; w < W; ++w) lst[i * W + w] = P.ident[w];
+    }
   }
   __syncthreads();
   const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
@@ -750,6 +785,10 @@ __global__ void __launch_bounds__(512) k_sub_aggregate(const DevPlan *__restrict
   }
 }
 
-#endif  // !DFR_JIT
+#if !DFR_JIT
+__global__ void __launch_bounds__(512) k_sub_aggregate(const DevPlan *__restrict__ plan) {
+  sub_aggregate_body(plan_ref(plan));
+}
+#endif
 
 }  // namespace dfr

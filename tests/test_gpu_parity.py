@@ -366,7 +366,8 @@ def test_jit_scan_matches_precompiled(ki, fi, gpu_ctx):
         before = gpu_ctx.observe()["jit_launches"]
         gpu_ctx.tune("jit", 2)
         got = lf.collect()
-        assert gpu_ctx.observe()["jit_launches"] > before  # the runtime-compiled kernel ran
+        # pass A and pass C both ran as runtime-compiled kernels
+        assert gpu_ctx.observe()["jit_launches"] >= before + 2
     finally:
         gpu_ctx.tune("jit", 1)
     for name in base.columns:
