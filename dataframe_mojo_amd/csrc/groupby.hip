@@ -658,10 +658,11 @@ int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan
   HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
   hipLaunchKernelGGL(k_part_offsets, dim3(P.n_parts), dim3(kBlock), 0, ctx->stream, dplan);
   hipLaunchKernelGGL(k_part_base, dim3(1), dim3(kBlock), 0, ctx->stream, dplan);
-  // k_part_scatter keeps one 64-bit write cursor per partition and kind
-  const size_t cur_bytes = (size_t)2 * P.n_parts * sizeof(unsigned long long);
+  // k_part_scatter: 64-bit write cursors per partition and kind, then its
+  // chunk sort's counters and record indices
+  const size_t cur_bytes = part_scatter_lds(P.n_parts);
   if ((st = set_lds(k_part_scatter, cur_bytes, error))) return st;
-  hipLaunchKernelGGL(k_part_scatter, dim3(P.n_segs), dim3(1024), cur_bytes, ctx->stream, dplan);
+  hipLaunchKernelGGL(k_part_scatter, dim3(P.n_segs), dim3(kScatterThreads), cur_bytes, ctx->stream, dplan);
   ctx->obs.launches += 3;
   if (!P.two_level) {
     void *args[] = {(void *)&dplan};
@@ -715,8 +716,9 @@ int launch_second_level(dfr_context *ctx, const Compiled &C, const DevPlan *dpla
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, ctx->stream, bs, nb);
     hipLaunchKernelGGL(k_count_apply, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, h, N2, bs, b2);
   }
-  if ((st = set_lds(k_sub_scatter, NS * 8, error))) return st;
-  hipLaunchKernelGGL(k_sub_scatter, grid2, dim3(256), NS * 8, ctx->stream, dplan);
+  const size_t sc_bytes = sub_scatter_lds((int)NS);
+  if ((st = set_lds(k_sub_scatter, sc_bytes, error))) return st;
+  hipLaunchKernelGGL(k_sub_scatter, grid2, dim3(kScatterThreads), sc_bytes, ctx->stream, dplan);
   void *args[] = {(void *)&dplan};
   const unsigned n_sub_parts = (unsigned)((uint64_t)P.n_parts * NS);
   if (!jit_launch(ctx, C, JIT_SUB, n_sub_parts, ctx->tune.sub_block, C.c_lds_bytes, args)) {

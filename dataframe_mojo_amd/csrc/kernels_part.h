@@ -430,55 +430,136 @@ __global__ void k_part_base(const DevPlan *__restrict__ plan) {
   }
 }
 
-// Pass B, step 3: segment s scatters its records into partition order. The
-// segment's write cursor of every partition sits in LDS (start of the
-// partition + this segment's offset); each thread moves U records at once.
-template <int RW, int U>
-__device__ __forceinline__ void scatter_records(const DFR_GLOBAL uint64_t *seg, uint32_t n, int rw, uint64_t keymask,
-                                                PlanRef P, unsigned long long *cur, DFR_GLOBAL uint64_t *dst) {
-  for (uint32_t i0 = 0; i0 < n; i0 += U * blockDim.x) {
-    uint64_t w[U][RW];
-    bool ok[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t i = i0 + u * blockDim.x + threadIdx.x;
-      ok[u] = i < n;
-#pragma unroll
-      for (int k = 0; k < RW; ++k) w[u][k] = (ok[u] && k < rw) ? seg[(uint64_t)i * rw + k] : 0;
+// Pass B, step 3: segment s moves its records into partition order.
+//
+// Records are taken kScatterChunk at a time, counting-sorted by partition in
+// LDS, and written out in that order: each partition then receives runs of
+// consecutive records instead of isolated ones, so HBM sees whole lines
+// rather than partial writes. Every write position is the partition's
+// cursor for this segment plus the record's rank, so the result is the same
+// as moving the records one by one.
+constexpr int kScatterChunk = 16384;  // records sorted per round (chunk index fits 16 bits)
+constexpr int kScatterThreads = 1024;
+
+// Exclusive scan of a[0..n) in place by a 1024-thread block; a[n] = total.
+__device__ __forceinline__ void block_scan_u32(uint32_t *a, int n, uint32_t *wsum) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int per = (n + kScatterThreads - 1) / kScatterThreads;
+  const int t0 = t * per;
+  uint32_t sum = 0;
+  for (int j = 0; j < per; ++j)
+    if (t0 + j < n) sum += a[t0 + j];
+  uint32_t inc = sum;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  uint32_t off = 0, total = 0;
+  for (int w = 0; w < kScatterThreads / 64; ++w) {
+    if (w < wave) off += wsum[w];
+    total += wsum[w];
+  }
+  uint32_t ex = off + inc - sum;
+  for (int j = 0; j < per; ++j)
+    if (t0 + j < n) {
+      const uint32_t v = a[t0 + j];
+      a[t0 + j] = ex;
+      ex += v;
     }
+  if (t == 0) a[n] = total;
+  __syncthreads();
+}
+
+// Moves n records of rw words from `src` to dst[cur[b] ...] in bucket order
+// (bucket_of(record word 0) < nb), CH records a round, advancing the
+// cursors. LDS: cnt[nb + 1], idx[CH], wsum[16].
+template <int RW, int CH, class BucketOf>
+__device__ __forceinline__ void scatter_sorted(const DFR_GLOBAL uint64_t *src, uint32_t n, int rw, int nb,
+                                               BucketOf bucket_of, unsigned long long *cur, uint32_t *cnt,
+                                               uint32_t *idx, uint32_t *wsum, DFR_GLOBAL uint64_t *dst) {
+  static_assert(CH <= 65536 && CH % kScatterThreads == 0, "chunk indices fit 16 bits");
+  constexpr int IT = CH / kScatterThreads;
+  for (uint32_t c0 = 0; c0 < n; c0 += CH) {
+    const uint32_t m = min((uint32_t)CH, n - c0);
+    for (int b = threadIdx.x; b <= nb; b += kScatterThreads) cnt[b] = 0;
+    __syncthreads();
+    uint32_t br[IT];  // bucket << 16 | rank within the bucket
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (!ok[u]) continue;
-      const uint32_t p = part_of(P, w[u][0] & keymask);
-      const uint64_t pos = atomicAdd(&cur[p], 1ull);
+    for (int it = 0; it < IT; ++it) {
+      const uint32_t i = it * kScatterThreads + threadIdx.x;
+      br[it] = 0;
+      if (i < m) {
+        const uint32_t b = bucket_of(src[(uint64_t)(c0 + i) * rw]);
+        br[it] = (b << 16) | atomicAdd(&cnt[b], 1u);
+      }
+    }
+    __syncthreads();
+    block_scan_u32(cnt, nb, wsum);
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const uint32_t i = it * kScatterThreads + threadIdx.x;
+      if (i < m) {
+        const uint32_t b = br[it] >> 16;
+        idx[cnt[b] + (br[it] & 0xffffu)] = (b << 16) | i;
+      }
+    }
+    __syncthreads();
+    for (uint32_t slot = threadIdx.x; slot < m; slot += kScatterThreads) {
+      const uint32_t v = idx[slot];
+      const uint32_t b = v >> 16, i = v & 0xffffu;
+      const uint64_t pos = cur[b] + (slot - cnt[b]);
+      const DFR_GLOBAL uint64_t *r = src + (uint64_t)(c0 + i) * rw;
       DFR_GLOBAL uint64_t *d = dst + pos * rw;
+      if constexpr (RW <= 16) {
+        uint64_t w[RW];  // every load before the stores
 #pragma unroll
-      for (int k = 0; k < RW; ++k)
-        if (k < rw) d[k] = w[u][k];
+        for (int k = 0; k < RW; ++k) w[k] = k < rw ? r[k] : 0;
+#pragma unroll
+        for (int k = 0; k < RW; ++k)
+          if (k < rw) d[k] = w[k];
+      } else {
+        for (int k = 0; k < rw; ++k) d[k] = r[k];
+      }
     }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += kScatterThreads) cur[b] += cnt[b + 1] - cnt[b];
+    __syncthreads();
   }
 }
 
+// LDS bytes of k_part_scatter for np partitions.
+__host__ __device__ constexpr size_t part_scatter_lds(int np) {
+  return (size_t)2 * np * 8 + (size_t)(np + 1) * 4 + (size_t)kScatterChunk * 4;
+}
+
 __global__ void __launch_bounds__(1024) k_part_scatter(const DevPlan *__restrict__ plan) {
-  extern __shared__ unsigned long long cur[];  // [2 * n_parts]
+  extern __shared__ unsigned long long cur[];  // [2 * n_parts], then cnt and idx
+  __shared__ uint32_t wsum[kScatterThreads / 64];
   PlanRef P = plan_ref(plan);
   const int s = blockIdx.x, NP = P.n_parts, W = P.words;
+  uint32_t *cnt = (uint32_t *)(cur + 2 * NP);
+  uint32_t *idx = cnt + NP + 1;
   for (int p = threadIdx.x; p < NP; p += blockDim.x) {
     cur[p] = P.base_raw[p] + P.hist_raw[(uint64_t)s * NP + p];
     cur[NP + p] = P.base_par[p] + P.hist_par[(uint64_t)s * NP + p];
   }
   __syncthreads();
   const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
+  auto bucket = [&](uint64_t w0) { return part_of(P, w0 & keymask); };
   for (int w = 0; w < P.seg_waves; ++w) {
     const uint64_t wseg = (uint64_t)s * P.seg_waves + w;
     const DFR_GLOBAL uint64_t *rseg = gp(P.spill) + wseg * (uint64_t)P.seg_rows * P.rw;
     const uint32_t n = P.cnt_raw[wseg];
-    if (P.rw <= 4)
-      scatter_records<4, 4>(rseg, n, P.rw, keymask, P, cur, gpw(P.part_raw));
+    if (P.rw <= 2)
+      scatter_sorted<2, kScatterChunk>(rseg, n, P.rw, NP, bucket, cur, cnt, idx, wsum, gpw(P.part_raw));
+    else if (P.rw <= 4)
+      scatter_sorted<4, kScatterChunk>(rseg, n, P.rw, NP, bucket, cur, cnt, idx, wsum, gpw(P.part_raw));
     else if (P.rw <= 8)
-      scatter_records<8, 2>(rseg, n, P.rw, keymask, P, cur, gpw(P.part_raw));
+      scatter_sorted<8, kScatterChunk>(rseg, n, P.rw, NP, bucket, cur, cnt, idx, wsum, gpw(P.part_raw));
     else
-      scatter_records<kMaxCols + 3, 1>(rseg, n, P.rw, keymask, P, cur, gpw(P.part_raw));
+      scatter_sorted<kMaxCols + 3, kScatterChunk>(rseg, n, P.rw, NP, bucket, cur, cnt, idx, wsum, gpw(P.part_raw));
   }
   const DFR_GLOBAL uint64_t *pseg = gp(P.partials) + (uint64_t)s * (uint64_t)P.seg_partials * (1 + W);
   const uint32_t n = P.cnt_par[s];
@@ -683,29 +764,42 @@ __global__ void k_count_apply(const uint32_t *c, uint64_t n, const uint64_t *blo
 }
 
 // Same grid as k_sub_hist: chunk x of partition y moves its records into
-// sub-partition order, write cursors in LDS.
-__global__ void __launch_bounds__(256) k_sub_scatter(const DevPlan *__restrict__ plan) {
-  extern __shared__ unsigned long long cur2[];  // [n_sub]
+// sub-partition order (counting-sorted in LDS, written as runs), from the
+// cursors the scan of the counts gives.
+constexpr int kSubScatterChunk = 8192;
+
+__host__ __device__ constexpr size_t sub_scatter_lds(int ns) {
+  return (size_t)ns * 8 + (size_t)(ns + 1) * 4 + (size_t)kSubScatterChunk * 4;
+}
+
+__global__ void __launch_bounds__(1024) k_sub_scatter(const DevPlan *__restrict__ plan) {
+  extern __shared__ unsigned long long cur2[];  // [n_sub], then cnt and idx
+  __shared__ uint32_t wsum[kScatterThreads / 64];
   PlanRef P = plan_ref(plan);
   const int j = blockIdx.x, p = blockIdx.y, which = blockIdx.z;
   const int NS = P.n_sub;
+  uint32_t *cnt = (uint32_t *)(cur2 + NS);
+  uint32_t *idx = cnt + NS + 1;
   const uint64_t *b2 = which ? P.base2_par : P.base2_raw;
   for (int i = threadIdx.x; i < NS; i += blockDim.x) cur2[i] = b2[((uint64_t)p * NS + i) * P.sub_chunks + j];
   __syncthreads();
   const DFR_GLOBAL uint64_t *src = gp(which ? P.part_par : P.part_raw);
   DFR_GLOBAL uint64_t *dst = gpw(which ? P.part2_par : P.part2_raw);
-  const uint64_t rw = which ? 1 + P.words : P.rw;
+  const int rw = which ? 1 + P.words : P.rw;
   const uint64_t keymask = (!which && P.vmask_in_key) ? ((1ull << P.key_bits) - 1) : ~0ull;
   uint64_t lo, hi;
   chunk_range(which ? P.base_par : P.base_raw, p, j, P.sub_chunks, lo, hi);
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    const DFR_GLOBAL uint64_t *r = src + i * rw;
-    const uint64_t w0 = r[0];
-    const uint64_t pos = atomicAdd(&cur2[sub_of(P, w0 & keymask)], 1ull);
-    DFR_GLOBAL uint64_t *d = dst + pos * rw;
-    d[0] = w0;
-    for (uint64_t k = 1; k < rw; ++k) d[k] = r[k];
-  }
+  auto bucket = [&](uint64_t w0) { return sub_of(P, w0 & keymask); };
+  const DFR_GLOBAL uint64_t *s0 = src + lo * rw;
+  const uint32_t n = (uint32_t)(hi - lo);
+  if (rw <= 2)
+    scatter_sorted<2, kSubScatterChunk>(s0, n, rw, NS, bucket, cur2, cnt, idx, wsum, dst);
+  else if (rw <= 4)
+    scatter_sorted<4, kSubScatterChunk>(s0, n, rw, NS, bucket, cur2, cnt, idx, wsum, dst);
+  else if (rw <= 8)
+    scatter_sorted<8, kSubScatterChunk>(s0, n, rw, NS, bucket, cur2, cnt, idx, wsum, dst);
+  else
+    scatter_sorted<kMaxWords + 1, kSubScatterChunk>(s0, n, rw, NS, bucket, cur2, cnt, idx, wsum, dst);
 }
 #endif  // !DFR_JIT
 
