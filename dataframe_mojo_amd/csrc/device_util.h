@@ -41,6 +41,14 @@ DFR_HD uint64_t ord_key_inverse(uint64_t key) {
   return (key >> 63) ? (key & ~(1ull << 63)) : ~key;
 }
 
+// The canonical word of a Float64 group key: -0.0 and 0.0 are one key, as
+// are all NaNs.
+DFR_HD uint64_t canon_key_bits(uint64_t bits) {
+  if ((bits << 1) == 0) return 0;
+  if ((bits & 0x7FF0000000000000ull) == 0x7FF0000000000000ull && (bits << 12) != 0) return 0x7FF8000000000000ull;
+  return bits;
+}
+
 // 128-bit shifts, shift amounts in [0, 127]; larger right shifts give 0.
 DFR_HD void shl128(uint64_t &lo, uint64_t &hi, int s) {
   if (s <= 0) return;

@@ -156,11 +156,24 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
   // keys: mixed-radix packing from the column statistics
   unsigned __int128 domain = 1;
   C.key_dtypes.clear();
+  bool float_key = false;
   for (int k = 0; k < q->n_keys; ++k) {
     dfr_column *c = C.cols[k];
-    if (c->dtype == DFR_FLOAT64)
-      return dfr_fail(error, DFR_UNSUPPORTED, "Float64 group keys are not on the device path");
     DevKey &K = P.keys[k];
+    if (c->dtype == DFR_FLOAT64) {
+      // a Float64 key groups by its canonical 64-bit word: the whole word
+      // range, so it is the query's only key
+      if (q->n_keys != 1)
+        return dfr_fail(error, DFR_UNSUPPORTED, "a Float64 group key is the only key on the device path");
+      K.is_float = 1;
+      K.nullable = c->nulls > 0 ? 1 : 0;
+      K.minv = 0;
+      K.radix = 0;
+      K.stride = 1;
+      float_key = true;
+      C.key_dtypes.push_back(c->dtype);
+      continue;
+    }
     bool has_values = c->stats.valid > 0;
     K.minv = has_values ? c->stats.imin : 0;
     unsigned __int128 span =
@@ -177,7 +190,7 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
       return dfr_fail(error, DFR_UNSUPPORTED, "the combined key domain exceeds 2^62");
     C.key_dtypes.push_back(c->dtype);
   }
-  P.domain = (uint64_t)domain;
+  P.domain = float_key ? ~0ull : (uint64_t)domain;
 
   // predicate leaves, grouped by column; leaf i keeps truth bit i
   const dfr_predicate &pr = q->predicate;
@@ -302,7 +315,9 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
 
   // slot layout
   int w = 1;  // ROWS
-  P.has_first = q->maintain_order ? 1 : 0;
+  // FIRST is tracked for ordered output, and for a Float64 key, whose
+  // output is its group's first row's value
+  P.has_first = (q->maintain_order || float_key) ? 1 : 0;
   if (P.has_first) w = 2;
   int lw = w;
   int l2g[kMaxWords];

@@ -100,7 +100,9 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U) {
       << ";\n";
   }
   for (int c = 0; c <= P.n_cols; ++c) o << "  L.leaf_begin[" << c << "] = " << (int)P.leaf_begin[c] << ";\n";
-  for (int k = 0; k < P.n_keys; ++k) o << "  L.keys[" << k << "].nullable = " << P.keys[k].nullable << ";\n";
+  for (int k = 0; k < P.n_keys; ++k)
+    o << "  L.keys[" << k << "].nullable = " << P.keys[k].nullable << "; L.keys[" << k << "].is_float = "
+      << P.keys[k].is_float << ";\n";
   for (int l = 0; l < P.n_leaves; ++l)
     o << "  L.leaves[" << l << "].op = " << P.leaves[l].op << "; L.leaves[" << l << "].flags = " << P.leaves[l].flags
       << "; L.leaves[" << l << "].bit = " << P.leaves[l].bit << ";\n";

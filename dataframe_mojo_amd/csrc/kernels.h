@@ -308,12 +308,16 @@ __device__ __forceinline__ void fold_kept(PlanRef P, uint64_t *lkeys, uint64_t *
                                           int64_t row, const uint64_t (&vals)[NC],
                                           uint32_t validbits) {
   uint64_t packed = 0;
+  if (P.keys[0].is_float) {
+    packed = (validbits & 1u) ? canon_key_bits(vals[0]) : kNullFloatKey;
+  } else {
 #pragma unroll
-  for (int k = 0; k < kMaxKeys; ++k) {
-    if (k >= P.n_keys) continue;
-    const DevKey K = cget(P.keys[k]);
-    uint64_t digit = ((validbits >> k) & 1u) ? (vals[k] - (uint64_t)K.minv) : (K.radix - 1);
-    packed += digit * K.stride;
+    for (int k = 0; k < kMaxKeys && k < NC; ++k) {
+      if (k >= P.n_keys) continue;
+      const DevKey K = cget(P.keys[k]);
+      uint64_t digit = ((validbits >> k) & 1u) ? (vals[k] - (uint64_t)K.minv) : (K.radix - 1);
+      packed += digit * K.stride;
+    }
   }
   if (LDSM != LDS_NONE) {
     int e = lds_find<LDSM>(P, lkeys, packed);
@@ -692,9 +696,17 @@ __device__ __forceinline__ void finalize_group(PlanRef P, uint64_t i, uint64_t s
   if (active && P.out_first) P.out_first[i] = slot[1];
   for (int k = 0; k < P.n_keys; ++k) {
     const DevKey K = cget(P.keys[k]);
-    uint64_t digit = (packed / K.stride) % K.radix;
-    bool valid = !(K.nullable && digit == K.radix - 1);
-    uint64_t v = valid ? (uint64_t)K.minv + digit : 0;
+    bool valid;
+    uint64_t v;
+    if (K.is_float) {
+      // the group's first row's own value (FIRST is tracked for such keys)
+      valid = packed != kNullFloatKey;
+      v = (active && valid) ? ((const uint64_t *)P.cols[k].values)[slot[1]] : 0;
+    } else {
+      uint64_t digit = (packed / K.stride) % K.radix;
+      valid = !(K.nullable && digit == K.radix - 1);
+      v = valid ? (uint64_t)K.minv + digit : 0;
+    }
     if (active) store_value(P.out_keys[k], P.cols[k].dtype, i, v);
     uint64_t ball = __ballot(active && valid);
     if (writer) P.out_key_valid[k][i >> 6] = ball;

@@ -208,7 +208,9 @@ __device__ __forceinline__ void spill_row(const PV &P, uint64_t *lkeys, uint64_t
   uint64_t packed = 0;
   int e = -1;
   if (kept) {
-    if (P.n_keys == 1) {
+    if (P.n_keys == 1 && P.keys[0].is_float) {
+      packed = (vbits & 1u) ? canon_key_bits(vals[0]) : kNullFloatKey;
+    } else if (P.n_keys == 1) {
       packed = (vbits & 1u) ? (vals[0] - (uint64_t)P.keys[0].minv) : (P.keys[0].radix - 1);
     } else {
 #pragma unroll

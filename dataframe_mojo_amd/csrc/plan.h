@@ -94,8 +94,12 @@ struct DevKey {
   uint64_t radix;
   uint64_t stride;
   int32_t nullable;
-  int32_t pad;
+  int32_t is_float;  // a Float64 key (the only key): packed as its canonical word
 };
+
+// Packed key of a null Float64 key: a NaN pattern canonical words never take
+// (every NaN canonicalises to 0x7FF8000000000000).
+constexpr uint64_t kNullFloatKey = 0x7FF0000000000001ull;
 
 struct DevLeaf {
   int32_t op;

@@ -199,8 +199,10 @@ def compile_plan(plan: LazyFrame) -> DevicePlan:
         raise Unsupported("group_by needs at least one key")
     for k in plan.keys:
         dt = frame.column(k).dtype
-        if dt == FLOAT64:
-            raise Unsupported("Float64 group keys are not on the device path")
+        if dt == FLOAT64 and len(plan.keys) > 1:
+            # a Float64 key groups by its canonical 64-bit word (-0.0 == 0.0,
+            # one NaN group; group_by.mojo:319-329), the whole word range
+            raise Unsupported("a Float64 group key is the only key on the device path")
     program = compile_predicate(frame, plan.filters)
     columns: list[str] = []
 

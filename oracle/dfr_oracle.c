@@ -89,6 +89,18 @@ static int64_t col_int(const dfo_col *c, int64_t r) {
     }
 }
 
+/* A Float64 key's canonical word (group_by.mojo:319-329, nested_keys.mojo
+ * _canonical): -0.0 and 0.0 are one key, every NaN is one key. */
+static int64_t key_word(const dfo_col *c, int64_t r) {
+    if (c->dtype != DFO_FLOAT64) return col_int(c, r);
+    double x = ((const double *)c->values)[r];
+    if (x != x) return (int64_t)0x7FF8000000000000ull;
+    if (x == 0.0) return 0;
+    int64_t w;
+    memcpy(&w, &x, 8);
+    return w;
+}
+
 static double col_f64(const dfo_col *c, int64_t r) {
     if (c->dtype == DFO_FLOAT64) return ((const double *)c->values)[r];
     return (double)col_int(c, r);
@@ -199,7 +211,7 @@ int dfo_groupby(int64_t n, int n_keys, const dfo_col *keys, const uint8_t *kept,
         if (!kept[r]) continue;
         for (int k = 0; k < n_keys; ++k) {
             kn[k] = !col_valid(&keys[k], r);
-            kv[k] = kn[k] ? 0 : col_int(&keys[k], r);
+            kv[k] = kn[k] ? 0 : key_word(&keys[k], r);
         }
         int created;
         int64_t g = table_find(&t, kv, kn, &created);
@@ -274,6 +286,10 @@ int dfo_groupby(int64_t n, int n_keys, const dfo_col *keys, const uint8_t *kept,
         for (int64_t g = 0; g < G; ++g) {
             key_valid[k * n + g] = !t.keynull[g * n_keys + k];
             key_vals[k * n + g] = t.keyvals[g * n_keys + k];
+            /* a Float64 key reports the group's first row's own value, as
+             * the reference's first() of the key (group_by.mojo:319-329) */
+            if (keys[k].dtype == DFO_FLOAT64 && !t.keynull[g * n_keys + k])
+                memcpy(&key_vals[k * n + g], &((const double *)keys[k].values)[first_rows[g]], 8);
         }
     for (int a = 0; a < n_aggs && status == 0; ++a) {
         const dfo_agg *A = &aggs[a];

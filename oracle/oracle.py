@@ -209,6 +209,8 @@ def groupby(frame, keep: np.ndarray, keys: list, aggs: list, maintain_order: boo
         valid = kval[i * cap : i * cap + G].astype(bool)
         if src.dtype == STRING:
             out.append((k, Column(STRING, vals.astype(np.uint32), valid, src.labels)))
+        elif src.dtype == FLOAT64:
+            out.append((k, Column(FLOAT64, vals.view(np.float64).copy(), valid)))
         else:
             out.append((k, Column(src.dtype, vals.astype(np.int32 if src.dtype == INT32 else np.int64), valid)))
     agg_cols = []

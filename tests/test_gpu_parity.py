@@ -376,3 +376,48 @@ def test_jit_scan_matches_precompiled(ki, fi, gpu_ctx):
         assert np.array_equal(a.is_valid(), b.is_valid()), name
     exp, _ = oracle.collect(lf)
     assert_same(got, exp, True, ("x_min", "x_max"), len(KEYS[ki]))
+
+
+@pytest.mark.parametrize("mo", [True, False])
+@pytest.mark.parametrize("setting", [{}, {"agg_mode": 0}, {"jit": 2}, {"two_level": 1}])
+def test_float_keys(mo, setting, gpu_ctx):
+    """A Float64 key: canonical groups (-0.0 == 0.0, NaNs together), the
+    first row's own value reported, nulls one group; against the oracle,
+    with the key's bits compared exactly."""
+    rng = np.random.default_rng(21)
+    n = 30000
+    base = rng.choice(np.array([0.0, -0.0, 1.5, -2.25, np.nan, np.inf, -np.inf, 3e300, 5e-324]), n)
+    base = np.where(rng.random(n) < 0.4, rng.integers(-50, 50, n) / 4.0, base)
+    df = DataFrame([Series("k", Column.float64(base, rng.random(n) > 0.03)),
+                    Series("v", Column.int64(rng.integers(-1000, 1000, n))),
+                    Series("x", Column.float64(rng.normal(size=n)))])
+    lf = df.filter(col("v") > -800).group_by("k", maintain_order=mo).agg(
+        len_(), col("v").sum().alias("v_sum"), col("x").mean().alias("x_mean"), col("x").min().alias("x_min"))
+    try:
+        for k, v in setting.items():
+            gpu_ctx.tune(k, v)
+        got = lf.collect(engine="accel")
+    finally:
+        for k, v in TUNE_DEFAULTS.items():
+            gpu_ctx.tune(k, v)
+    exp, _ = oracle.collect(lf)
+    if not mo:
+        # order is unspecified: match rows by the key's canonical word
+        def canon(c):
+            v = c.values.copy()
+            v[np.isnan(v)] = np.nan
+            w = v.view(np.uint64).copy()
+            w[v == 0] = 0
+            w[np.isnan(v)] = 0x7FF8000000000000
+            w[~c.is_valid()] = 0
+            return np.lexsort((w, ~c.is_valid()))  # nulls last; 5e-324's word is 1
+        ga, ea = canon(got.column("k").column), canon(exp.column("k").column)
+        got = DataFrame([Series(s.name, Column(s.column.dtype, s.column.values[ga], s.column.is_valid()[ga]))
+                         for s in got._series])
+        exp = DataFrame([Series(s.name, Column(s.column.dtype, s.column.values[ea], s.column.is_valid()[ea]))
+                         for s in exp._series])
+    gk, ek = got.column("k").column, exp.column("k").column
+    assert np.array_equal(gk.is_valid(), ek.is_valid())
+    m = ek.is_valid()
+    assert np.array_equal(gk.values[m].view(np.uint64), ek.values[m].view(np.uint64))
+    assert_same(got, exp, True, ("x_min",), 1)

@@ -111,14 +111,15 @@ def test_truth_tables_match_kleene_evaluation(i):
 def test_unsupported_plans_raise_before_submission():
     df = _frame(10)
     with pytest.raises(Unsupported):
-        compile_plan(df.lazy().group_by("x").agg(col("a").sum()))  # Float64 key
+        compile_plan(df.lazy().group_by("x", "a").agg(col("a").sum()))  # Float64 key beside another
     with pytest.raises(Unsupported):
         compile_plan(df.filter(col("a") > col("b")).group_by("a").agg(col("b").sum()))
     with pytest.raises(Unsupported):
         compile_plan(df.lazy().group_by("a").agg(col("s").sum()))
+    compile_plan(df.lazy().group_by("x").agg(col("a").sum()))  # a Float64 key alone is on the path
     assert provider.pipeline_supports(df.lazy().group_by("a").agg(col("b").sum()), 0)
-    assert not provider.pipeline_supports(df.lazy().group_by("x").agg(col("b").sum()), 0)
-    assert provider.describe(df.lazy().group_by("x").agg(col("b").sum())).startswith("ENGINE accel: not on")
+    assert not provider.pipeline_supports(df.lazy().group_by("x", "a").agg(col("b").sum()), 0)
+    assert provider.describe(df.lazy().group_by("x", "a").agg(col("b").sum())).startswith("ENGINE accel: not on")
 
 
 def test_provider_boundary_without_device():

@@ -254,3 +254,20 @@ def test_dd_moments_closed_form():
     assert cnt.tolist() == [10, 2]
     assert hi[0] + lo[0] == 1e9 + 45
     assert abs(var[0] - 55 / 6) < 1e-12 and var[1] == 0.5
+
+
+def test_float_keys_canonical_groups_keep_first_value():
+    """Float64 keys group by canonical words (-0.0 == 0.0, every NaN one
+    group) and report the first row's own value (group_by.mojo:319-329)."""
+    import math
+
+    df = DataFrame([
+        Series("k", Column.float64([-0.0, 0.0, math.nan, 1.5, -math.nan, 0.0, 1.5, 2.0], [True] * 7 + [False])),
+        Series("v", Column.int64([1, 2, 3, 4, 5, 6, 7, 8])),
+    ])
+    out, first = oracle.collect(df.lazy().group_by("k", maintain_order=True).agg(col("v").sum(), len_()))
+    k = out.column("k").column
+    assert list(first) == [0, 2, 3, 7]
+    assert np.signbit(k.values[0]) and k.values[0] == 0.0  # the first row's -0.0
+    assert math.isnan(k.values[1]) and k.values[2] == 1.5 and not k.is_valid()[3]
+    assert out.to_dict()["v"] == [9, 8, 11, 8] and out.to_dict()["len"] == [3, 2, 2, 1]
