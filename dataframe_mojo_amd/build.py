@@ -11,6 +11,7 @@ Run as ``python -m dataframe_mojo_amd.build`` or through
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -46,21 +47,40 @@ LIBS = {
 }
 
 
+def _run(cmd: list[str], verbose: bool, what: str) -> None:
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stderr)
+        raise RuntimeError(f"{what} failed (exit {proc.returncode})")
+
+
 def build(verbose: bool = False) -> list[str]:
-    """Compiles every library; returns their paths. Raises on failure."""
+    """Compiles every library; returns their paths. Raises on failure.
+
+    Each source compiles to its own object (in parallel; every kernel is
+    launched from the source that defines it, so the objects need no
+    device-code linking), then the objects link into the library."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    jobs = []
+    for name, spec in LIBS.items():
+        for src in spec["sources"]:
+            path = os.path.join(CSRC, src)
+            obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+            flags = [f for f in COMMON if f != "-shared"]
+            jobs.append(([HIPCC, *flags, "-c", "-o", obj, path], f"compiling {src}"))
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for f in [pool.submit(_run, cmd, verbose, what) for cmd, what in jobs]:
+            f.result()
     out = []
     for name, spec in LIBS.items():
         target = os.path.join(HERE, name)
-        sources = [os.path.join(CSRC, s) for s in spec["sources"]]
-        sources = [s for s in sources if os.path.exists(s)]
+        objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in spec["sources"]]
         tmp = target + ".tmp"
-        cmd = [HIPCC, *COMMON, *spec["extra"], "-o", tmp, *sources]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        proc = subprocess.run(cmd, capture_output=True, text=True)
-        if proc.returncode != 0:
-            sys.stderr.write(proc.stderr)
-            raise RuntimeError(f"building {name} failed (exit {proc.returncode})")
+        _run([HIPCC, *COMMON, *spec["extra"], "-o", tmp, *objs], verbose, f"linking {name}")
         os.replace(tmp, target)
         out.append(target)
     return out

@@ -297,6 +297,8 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.batch_tile = (int)value;
   else if (k == "batch_graph" && (value == 0 || value == 1))
     t.batch_graph = (int)value;
+  else if (k == "scan_prefetch" && (value == 0 || value == 1))
+    t.scan_prefetch = (int)value;
   else if (k == "jit" && value >= 0 && value <= 2)
     t.jit = (int)value;
   else if (k == "jit_rows" && value >= 0)

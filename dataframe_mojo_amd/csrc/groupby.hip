@@ -61,6 +61,7 @@ Tuning default_tuning(dfr_context *ctx) {
   t.sub_lds_bytes = std::min(env_int("DFR_SUB_LDS_BYTES", 49152), ctx->lds_per_block - 2048);
   t.sub_block = std::max(64, std::min(512, env_int("DFR_SUB_BLOCK", 256) / 64 * 64));
   t.jit = env_int("DFR_JIT", 1);
+  t.scan_prefetch = env_int("DFR_SCAN_PREFETCH", 1) ? 1 : 0;
   return t;
 }
 
@@ -524,6 +525,9 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
   int64_t blocks = (int64_t)ctx->cus * std::max(1, T.wg_per_cu);
   int64_t chunk = (rows + blocks - 1) / std::max<int64_t>(blocks, 1);
   chunk = std::max<int64_t>(step, (chunk + step - 1) / step * step);
+  // pass A addresses a workgroup's rows with 32-bit buffer offsets
+  const int64_t max_chunk = ((int64_t)1 << 28) / step * step;
+  chunk = std::min(chunk, max_chunk);
   C.chunk = chunk;
   C.grid = (int)std::max<int64_t>(1, (rows + chunk - 1) / chunk);
   C.lds_bytes = P.lds_mode == LDS_NONE

@@ -51,7 +51,7 @@ std::string header_dir() {
 }
 
 // The kernel source for this plan's shape.
-std::string shape_source(const DevPlan &P, int LDSM, int NC, int U) {
+std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF) {
   std::ostringstream o;
   o << "#define DFR_JIT 1\n#include \"kernels_part.h\"\nusing namespace dfr;\n"
     << "__device__ __forceinline__ DevPlan make_plan(const DevPlan *plan) {\n"
@@ -67,9 +67,13 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U) {
     << "  L.mode = R.mode; L.cap = R.cap; L.cap_log2 = R.cap_log2; L.max_probe = R.max_probe; L.fault = R.fault;\n"
     << "  L.n_sub = R.n_sub; L.sub_chunks = R.sub_chunks; L.base2_raw = R.base2_raw; L.base2_par = R.base2_par;\n"
     << "  L.part2_raw = R.part2_raw; L.part2_par = R.part2_par; L.gcount = R.gcount;\n";
+  // a column without a validity bitmap has a null one in the shape, so its
+  // validity code folds away
   for (int c = 0; c < P.n_cols; ++c)
-    o << "  L.cols[" << c << "].values = R.cols[" << c << "].values; L.cols[" << c << "].valid = R.cols[" << c
-      << "].valid; L.cols[" << c << "].vbit = R.cols[" << c << "].vbit;\n"
+    o << "  L.cols[" << c << "].values = R.cols[" << c << "].values; L.cols[" << c << "].valid = "
+      << (P.cols[c].valid ? "R.cols[" + std::to_string(c) + "].valid" : std::string("nullptr")) << "; L.cols[" << c
+      << "].vbit = R.cols[" << c << "].vbit;\n"
+      << (P.cols[c].valid ? "  __builtin_assume(L.cols[" + std::to_string(c) + "].valid != nullptr);\n" : "")
       << "  L.cagg[" << c << "].fshift = R.cagg[" << c << "].fshift; L.cagg[" << c << "].fshift2 = R.cagg[" << c
       << "].fshift2;\n";
   for (int k = 0; k < P.n_keys; ++k)
@@ -109,11 +113,17 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U) {
   for (int w = 0; w < P.words; ++w)
     o << "  L.ident[" << w << "] = " << P.ident[w] << "ull; L.wop[" << w << "] = " << (int)P.wop[w] << ";\n";
   o << "  return L;\n}\n";
+  // loads of one pass A tile: per row pair and column, the values (one
+  // vector load or two) and, with a bitmap, two validity words
+  int tile_loads = 0;
+  for (int c = 0; c < P.n_cols; ++c) tile_loads += (P.vec_ok ? 1 : 2) + (P.cols[c].valid ? 2 : 0);
+  tile_loads *= U;
   // pass A, pass C (one level), pass C (second level)
   o << "extern \"C\" __global__ void __launch_bounds__(1024) dfr_scan_jit(const DevPlan *__restrict__ plan, "
        "int64_t chunk) {\n"
     << "  const DevPlan L = make_plan(plan);\n"
-    << "  scan_spill_body<" << LDSM << ", " << NC << ", " << U << ">(L, chunk);\n}\n";
+    << "  scan_spill_body<" << LDSM << ", " << NC << ", " << U << ", " << (PF ? "true" : "false") << ", "
+    << tile_loads << ">(L, chunk);\n}\n";
   o << "extern \"C\" __global__ void __launch_bounds__(1024) dfr_part_jit(const DevPlan *__restrict__ plan) {\n"
     << "  const DevPlan L = make_plan(plan);\n  part_aggregate_body(L);\n}\n";
   o << "extern \"C\" __global__ void __launch_bounds__(512) dfr_sub_jit(const DevPlan *__restrict__ plan) {\n"
@@ -157,7 +167,7 @@ JitEntry compile(const std::string &src, int device) {
 // shape on the context's device, or null when it cannot be built here (then
 // the precompiled kernel runs).
 hipFunction_t jit_function(dfr_context *ctx, const DevPlan &P, int LDSM, int NC, int U, int which) {
-  const std::string src = shape_source(P, LDSM, NC, U);
+  const std::string src = shape_source(P, LDSM, NC, U, ctx->tune.scan_prefetch);
   const std::string key = std::to_string(ctx->device) + "\n" + src;
   std::lock_guard<std::mutex> g(g_mu);
   auto it = g_cache.find(key);
@@ -178,4 +188,6 @@ hipFunction_t jit_function(dfr_context *ctx, const DevPlan &P, int LDSM, int NC,
 }
 
 // The generated source of a plan's pass A (for tests and inspection).
-std::string jit_scan_source(const DevPlan &P, int LDSM, int NC, int U) { return shape_source(P, LDSM, NC, U); }
+std::string jit_scan_source(const DevPlan &P, int LDSM, int NC, int U, int PF) {
+  return shape_source(P, LDSM, NC, U, PF);
+}

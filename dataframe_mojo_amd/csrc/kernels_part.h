@@ -247,42 +247,121 @@ __device__ __forceinline__ void spill_row(const PV &P, uint64_t *lkeys, uint64_t
   }
 }
 
-// U row pairs per thread: every load of the U pairs is issued before any of
-// them is processed, so each wave keeps U * NC * 16 bytes in flight.
-template <int LDSM, int NC, int U, class PV>
-__device__ __forceinline__ void spill_tile(const PV &P, uint64_t *lkeys, uint64_t *lst, uint64_t *seg,
-                                           uint32_t &wcur, uint32_t *hraw, int64_t base, int64_t end) {
+// The raw loads of U row pairs per thread: values as loaded (32-bit
+// columns not yet widened) and the validity words holding each pair's
+// bits. Nothing here reads a loaded register, so a tile's loads stay in
+// flight until fold_tile uses them.
+template <int NC, int U>
+struct ScanTile {
   uint64_t v[U][2][NC];
-  uint32_t vb[U][2];
+  uint64_t vw[U][NC];  // validity word of row r (all ones without a bitmap)
+  uint32_t vx[U][NC];  // low half of the word after it (row r+1's bit when r's is bit 63)
   bool any[U], two[U];
+};
+
+// Tile rows base + u * 2 * blockDim + 2 * thread (+1) of the even-length
+// range [b0, end), end - b0 >= 2. A pair past the range loads the range's
+// last pair instead (its rows are not folded), so every load is issued
+// unconditionally: a branch around a load, or between two load forms,
+// makes the compiler wait for the load where the paths merge.
+template <int NC, int U, class PV>
+__device__ __forceinline__ void load_tile(const PV &P, ScanTile<NC, U> &t, int64_t base, int64_t end) {
   const int64_t stride = 2 * (int64_t)blockDim.x;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int64_t r = base + u * stride + 2 * (int64_t)threadIdx.x;
-    any[u] = r < end;
-    two[u] = r + 1 < end;
-    uint32_t valid = 0;
+    t.any[u] = t.two[u] = r < end;
+    const int64_t rc = min(r, end - 2);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      v[u][0][c] = v[u][1][c] = 0;
-      if (c >= P.n_cols || !any[u]) continue;
-      load_pair(cget(P.cols[c]), r, two[u], P.vec_ok, v[u][0][c], v[u][1][c]);
-      valid |= load_valid_pair(cget(P.cols[c]), r, two[u]) << (2 * c);
-    }
-    vb[u][0] = vb[u][1] = 0;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      vb[u][0] |= ((valid >> (2 * c)) & 1u) << c;
-      vb[u][1] |= ((valid >> (2 * c + 1)) & 1u) << c;
+      t.v[u][0][c] = t.v[u][1][c] = 0;
+      t.vw[u][c] = ~0ull;
+      t.vx[u][c] = ~0u;
+      if (c >= P.n_cols) continue;
+      const DevCol C = cget(P.cols[c]);
+      if (C.dtype == T_INT64 || C.dtype == T_FLOAT64) {
+        const DFR_GLOBAL uint64_t *p = gp((const uint64_t *)C.values) + rc;
+        if (P.vec_ok) {
+          const u64x2 q = __builtin_nontemporal_load((const DFR_GLOBAL u64x2 *)p);
+          t.v[u][0][c] = q.x;
+          t.v[u][1][c] = q.y;
+        } else {
+          t.v[u][0][c] = __builtin_nontemporal_load(p);
+          t.v[u][1][c] = __builtin_nontemporal_load(p + 1);
+        }
+      } else {
+        const DFR_GLOBAL uint32_t *p = gp((const uint32_t *)C.values) + rc;
+        if (P.vec_ok) {
+          const u32x2 q = __builtin_nontemporal_load((const DFR_GLOBAL u32x2 *)p);
+          t.v[u][0][c] = q.x;
+          t.v[u][1][c] = q.y;
+        } else {
+          t.v[u][0][c] = __builtin_nontemporal_load(p);
+          t.v[u][1][c] = __builtin_nontemporal_load(p + 1);
+        }
+      }
+      if (C.valid) {
+        const DFR_GLOBAL uint64_t *vp = gp(C.valid);
+        const int64_t w = (C.vbit + rc) >> 6, wlast = (C.vbit + end - 1) >> 6;
+        t.vw[u][c] = vp[w];
+        t.vx[u][c] = ((const DFR_GLOBAL uint32_t *)vp)[2 * min(w + 1, wlast)];
+      }
     }
   }
-  // one rolled loop over the 2U rows keeps a single copy of the row code;
-  // the row's values are picked from the tile registers with selects
-#pragma unroll 1
+}
+
+// The last row of a range of odd length: thread 0's row pair 0 holds it.
+template <int NC, class PV>
+__device__ __forceinline__ void load_last_row(const PV &P, ScanTile<NC, 1> &t, int64_t r) {
+  t.any[0] = threadIdx.x == 0;
+  t.two[0] = false;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    t.v[0][0][c] = t.v[0][1][c] = 0;
+    t.vw[0][c] = ~0ull;
+    t.vx[0][c] = ~0u;
+    if (c >= P.n_cols || !t.any[0]) continue;
+    const DevCol C = cget(P.cols[c]);
+    if (C.dtype == T_INT64 || C.dtype == T_FLOAT64)
+      t.v[0][0][c] = gp((const uint64_t *)C.values)[r];
+    else
+      t.v[0][0][c] = gp((const uint32_t *)C.values)[r];
+    if (C.valid) t.vw[0][c] = gp(C.valid)[(C.vbit + r) >> 6];
+  }
+}
+
+// A raw 32-bit value as the column's 64-bit value (Int32 sign-extends).
+__device__ __forceinline__ uint64_t widen(const DevCol C, uint64_t x) {
+  if (C.dtype == T_INT32) return (uint64_t)(int64_t)(int32_t)(uint32_t)x;
+  if (C.dtype == T_INT64 || C.dtype == T_FLOAT64) return x;
+  return (uint32_t)x;
+}
+
+// Folds a loaded tile's 2U rows: predicate, then the LDS table or a spilled
+// record. Rolled (ROLL), one loop over the rows keeps a single copy of the
+// row code and picks the row's values from the tile registers with
+// selects. The pipelined scan unrolls it: a loop that issues no loads but
+// reads registers loaded before it makes the compiler wait for every load
+// in flight at its entry, the next tile's included.
+//
+// TL > 0 is the number of loads the next tile has in flight behind this
+// one: the fold first waits until only those remain, so this tile's
+// registers are complete on every path through it (otherwise a register
+// whose load a branch skipped waiting for is still pending where it is
+// next written, and that write waits for every load in flight). The
+// compiler still inserts any wait a use needs: the count only saves waits.
+template <int LDSM, int NC, int U, bool ROLL, int TL = 0, class PV>
+__device__ __forceinline__ void fold_tile(const PV &P, const ScanTile<NC, U> &t, uint64_t *lkeys, uint64_t *lst,
+                                          uint64_t *seg, uint32_t &wcur, uint32_t *hraw, int64_t base) {
+  const int64_t stride = 2 * (int64_t)blockDim.x;
+  if constexpr (TL > 0 && TL < 64)  // gfx9 s_waitcnt: vmcnt in bits 3:0 and 15:14; expcnt and lgkmcnt not waited
+    __builtin_amdgcn_s_waitcnt((TL & 15) | (7 << 4) | (15 << 8) | ((TL >> 4) << 14));
+#pragma unroll(ROLL ? 1 : 2 * U)
   for (int i = 0; i < 2 * U; ++i) {
     uint64_t vals[NC];
     uint32_t vbits = 0;
     bool in = false;
+    const int64_t row = base + (i >> 1) * stride + 2 * (int64_t)threadIdx.x + (i & 1);
 #pragma unroll
     for (int c = 0; c < NC; ++c) vals[c] = 0;
 #pragma unroll
@@ -290,12 +369,22 @@ __device__ __forceinline__ void spill_tile(const PV &P, uint64_t *lkeys, uint64_
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         if (i != 2 * u + j) continue;
+        in = j ? t.two[u] : t.any[u];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) vals[c] = v[u][j][c];
-        vbits = vb[u][j];
-        in = j ? two[u] : any[u];
+        for (int c = 0; c < NC; ++c) {
+          if (c >= P.n_cols) continue;
+          const DevCol C = cget(P.cols[c]);
+          vals[c] = widen(C, t.v[u][j][c]);
+          uint32_t bitv;
+          if (!C.valid) {
+            bitv = 1u;
+          } else {
+            const int sh = (int)((C.vbit + row - j) & 63) + j;
+            bitv = sh < 64 ? (uint32_t)(t.vw[u][c] >> sh) & 1u : t.vx[u][c] & 1u;
+          }
+          vbits |= bitv << c;
+        }
       }
-    const int64_t row = base + (i >> 1) * stride + 2 * (int64_t)threadIdx.x + (i & 1);
     bool kept = in;
     if (in && P.n_leaves) {
       uint32_t m = 0;
@@ -324,10 +413,12 @@ __device__ __forceinline__ void spill_tile(const PV &P, uint64_t *lkeys, uint64_
 }
 
 // Pass A. Workgroup s folds rows [s * chunk, (s + 1) * chunk); NC bounds
-// the number of input columns, U the row pairs a thread loads at once. PV is
-// the plan's type: the plan in constant memory, or (runtime-compiled) a
-// register copy whose fields of the query's shape are compile-time values.
-template <int LDSM, int NC, int U, class PV>
+// the number of input columns, U the row pairs a thread loads at once, PF
+// whether the next tile's loads overlap this tile's fold, TL the loads one
+// tile issues (when known at compile time, else 0). PV is the plan's
+// type: the plan in constant memory, or (runtime-compiled) a register copy
+// whose fields of the query's shape are compile-time values.
+template <int LDSM, int NC, int U, bool PF, int TL = 0, class PV>
 __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
   extern __shared__ uint64_t smem[];
   const int E = P.lds_entries, W = P.words, NP = P.n_parts;
@@ -359,8 +450,37 @@ __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
   const int64_t b0 = (int64_t)s * chunk;
   const int64_t b1 = min(b0 + chunk, P.rows);
   const int64_t step = 2 * (int64_t)blockDim.x * U;
-  for (int64_t base = b0; base < b1; base += step)
-    spill_tile<LDSM, NC, U>(P, lkeys, lst, seg, wcur, hraw, base, b1);
+  // whole row pairs through buffer loads; an odd range's last row after
+  const int64_t b1e = b0 + (max(b1 - b0, (int64_t)0) & ~(int64_t)1);
+  if constexpr (PF) {
+    // software pipeline: the next tile's loads are issued before this
+    // tile is folded, so they are in flight during the fold. Unrolled by
+    // two so the tiles swap roles without register copies (a copy would
+    // wait for the loads it copies).
+    ScanTile<NC, U> ta, tb;
+    if (b0 < b1e) load_tile<NC, U>(P, ta, b0, b1e);
+    // (a tile past the range loads the range's last pair: in bounds)
+    for (int64_t base = b0; base < b1e;) {
+      load_tile<NC, U>(P, tb, base + step, b1e);
+      fold_tile<LDSM, NC, U, false, TL>(P, ta, lkeys, lst, seg, wcur, hraw, base);
+      base += step;
+      if (base >= b1e) break;
+      load_tile<NC, U>(P, ta, base + step, b1e);
+      fold_tile<LDSM, NC, U, false, TL>(P, tb, lkeys, lst, seg, wcur, hraw, base);
+      base += step;
+    }
+  } else {
+    for (int64_t base = b0; base < b1e; base += step) {
+      ScanTile<NC, U> t;
+      load_tile<NC, U>(P, t, base, b1e);
+      fold_tile<LDSM, NC, U, true>(P, t, lkeys, lst, seg, wcur, hraw, base);
+    }
+  }
+  if (b1e < b1) {
+    ScanTile<NC, 1> t;
+    load_last_row<NC>(P, t, b1e);
+    fold_tile<LDSM, NC, 1, true>(P, t, lkeys, lst, seg, wcur, hraw, b1e);
+  }
   if ((threadIdx.x & 63) == 0) P.cnt_raw[wseg] = wcur;
   __syncthreads();
   if (LDSM != LDS_NONE) {
@@ -387,9 +507,14 @@ __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
   if (threadIdx.x == 0) P.cnt_par[s] = cnt[1];
 }
 
+// The precompiled pass A runs unpipelined (it serves inputs below the
+// runtime-compiled kernel's row threshold, and the pipeline doubles its code
+// for every instantiation).
+constexpr bool kScanPrefetch = false;
+
 template <int LDSM, int NC, int U>
 __global__ void __launch_bounds__(1024) k_scan_spill(const DevPlan *__restrict__ plan, int64_t chunk) {
-  scan_spill_body<LDSM, NC, U>(plan_ref(plan), chunk);
+  scan_spill_body<LDSM, NC, U, kScanPrefetch>(plan_ref(plan), chunk);
 }
 
 #if !DFR_JIT
@@ -488,12 +613,18 @@ __device__ __forceinline__ void scatter_sorted(const DFR_GLOBAL uint64_t *src, u
     for (int b = threadIdx.x; b <= nb; b += kScatterThreads) cnt[b] = 0;
     __syncthreads();
     uint32_t br[IT];  // bucket << 16 | rank within the bucket
+    uint64_t w0[IT];  // first record words, all loaded before any is used
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const uint32_t i = min((uint32_t)(it * kScatterThreads + threadIdx.x), m - 1);
+      w0[it] = src[(uint64_t)(c0 + i) * rw];
+    }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const uint32_t i = it * kScatterThreads + threadIdx.x;
       br[it] = 0;
       if (i < m) {
-        const uint32_t b = bucket_of(src[(uint64_t)(c0 + i) * rw]);
+        const uint32_t b = bucket_of(w0[it]);
         br[it] = (b << 16) | atomicAdd(&cnt[b], 1u);
       }
     }
@@ -508,20 +639,38 @@ __device__ __forceinline__ void scatter_sorted(const DFR_GLOBAL uint64_t *src, u
       }
     }
     __syncthreads();
-    for (uint32_t slot = threadIdx.x; slot < m; slot += kScatterThreads) {
-      const uint32_t v = idx[slot];
-      const uint32_t b = v >> 16, i = v & 0xffffu;
-      const uint64_t pos = cur[b] + (slot - cnt[b]);
-      const DFR_GLOBAL uint64_t *r = src + (uint64_t)(c0 + i) * rw;
-      DFR_GLOBAL uint64_t *d = dst + pos * rw;
-      if constexpr (RW <= 16) {
-        uint64_t w[RW];  // every load before the stores
+    if constexpr (RW <= 16) {
+      // G records per thread at once: every load before the stores
+      constexpr int G = RW <= 4 ? 4 : (RW <= 8 ? 2 : 1);
+      for (uint32_t s0 = 0; s0 < m; s0 += G * kScatterThreads) {
+        uint64_t w[G][RW];
+        uint64_t pos[G];
 #pragma unroll
-        for (int k = 0; k < RW; ++k) w[k] = k < rw ? r[k] : 0;
+        for (int g = 0; g < G; ++g) {
+          const uint32_t slot = min(s0 + g * kScatterThreads + threadIdx.x, m - 1);
+          const uint32_t v = idx[slot];
+          const uint32_t b = v >> 16, i = v & 0xffffu;
+          pos[g] = cur[b] + (slot - cnt[b]);
+          const DFR_GLOBAL uint64_t *r = src + (uint64_t)(c0 + i) * rw;
 #pragma unroll
-        for (int k = 0; k < RW; ++k)
-          if (k < rw) d[k] = w[k];
-      } else {
+          for (int k = 0; k < RW; ++k) w[g][k] = k < rw ? r[k] : 0;
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (s0 + g * kScatterThreads + threadIdx.x >= m) continue;
+          DFR_GLOBAL uint64_t *d = dst + pos[g] * rw;
+#pragma unroll
+          for (int k = 0; k < RW; ++k)
+            if (k < rw) d[k] = w[g][k];
+        }
+      }
+    } else {
+      for (uint32_t slot = threadIdx.x; slot < m; slot += kScatterThreads) {
+        const uint32_t v = idx[slot];
+        const uint32_t b = v >> 16, i = v & 0xffffu;
+        const uint64_t pos = cur[b] + (slot - cnt[b]);
+        const DFR_GLOBAL uint64_t *r = src + (uint64_t)(c0 + i) * rw;
+        DFR_GLOBAL uint64_t *d = dst + pos * rw;
         for (int k = 0; k < rw; ++k) d[k] = r[k];
       }
     }
@@ -606,10 +755,13 @@ __device__ __forceinline__ void part_fold_records(const PV &P, uint64_t *lkeys, 
     bool okr[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
+      // past the range, the range's last record is loaded (and not folded):
+      // every load is unconditional, so all are in flight together
       const uint64_t i = i0 + (uint64_t)u * blockDim.x + threadIdx.x;
       okr[u] = i < b1;
+      const uint64_t ic = min(i, b1 - 1);
 #pragma unroll
-      for (int k = 0; k < RW; ++k) w[u][k] = (okr[u] && k < rw) ? recs[i * rw + k] : 0;
+      for (int k = 0; k < RW; ++k) w[u][k] = k < rw ? recs[ic * rw + k] : 0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -730,8 +882,15 @@ __global__ void __launch_bounds__(256) k_sub_hist(const DevPlan *__restrict__ pl
   const uint64_t keymask = (!which && P.vmask_in_key) ? ((1ull << P.key_bits) - 1) : ~0ull;
   uint64_t lo, hi;
   chunk_range(which ? P.base_par : P.base_raw, p, j, P.sub_chunks, lo, hi);
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
-    atomicAdd(&h2[sub_of(P, recs[i * rw] & keymask)], 1u);
+  constexpr int K = 8;  // records per thread at once, loads first
+  for (uint64_t i0 = lo; i0 < hi; i0 += (uint64_t)K * blockDim.x) {
+    uint64_t w0[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) w0[k] = recs[min(i0 + (uint64_t)k * blockDim.x + threadIdx.x, hi - 1) * rw];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (i0 + (uint64_t)k * blockDim.x + threadIdx.x < hi) atomicAdd(&h2[sub_of(P, w0[k] & keymask)], 1u);
+  }
   __syncthreads();
   uint32_t *out = which ? P.hist2_par : P.hist2_raw;
   for (int i = threadIdx.x; i < NS; i += blockDim.x)

@@ -38,11 +38,12 @@ struct Tuning {
   int max_parts = 4096;        // first-level key partitions, at most (a power of two)
   int sub_lds_bytes = 49152;   // LDS bytes of a second-level pass C table
   int sub_block = 256;         // threads of a second-level pass C workgroup
-  int two_level = -1;
+  int two_level = -1;          // second partition level: -1 when groups may outgrow one level, 0 never, 1 always
   int batch_tile = 131072;     // rows of a batch work item
   int batch_graph = 0;         // 1: a batch's launch sequence runs as one captured HIP graph
   int jit = 1;                 // pass A compiled for the query's shape: 0 never, 1 from jit_rows rows, 2 always
-  int64_t jit_rows = 1 << 24;          // second partition level: -1 when groups may outgrow one level, 0 never, 1 always
+  int64_t jit_rows = 1 << 24;  // rows from which jit = 1 compiles
+  int scan_prefetch = 1;       // runtime-compiled pass A: 1 overlaps the next tile's loads with this tile's fold
 };
 
 }  // namespace dfr

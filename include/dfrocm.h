@@ -124,7 +124,8 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * is captured and run as one HIP graph), "jit" (pass A compiled at run time
  * for the query's shape: 0 never, 1 when the input has at least "jit_rows"
  * rows, 2 always; the precompiled kernel runs when it cannot be built),
- * "jit_rows". */
+ * "jit_rows", "scan_prefetch" (1: the runtime-compiled pass A issues the
+ * next tile's loads before it folds the current tile; 0: load, then fold). */
 int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
 
 /* ---- device columns (the column hand-off) ------------------------------ */

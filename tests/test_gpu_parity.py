@@ -176,7 +176,8 @@ def test_random_queries(seed, fi, ki):
 
 TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26, "hash_slots": 0, "block": 512,
                  "wg_per_cu": 2, "lds_probe": 4, "scan_unroll": 1, "agg_mode": 1, "two_level": -1,
-                 "max_parts": 4096, "part_lds_bytes": 98304, "sub_lds_bytes": 49152, "sub_block": 256, "jit": 1}
+                 "max_parts": 4096, "part_lds_bytes": 98304, "sub_lds_bytes": 49152, "sub_block": 256, "jit": 1,
+                 "scan_prefetch": 1}
 
 
 @pytest.mark.parametrize("setting", [
@@ -190,6 +191,9 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
     {"two_level": 1, "sub_lds_bytes": 98304, "sub_block": 512},
     # pass A compiled at run time for the query's shape
     {"jit": 2}, {"jit": 2, "lds_mode": 0}, {"jit": 2, "two_level": 1}, {"jit": 2, "scan_unroll": 4},
+    # its loads pipelined (the default) or not, over ranges of odd and even tile counts
+    {"jit": 2, "scan_prefetch": 0}, {"jit": 2, "scan_unroll": 2, "block": 256},
+    {"jit": 2, "block": 64, "wg_per_cu": 1},
 ])
 def test_table_modes_give_identical_results(setting, gpu_ctx):
     df = random_frame(30000, 5, key_card=400)
