@@ -269,6 +269,12 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.wg_per_cu = (int)value;
   else if (k == "lds_bytes" && value >= 0 && value <= ctx->lds_per_block)
     t.lds_bytes = (int)value;
+  else if (k == "scan_block" && value >= 64 && value <= 1024 && value % 64 == 0)
+    t.scan_block = (int)value;
+  else if (k == "scan_wg_per_cu" && value >= 1 && value <= 32)
+    t.scan_wg_per_cu = (int)value;
+  else if (k == "scan_lds_bytes" && value >= 0 && value <= ctx->lds_per_block)
+    t.scan_lds_bytes = (int)value;
   else if (k == "lds_probe" && value >= 1 && value <= 64)
     t.lds_probe = (int)value;
   else if (k == "lds_mode" && value >= -1 && value <= 2)
@@ -297,6 +303,10 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.batch_tile = (int)value;
   else if (k == "batch_graph" && (value == 0 || value == 1))
     t.batch_graph = (int)value;
+  else if (k == "part_block" && value >= 64 && value <= 1024 && value % 64 == 0)
+    t.part_block = (int)value;
+  else if (k == "lds_odd_stride" && (value == 0 || value == 1))
+    t.lds_odd_stride = (int)value;
   else if (k == "scan_prefetch" && (value == 0 || value == 1))
     t.scan_prefetch = (int)value;
   else if (k == "jit" && value >= 0 && value <= 2)

@@ -55,13 +55,18 @@ Tuning default_tuning(dfr_context *ctx) {
   t.lds_probe = env_int("DFR_LDS_PROBE", 4);
   t.lds_mode = env_int("DFR_LDS_MODE", -1);
   t.agg_mode = env_int("DFR_AGG_MODE", AGG_PARTITION);
-  t.part_lds_bytes = std::min(env_int("DFR_PART_LDS_BYTES", 98304), ctx->lds_per_block);
+  t.part_lds_bytes = std::min(env_int("DFR_PART_LDS_BYTES", 131072), ctx->lds_per_block);
   t.scan_unroll = env_int("DFR_SCAN_UNROLL", 1);
   t.two_level = env_int("DFR_TWO_LEVEL", -1);
   t.sub_lds_bytes = std::min(env_int("DFR_SUB_LDS_BYTES", 49152), ctx->lds_per_block - 2048);
   t.sub_block = std::max(64, std::min(512, env_int("DFR_SUB_BLOCK", 256) / 64 * 64));
   t.jit = env_int("DFR_JIT", 1);
   t.scan_prefetch = env_int("DFR_SCAN_PREFETCH", 1) ? 1 : 0;
+  t.lds_odd_stride = env_int("DFR_LDS_ODD_STRIDE", 0) ? 1 : 0;
+  t.part_block = env_int("DFR_PART_BLOCK", 1024);
+  t.scan_block = env_int("DFR_SCAN_BLOCK", 1024);
+  t.scan_wg_per_cu = env_int("DFR_SCAN_WG_PER_CU", 1);
+  t.scan_lds_bytes = std::min(env_int("DFR_SCAN_LDS_BYTES", 159744), ctx->lds_per_block);
   return t;
 }
 
@@ -415,6 +420,7 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
   const int64_t rows = C.rows;
   const uint64_t D = P.domain;
   const size_t slot_bytes = (size_t)P.words * 8;
+  P.lds_stride = T.lds_odd_stride ? (P.words | 1) : P.words;
   const bool direct_fits = D <= (uint64_t)T.direct_max_slots &&
                            (int64_t)(D * slot_bytes) <= std::max<int64_t>(ctx->pool.budget / 4, 1 << 20);
   if (direct_fits) {
@@ -458,12 +464,14 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
   P.lds_probe = T.lds_probe;
   P.agg_mode = T.agg_mode;
   if (P.agg_mode == AGG_PARTITION) {
-    // partition tables: E entries of (key + W words) within part_lds_bytes
+    // partition tables: E entries of (key + W words, at an odd stride)
+    // within part_lds_bytes
     const int W = P.words;
+    const int S = P.lds_stride;
     uint64_t ce = 64;
     // (2 KiB of LDS stay free for the second level's static arrays)
     const uint64_t c_budget = (uint64_t)std::min(T.part_lds_bytes, ctx->lds_per_block - 2048);
-    while (ce * 2 * 8ull * (W + 1) <= c_budget && ce < (uint64_t)kMaxPartEntries) ce *= 2;
+    while (ce * 2 * 8ull * (S + 1) <= c_budget && ce < (uint64_t)kMaxPartEntries) ce *= 2;
     P.c_entries = (int32_t)ce;
     P.c_log2 = log2_exact(ce);
     const uint64_t groups_est = std::max<uint64_t>(1, std::min<uint64_t>(D, (uint64_t)std::max<int64_t>(rows, 1)));
@@ -486,16 +494,16 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
       P.part_log2 = log2_exact(max_parts);
       // smaller tables than one level's: more workgroups per CU
       uint64_t e2 = 64;
-      while (e2 * 2 * 8ull * (W + 1) <= (uint64_t)T.sub_lds_bytes && e2 < (uint64_t)kMaxPartEntries) e2 *= 2;
+      while (e2 * 2 * 8ull * (S + 1) <= (uint64_t)T.sub_lds_bytes && e2 < (uint64_t)kMaxPartEntries) e2 *= 2;
       P.c_entries = (int32_t)e2;
       P.c_log2 = log2_exact(e2);
       P.mode = MODE_HASH;
       P.cap = std::max<uint64_t>(1, std::min<uint64_t>(D, (uint64_t)std::max<int64_t>(rows, 1)));
     }
-    // the scan's LDS table (full state words) next to two histograms
-    const int hist_bytes = (2 * P.n_parts + 2) * 4;
-    const int avail = std::max(0, std::min(T.lds_bytes, ctx->lds_per_block) - hist_bytes);
-    const uint64_t direct_e = (uint64_t)avail / (8ull * W);
+    // the scan's LDS table (full state words) next to its histogram
+    const int hist_bytes = (P.n_parts + 2) * 4;
+    const int avail = std::max(0, std::min(T.scan_lds_bytes, ctx->lds_per_block) - hist_bytes);
+    const uint64_t direct_e = (uint64_t)avail / (8ull * S);
     int m = T.lds_mode;
     if (m < 0) m = (D <= direct_e) ? LDS_DIRECT : LDS_HASHED;
     if (m == LDS_DIRECT && D > direct_e) m = LDS_HASHED;
@@ -504,25 +512,29 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
       P.lds_entries = (int32_t)std::max<uint64_t>(D, 1);
       P.lds_log2 = 1;
     } else if (m == LDS_HASHED) {
-      uint64_t e = 1;
-      while (e * 2 * 8ull * (W + 1) <= (uint64_t)avail && e < 16384) e *= 2;
-      if (e < 64) {
+      // as many 4-entry buckets as the LDS holds (the table caches the
+      // first keys to arrive; a full bucket's rows spill)
+      const uint64_t nb = std::min<uint64_t>((uint64_t)avail / (4 * 8ull * (S + 1)), 16384 / 4);
+      if (nb < 16) {
         P.lds_mode = LDS_NONE;
         P.lds_entries = 0;
       } else {
         P.lds_mode = LDS_HASHED;
-        P.lds_entries = (int32_t)e;
-        P.lds_log2 = log2_exact(e);
+        P.lds_buckets = (int32_t)nb;
+        P.lds_entries = (int32_t)(4 * nb);
+        P.lds_log2 = 0;
       }
     } else {
       P.lds_mode = LDS_NONE;
       P.lds_entries = 0;
     }
   }
-  // launch shape: persistent workgroups over contiguous row chunks
-  C.block = std::max(64, std::min(1024, T.block));
+  // launch shape: persistent workgroups over contiguous row chunks (the
+  // partitioned pass A and the one-pass kernel have their own)
+  const bool part = P.agg_mode == AGG_PARTITION;
+  C.block = std::max(64, std::min(1024, part ? T.scan_block : T.block));
   int64_t step = 8 * (int64_t)C.block;  // 2 rows x up to 4 pairs per thread
-  int64_t blocks = (int64_t)ctx->cus * std::max(1, T.wg_per_cu);
+  int64_t blocks = (int64_t)ctx->cus * std::max(1, part ? T.scan_wg_per_cu : T.wg_per_cu);
   int64_t chunk = (rows + blocks - 1) / std::max<int64_t>(blocks, 1);
   chunk = std::max<int64_t>(step, (chunk + step - 1) / step * step);
   // pass A addresses a workgroup's rows with 32-bit buffer offsets
@@ -538,11 +550,11 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
     P.seg_waves = C.block / 64;
     P.seg_rows = chunk / P.seg_waves;  // a wave sees chunk / waves rows
     P.seg_partials = std::max(P.lds_entries, 1);
-    const int W = P.words;
+    const int S = P.lds_stride;
     C.lds_bytes = (P.lds_mode == LDS_NONE ? 0
-                                          : (size_t)P.lds_entries * 8 * (W + (P.lds_mode == LDS_HASHED ? 1 : 0))) +
-                  (size_t)(2 * P.n_parts + 2) * 4;
-    C.c_lds_bytes = (size_t)P.c_entries * 8 * (W + 1);
+                                          : (size_t)P.lds_entries * 8 * (S + (P.lds_mode == LDS_HASHED ? 1 : 0))) +
+                  (size_t)(P.n_parts + 2) * 4;
+    C.c_lds_bytes = (size_t)P.c_entries * 8 * (S + 1);
   }
 }
 
@@ -685,9 +697,10 @@ int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan
   ctx->obs.launches += 3;
   if (!P.two_level) {
     void *args[] = {(void *)&dplan};
-    if (!jit_launch(ctx, C, JIT_PART, P.n_parts, 512, C.c_lds_bytes, args)) {
+    const int pb = ctx->tune.part_block;
+    if (!jit_launch(ctx, C, JIT_PART, P.n_parts, pb, C.c_lds_bytes, args)) {
       if ((st = set_lds(k_part_aggregate, C.c_lds_bytes, error))) return st;
-      hipLaunchKernelGGL(k_part_aggregate, dim3(P.n_parts), dim3(512), C.c_lds_bytes, ctx->stream, dplan);
+      hipLaunchKernelGGL(k_part_aggregate, dim3(P.n_parts), dim3(pb), C.c_lds_bytes, ctx->stream, dplan);
     }
     ctx->obs.launches += 1;
   }

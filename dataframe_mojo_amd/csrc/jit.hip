@@ -58,7 +58,7 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF) {
     << "  PlanRef R = plan_ref(plan);\n"
     << "  DevPlan L{};\n";
   // values of the data and of the query's constants: from the plan
-  o << "  L.rows = R.rows; L.lds_entries = R.lds_entries; L.lds_log2 = R.lds_log2;\n"
+  o << "  L.rows = R.rows; L.lds_entries = R.lds_entries; L.lds_log2 = R.lds_log2; L.lds_buckets = R.lds_buckets;\n"
     << "  L.n_parts = R.n_parts; L.part_log2 = R.part_log2; L.key_bits = R.key_bits;\n"
     << "  L.spill = R.spill; L.partials = R.partials; L.seg_rows = R.seg_rows; L.seg_partials = R.seg_partials;\n"
     << "  L.hist_raw = R.hist_raw; L.hist_par = R.hist_par; L.cnt_raw = R.cnt_raw; L.cnt_par = R.cnt_par;\n"
@@ -89,7 +89,7 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF) {
     << ";\n";
   o << "  L.rw = " << P.rw << "; L.n_vals = " << P.n_vals << "; L.rec_row = " << P.rec_row
     << "; L.rec_meta = " << P.rec_meta << "; L.rec_val0 = " << P.rec_val0 << "; L.vmask_in_key = " << P.vmask_in_key
-    << ";\n";
+    << "; L.lds_stride = " << P.lds_stride << ";\n";
   for (int i = 0; i < 8; ++i) o << "  L.truth[" << i << "] = " << P.truth[i] << "u;\n";
   for (int c = 0; c < P.n_cols; ++c) {
     const DevColAgg &A = P.cagg[c];

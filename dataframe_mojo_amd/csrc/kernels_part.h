@@ -167,7 +167,7 @@ __device__ __forceinline__ uint32_t wave_append(bool take, uint32_t *counter) {
 template <int LDSM, class PV>
 __device__ __forceinline__ int bucket_find(const PV &P, uint64_t *lkeys, uint64_t key) {
   if (LDSM == LDS_DIRECT) return (int)key;
-  const int b = (int)((key32(key) * 0x9E3779B1u) >> (32 - (P.lds_log2 - 2)));
+  const int b = (int)(((uint64_t)(key32(key) * 0x9E3779B1u) * (uint32_t)P.lds_buckets) >> 32);
   uint64_t *kb = lkeys + 4 * b;
   uint64_t k[4];
 #pragma unroll
@@ -223,7 +223,7 @@ __device__ __forceinline__ void spill_row(const PV &P, uint64_t *lkeys, uint64_t
     }
     if (LDSM != LDS_NONE) e = bucket_find<LDSM>(P, lkeys, packed);
     if (e >= 0)
-      fold_full<NC>(P, lst + (int64_t)e * P.words, row, [&](int c) { return vals[c]; },
+      fold_full<NC>(P, lst + (int64_t)e * P.lds_stride, row, [&](int c) { return vals[c]; },
                     [&](int c) { return ((vbits >> c) & 1u) != 0; });
   }
   // the wave appends to its own segment: cursor uniform across the wave
@@ -424,22 +424,25 @@ __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
   const int E = P.lds_entries, W = P.words, NP = P.n_parts;
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + (LDSM == LDS_HASHED ? E : 0);
-  uint32_t *hraw = (uint32_t *)(lst + (LDSM == LDS_NONE ? 0 : (size_t)E * W));
-  uint32_t *hpar = hraw + NP;
-  uint32_t *cnt = hpar + NP;  // [0] raw records, [1] partial records
+  const int S = P.lds_stride;
+  // one per-partition histogram: raw records during the scan, then (once
+  // written out) the flushed partial states
+  uint32_t *hraw = (uint32_t *)(lst + (LDSM == LDS_NONE ? 0 : (size_t)E * S));
+  uint32_t *hpar = hraw;
+  uint32_t *cnt = hraw + NP;  // [0] raw records, [1] partial records
   if (LDSM != LDS_NONE) {
     for (int i = threadIdx.x; i < E; i += blockDim.x) {
       if (LDSM == LDS_HASHED) lkeys[i] = kEmpty;
       if constexpr (RegPlan<PV>::value) {
 #pragma unroll
         for (int w = 0; w < kMaxWords; ++w)
-          if (w < W) lst[i * W + w] = P.ident[w];
+          if (w < W) lst[i * S + w] = P.ident[w];
       } else {
-        for (int w = 0; w < W; ++w) lst[i * W + w] = P.ident[w];
+        for (int w = 0; w < W; ++w) lst[i * S + w] = P.ident[w];
       }
     }
   }
-  for (int i = threadIdx.x; i < 2 * NP + 2; i += blockDim.x) hraw[i] = 0;
+  for (int i = threadIdx.x; i < NP + 2; i += blockDim.x) hraw[i] = 0;
   __syncthreads();
   const int s = blockIdx.x;
   // wave segment: the rows a wave sees, seg_rows records at most
@@ -483,11 +486,16 @@ __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
   }
   if ((threadIdx.x & 63) == 0) P.cnt_raw[wseg] = wcur;
   __syncthreads();
+  for (int p = threadIdx.x; p < NP; p += blockDim.x) {
+    P.hist_raw[(uint64_t)s * NP + p] = hraw[p];
+    hpar[p] = 0;
+  }
+  __syncthreads();
   if (LDSM != LDS_NONE) {
     uint64_t *pseg = P.partials + (uint64_t)s * (uint64_t)P.seg_partials * (1 + W);
     const int Eround = (E + 63) & ~63;
     for (int e = threadIdx.x; e < Eround; e += blockDim.x) {
-      const uint64_t *st = lst + (int64_t)e * W;
+      const uint64_t *st = lst + (int64_t)e * S;
       const bool occ = e < E && st[0] != 0;
       const uint64_t key = (LDSM == LDS_DIRECT) ? (uint64_t)e : (e < E ? lkeys[e] : 0);
       const uint32_t at = wave_append(occ, &cnt[1]);
@@ -500,10 +508,7 @@ __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
     }
   }
   __syncthreads();
-  for (int p = threadIdx.x; p < NP; p += blockDim.x) {
-    P.hist_raw[(uint64_t)s * NP + p] = hraw[p];
-    P.hist_par[(uint64_t)s * NP + p] = hpar[p];
-  }
+  for (int p = threadIdx.x; p < NP; p += blockDim.x) P.hist_par[(uint64_t)s * NP + p] = hpar[p];
   if (threadIdx.x == 0) P.cnt_par[s] = cnt[1];
 }
 
@@ -782,7 +787,7 @@ __device__ __forceinline__ void part_fold_records(const PV &P, uint64_t *lkeys, 
       auto ok = [&](int c) { return ((vmask >> P.val_of[c]) & 1ull) != 0; };
       const int e = part_find(P, lkeys, key);
       if (e >= 0) {
-        fold_full(P, lst + (int64_t)e * W, row, val, ok);
+        fold_full(P, lst + (int64_t)e * P.lds_stride, row, val, ok);
       } else if (DENSE) {
         atomicOr(&P.fault[3], 1u);
       } else {
@@ -797,7 +802,7 @@ __device__ __forceinline__ void part_fold_records(const PV &P, uint64_t *lkeys, 
 template <class PV>
 __device__ __forceinline__ void part_aggregate_body(const PV &P) {
   extern __shared__ uint64_t smem[];
-  const int p = blockIdx.x, E = P.c_entries, W = P.words;
+  const int p = blockIdx.x, E = P.c_entries, W = P.words, S = P.lds_stride;
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + E;
   for (int i = threadIdx.x; i < E; i += blockDim.x) {
@@ -805,9 +810,9 @@ __device__ __forceinline__ void part_aggregate_body(const PV &P) {
     if constexpr (RegPlan<PV>::value) {
 #pragma unroll
       for (int w = 0; w < kMaxWords; ++w)
-        if (w < W) lst[i * W + w] = P.ident[w];
+        if (w < W) lst[i * S + w] = P.ident[w];
     } else {
-      for (int w = 0; w < W; ++w) lst[i * W + w] = P.ident[w];
+      for (int w = 0; w < W; ++w) lst[i * S + w] = P.ident[w];
     }
   }
   __syncthreads();
@@ -825,7 +830,7 @@ __device__ __forceinline__ void part_aggregate_body(const PV &P) {
     const uint64_t key = rec[0];
     const int e = part_find(P, lkeys, key);
     if (e >= 0) {
-      merge_full(P, lst + (int64_t)e * W, rec + 1);
+      merge_full(P, lst + (int64_t)e * S, rec + 1);
     } else {
       const uint64_t g = global_slot(P, key);
       if (g != kEmpty) merge_full(P, P.state + g * (uint64_t)W, rec + 1);
@@ -839,7 +844,7 @@ __device__ __forceinline__ void part_aggregate_body(const PV &P) {
     const uint64_t g = global_slot(P, key);
     if (g == kEmpty) continue;
     uint64_t *d = P.state + g * (uint64_t)W;
-    const uint64_t *s = lst + (int64_t)e * W;
+    const uint64_t *s = lst + (int64_t)e * S;
     for (int w = 0; w < W; ++w) d[w] = s[w];
   }
 }
@@ -974,7 +979,7 @@ __device__ __forceinline__ void sub_aggregate_body(const PV &P) {
   __shared__ uint32_t wsum[kMaxPartEntries / 64];
   __shared__ uint64_t gbase;
   const uint64_t sp = blockIdx.x;
-  const int E = P.c_entries, W = P.words, CH = P.sub_chunks;
+  const int E = P.c_entries, W = P.words, S = P.lds_stride, CH = P.sub_chunks;
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + E;
   for (int i = threadIdx.x; i < E; i += blockDim.x) {
@@ -982,9 +987,9 @@ __device__ __forceinline__ void sub_aggregate_body(const PV &P) {
     if constexpr (RegPlan<PV>::value) {
 #pragma unroll
       for (int w = 0; w < kMaxWords; ++w)
-        if (w < W) lst[i * W + w] = P.ident[w];
+        if (w < W) lst[i * S + w] = P.ident[w];
     } else {
-      for (int w = 0; w < W; ++w) lst[i * W + w] = P.ident[w];
+      for (int w = 0; w < W; ++w) lst[i * S + w] = P.ident[w];
     }
   }
   __syncthreads();
@@ -1002,7 +1007,7 @@ __device__ __forceinline__ void sub_aggregate_body(const PV &P) {
     const DFR_GLOBAL uint64_t *rec = gp(P.part2_par) + i * (1 + W);
     const int e = part_find(P, lkeys, rec[0]);
     if (e >= 0)
-      merge_full(P, lst + (int64_t)e * W, rec + 1);
+      merge_full(P, lst + (int64_t)e * S, rec + 1);
     else
       atomicOr(&P.fault[3], 1u);
   }
@@ -1035,7 +1040,7 @@ __device__ __forceinline__ void sub_aggregate_body(const PV &P) {
     const uint64_t slot = gbase + wsum[c] + (uint64_t)__popcll(ball & lt);
     P.keys_tab[slot] = key;
     uint64_t *d = P.state + slot * (uint64_t)W;
-    const uint64_t *s = lst + (int64_t)e * W;
+    const uint64_t *s = lst + (int64_t)e * S;
     for (int w = 0; w < W; ++w) d[w] = s[w];
   }
 }

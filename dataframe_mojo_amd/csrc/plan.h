@@ -204,6 +204,13 @@ struct DevPlan {
   int32_t two_level;
   int32_t n_sub;
   int32_t sub_chunks;
+  // words between consecutive entries of the partitioned path's LDS tables:
+  // words rounded up to odd, so 64-bit accesses of entries spread over
+  // every LDS bank pair
+  int32_t lds_stride;
+  // pass A's hashed LDS table: 4-entry buckets, any count (a key's bucket
+  // is the high half of its 32-bit hash times the count)
+  int32_t lds_buckets;
   uint32_t *hist2_raw;
   uint32_t *hist2_par;
   uint64_t *base2_raw;

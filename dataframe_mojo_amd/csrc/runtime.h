@@ -24,16 +24,19 @@ struct ColStats {
 // Launch and table parameters. Defaults come from the device; the autotuner
 // overrides them per context (dfr_context_tune).
 struct Tuning {
-  int block = 512;             // threads per workgroup of the scan
-  int wg_per_cu = 2;           // persistent workgroups per CU
-  int lds_bytes = 65536;       // LDS table bytes per workgroup
+  int block = 512;             // threads per workgroup of the one-pass (atomic) scan and of a batch
+  int wg_per_cu = 2;           // their persistent workgroups per CU
+  int lds_bytes = 65536;       // their LDS table bytes per workgroup
+  int scan_block = 1024;       // the partitioned pass A: threads per workgroup
+  int scan_wg_per_cu = 1;      // its persistent workgroups per CU
+  int scan_lds_bytes = 159744; // its LDS bytes per workgroup (table and histogram)
   int lds_probe = 4;           // LDS probes before a row goes to HBM
   int lds_mode = -1;           // -1 automatic, else LDS_NONE/LDS_DIRECT/LDS_HASHED
   int64_t direct_max_slots = int64_t(1) << 26;  // largest direct-addressed HBM table
   int hash_probe = 1 << 20;    // HBM probes before a capacity fault
   int64_t hash_slots = 0;      // 0: sized from the key domain and the rows
   int agg_mode = 1;            // AGG_PARTITION (1) or AGG_ATOMIC (0)
-  int part_lds_bytes = 98304;  // LDS bytes of a partition table (pass C)
+  int part_lds_bytes = 131072; // LDS bytes of a partition table (pass C)
   int scan_unroll = 1;         // row pairs a scan thread loads at once (1, 2, 4)
   int max_parts = 4096;        // first-level key partitions, at most (a power of two)
   int sub_lds_bytes = 49152;   // LDS bytes of a second-level pass C table
@@ -44,6 +47,8 @@ struct Tuning {
   int jit = 1;                 // pass A compiled for the query's shape: 0 never, 1 from jit_rows rows, 2 always
   int64_t jit_rows = 1 << 24;  // rows from which jit = 1 compiles
   int scan_prefetch = 1;       // runtime-compiled pass A: 1 overlaps the next tile's loads with this tile's fold
+  int lds_odd_stride = 0;      // partitioned path: LDS table entries at an odd number of words
+  int part_block = 1024;       // threads of a first-level pass C workgroup
 };
 
 }  // namespace dfr

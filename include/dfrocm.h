@@ -110,7 +110,10 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
 
 /* Sets one launch or table parameter of the context (no counterpart in the
  * reference: the autotuner's knob). Keys: "block", "wg_per_cu",
- * "lds_bytes", "lds_probe", "lds_mode" (-1 auto, 0 none, 1 direct,
+ * "lds_bytes" (threads per workgroup, persistent workgroups per CU and LDS
+ * table bytes of the one-pass atomic scan and of a batch), "scan_block",
+ * "scan_wg_per_cu", "scan_lds_bytes" (the same for the partitioned path's
+ * pass A), "lds_probe", "lds_mode" (-1 auto, 0 none, 1 direct,
  * 2 hashed), "direct_max_slots", "hash_probe", "hash_slots", "agg_mode"
  * (1 partitioned: rows that miss the LDS table are spilled and folded per
  * key partition; 0 atomic: they fold into HBM with device atomics),
@@ -125,7 +128,10 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * for the query's shape: 0 never, 1 when the input has at least "jit_rows"
  * rows, 2 always; the precompiled kernel runs when it cannot be built),
  * "jit_rows", "scan_prefetch" (1: the runtime-compiled pass A issues the
- * next tile's loads before it folds the current tile; 0: load, then fold). */
+ * next tile's loads before it folds the current tile; 0: load, then fold),
+ * "lds_odd_stride" (1: the partitioned path's LDS table entries are an odd
+ * number of 8-byte words apart; 0: exactly the state's words),
+ * "part_block" (threads of a first-level partition table's workgroup). */
 int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
 
 /* ---- device columns (the column hand-off) ------------------------------ */

@@ -176,14 +176,15 @@ def test_random_queries(seed, fi, ki):
 
 TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26, "hash_slots": 0, "block": 512,
                  "wg_per_cu": 2, "lds_probe": 4, "scan_unroll": 1, "agg_mode": 1, "two_level": -1,
-                 "max_parts": 4096, "part_lds_bytes": 98304, "sub_lds_bytes": 49152, "sub_block": 256, "jit": 1,
-                 "scan_prefetch": 1}
+                 "max_parts": 4096, "part_lds_bytes": 131072, "sub_lds_bytes": 49152, "sub_block": 256, "jit": 1,
+                 "scan_prefetch": 1, "lds_odd_stride": 0, "part_block": 1024, "scan_block": 1024,
+                 "scan_wg_per_cu": 1, "scan_lds_bytes": 159744}
 
 
 @pytest.mark.parametrize("setting", [
     {"lds_mode": 0}, {"lds_mode": 2}, {"lds_mode": 2, "lds_bytes": 8192}, {"direct_max_slots": 0},
     {"direct_max_slots": 0, "lds_mode": 0}, {"direct_max_slots": 0, "hash_slots": 64},
-    {"block": 256, "wg_per_cu": 1}, {"block": 1024, "wg_per_cu": 4}, {"lds_probe": 1},
+    {"block": 256, "wg_per_cu": 1, "agg_mode": 0}, {"scan_block": 1024, "scan_wg_per_cu": 4}, {"lds_probe": 1},
     {"scan_unroll": 2}, {"scan_unroll": 4}, {"agg_mode": 0},
     {"two_level": 1}, {"two_level": 1, "lds_mode": 0}, {"two_level": 1, "scan_unroll": 4},
     # one first-level partition, many small second-level tables
@@ -192,8 +193,14 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
     # pass A compiled at run time for the query's shape
     {"jit": 2}, {"jit": 2, "lds_mode": 0}, {"jit": 2, "two_level": 1}, {"jit": 2, "scan_unroll": 4},
     # its loads pipelined (the default) or not, over ranges of odd and even tile counts
-    {"jit": 2, "scan_prefetch": 0}, {"jit": 2, "scan_unroll": 2, "block": 256},
-    {"jit": 2, "block": 64, "wg_per_cu": 1},
+    {"jit": 2, "scan_prefetch": 0}, {"jit": 2, "scan_unroll": 2, "scan_block": 256},
+    {"jit": 2, "scan_block": 64, "scan_wg_per_cu": 1},
+    {"lds_odd_stride": 1}, {"lds_odd_stride": 1, "jit": 2, "two_level": 1},
+    {"part_block": 512}, {"part_block": 64, "jit": 2},
+    # pass A's launch shape and LDS budget (bucket counts that are not powers of two)
+    {"scan_block": 256, "scan_wg_per_cu": 2, "scan_lds_bytes": 40000},
+    {"scan_block": 512, "scan_wg_per_cu": 4, "scan_lds_bytes": 65536, "jit": 2},
+    {"scan_block": 64, "scan_lds_bytes": 12000, "lds_mode": 2},
 ])
 def test_table_modes_give_identical_results(setting, gpu_ctx):
     df = random_frame(30000, 5, key_card=400)
