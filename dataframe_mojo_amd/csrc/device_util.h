@@ -163,6 +163,29 @@ DFR_HD double i128_to_f64(uint64_t lo, uint64_t hi, int scale) {
 }
 
 // 256-bit unsigned arithmetic for the variance finish.
+// q = n / d, r = n % d for d >= 1, given inv = 1 / d rounded to double.
+// Below 2^53 the double estimate of n / d is within one of the quotient (n
+// is exact, and the two roundings change n / d by at most 2 / d), so one
+// correction step makes it exact; larger values use integer division.
+DFR_HD void divmod_u64(uint64_t n, uint64_t d, double inv, uint64_t &q, uint64_t &r) {
+  if (n < (1ull << 53) && d < (1ull << 53)) {
+    uint64_t e = (uint64_t)((double)n * inv);
+    int64_t rr = (int64_t)(n - e * d);
+    if (rr < 0) {
+      e -= 1;
+      rr += (int64_t)d;
+    } else if (rr >= (int64_t)d) {
+      e += 1;
+      rr -= (int64_t)d;
+    }
+    q = e;
+    r = (uint64_t)rr;
+  } else {
+    q = n / d;
+    r = n - q * d;
+  }
+}
+
 struct U256 {
   uint64_t w[4];
 };

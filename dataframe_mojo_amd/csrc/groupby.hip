@@ -59,11 +59,12 @@ Tuning default_tuning(dfr_context *ctx) {
   t.scan_unroll = env_int("DFR_SCAN_UNROLL", 1);
   t.two_level = env_int("DFR_TWO_LEVEL", -1);
   t.sub_lds_bytes = std::min(env_int("DFR_SUB_LDS_BYTES", 49152), ctx->lds_per_block - 2048);
-  t.sub_block = std::max(64, std::min(512, env_int("DFR_SUB_BLOCK", 256) / 64 * 64));
+  t.sub_block = std::max(64, std::min(512, env_int("DFR_SUB_BLOCK", 512) / 64 * 64));
   t.jit = env_int("DFR_JIT", 1);
   t.scan_prefetch = env_int("DFR_SCAN_PREFETCH", 1) ? 1 : 0;
   t.lds_odd_stride = env_int("DFR_LDS_ODD_STRIDE", 0) ? 1 : 0;
   t.part_block = env_int("DFR_PART_BLOCK", 1024);
+  t.two_level_parts = env_int("DFR_TWO_LEVEL_PARTS", 2048);
   t.scan_block = env_int("DFR_SCAN_BLOCK", 1024);
   t.scan_wg_per_cu = env_int("DFR_SCAN_WG_PER_CU", 1);
   t.scan_lds_bytes = std::min(env_int("DFR_SCAN_LDS_BYTES", 159744), ctx->lds_per_block);
@@ -190,6 +191,7 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
     if (radix > ((unsigned __int128)1 << 62))
       return dfr_fail(error, DFR_UNSUPPORTED, "key %d spans more than 2^62 values", k);
     K.radix = (uint64_t)radix;
+    K.inv_radix = 1.0 / (double)K.radix;
     K.stride = (uint64_t)domain;
     domain *= radix;
     if (domain > ((unsigned __int128)1 << 62))
@@ -490,8 +492,9 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
     const bool two = T.two_level > 0 || (T.two_level < 0 && 2 * groups_est > max_parts * ce);
     if (two) {
       P.two_level = 1;
-      P.n_parts = (int32_t)max_parts;
-      P.part_log2 = log2_exact(max_parts);
+      const uint64_t np2 = std::min<uint64_t>(max_parts, (uint64_t)T.two_level_parts);
+      P.n_parts = (int32_t)np2;
+      P.part_log2 = log2_exact(np2);
       // smaller tables than one level's: more workgroups per CU
       uint64_t e2 = 64;
       while (e2 * 2 * 8ull * (S + 1) <= (uint64_t)T.sub_lds_bytes && e2 < (uint64_t)kMaxPartEntries) e2 *= 2;
@@ -804,15 +807,86 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
   for (int attempt = 0;; ++attempt) {
     choose_layout(ctx, C, hash_slots, sub_boost);
     const bool dense = P.two_level != 0;
+    // unordered second level: pass C writes the output rows itself, so
+    // there are no state or key tables and no finalize pass
+    const bool fused = dense && !C.maintain_order;
+    P.fused = 0;
     Scratch S(ctx);
     int st;
     DevPlan *dplan;
     uint32_t *fault;
-    if ((st = S.get(1, &dplan, error)) || (st = S.get(4, &fault, error)) ||
-        (st = S.get(P.cap * (uint64_t)P.words, &P.state, error)))
-      return st;
-    if (P.mode == MODE_HASH && (st = S.get(P.cap, &P.keys_tab, error))) return st;
-    P.keys_tab = P.mode == MODE_HASH ? P.keys_tab : nullptr;
+    dfr_result *r = nullptr;  // released on every early return
+    struct Release {
+      dfr_result *&r;
+      ~Release() {
+        if (r) dfr_result_release(r);
+      }
+    } release{r};
+    // the result's columns (and first rows when ordered) for n groups at
+    // most, in one pool block; their addresses go into the plan
+    auto alloc_result = [&](uint64_t n) -> int {
+      r = new dfr_result();
+      r->ctx = ctx;
+      r->groups = (int64_t)n;
+      r->key_dtypes = C.key_dtypes;
+      r->agg_dtypes = C.agg_dtypes;
+      const uint64_t vwords = (n + 63) / 64 + 1;
+      size_t bytes = 0;
+      auto col_bytes = [&](int dtype) {
+        size_t b = (size_t)n * ((dtype == DFR_INT32 || dtype == DFR_DICT) ? 4 : 8);
+        return (b + 255) / 256 * 256 + vwords * 8;
+      };
+      for (int32_t d : C.key_dtypes) bytes += col_bytes(d);
+      for (int32_t d : C.agg_dtypes) bytes += col_bytes(d);
+      if (C.maintain_order) bytes += (size_t)n * 8 + 256;
+      int st2 = ctx->pool.alloc(bytes, &r->buffer, error);
+      if (st2) {
+        delete r;
+        r = nullptr;
+        return st2;
+      }
+      r->bytes = bytes;
+      char *cur = (char *)r->buffer;
+      auto take = [&](int dtype, void **vals, uint64_t **valid) {
+        size_t b = (size_t)n * ((dtype == DFR_INT32 || dtype == DFR_DICT) ? 4 : 8);
+        *vals = cur;
+        cur += (b + 255) / 256 * 256;
+        *valid = (uint64_t *)cur;
+        cur += vwords * 8;
+      };
+      for (size_t k = 0; k < C.key_dtypes.size(); ++k) {
+        void *v;
+        uint64_t *m;
+        take(C.key_dtypes[k], &v, &m);
+        r->key_values.push_back(v);
+        r->key_valid.push_back(m);
+        P.out_keys[k] = v;
+        P.out_key_valid[k] = m;
+      }
+      for (size_t a = 0; a < C.agg_dtypes.size(); ++a) {
+        void *v;
+        uint64_t *m;
+        take(C.agg_dtypes[a], &v, &m);
+        r->agg_values.push_back(v);
+        r->agg_valid.push_back(m);
+        P.aggs[a].out_values = v;
+        P.aggs[a].out_valid = m;
+      }
+      if (C.maintain_order) {
+        r->first_rows = (uint64_t *)cur;
+        P.out_first = r->first_rows;
+      } else {
+        P.out_first = nullptr;
+      }
+      return DFR_OK;
+    };
+    if ((st = S.get(1, &dplan, error)) || (st = S.get(4, &fault, error))) return st;
+    P.state = nullptr;
+    P.keys_tab = nullptr;
+    if (!fused) {
+      if ((st = S.get(P.cap * (uint64_t)P.words, &P.state, error))) return st;
+      if (P.mode == MODE_HASH && (st = S.get(P.cap, &P.keys_tab, error))) return st;
+    }
     if (P.agg_mode == AGG_PARTITION) {
       const uint64_t S_ = (uint64_t)P.n_segs, NP = (uint64_t)P.n_parts, W = (uint64_t)P.words;
       const uint64_t raw_words = S_ * (uint64_t)P.seg_waves * (uint64_t)P.seg_rows * (uint64_t)P.rw;
@@ -867,6 +941,14 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
             (st = S.get(N2 + 1, &P.base2_raw, error)) || (st = S.get(N2 + 1, &P.base2_par, error)) ||
             (st = S.get(nb + 1, &C.scan2_raw, error)) || (st = S.get(nb + 1, &C.scan2_par, error)))
           return st;
+        if (fused) {
+          // every group has a record: the record total bounds the groups
+          const uint64_t records = tot[0] + tot[1];
+          if ((st = alloc_result(std::min<uint64_t>(records, P.cap)))) return st;
+          for (int k = 0; k < P.n_keys; ++k) fill(ctx, P.out_key_valid[k], ((uint64_t)r->groups + 63) / 64 + 1, ~0ull);
+          for (int a = 0; a < P.n_aggs; ++a) fill(ctx, P.aggs[a].out_valid, ((uint64_t)r->groups + 63) / 64 + 1, ~0ull);
+          P.fused = 1;
+        }
         memcpy(ctx->staging, &P, sizeof P);
         HIP_TRY(hipMemcpyAsync(dplan, ctx->staging, sizeof P, hipMemcpyHostToDevice, ctx->stream));
         if ((st = launch_second_level(ctx, C, dplan, error))) return st;
@@ -901,7 +983,7 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
       if (sub_boost >= 6 || P.n_sub >= 8192)
         return dfr_fail(error, DFR_CAPACITY, "group table capacity exceeded");
       ++sub_boost;
-      continue;
+      continue;  // (the guard releases this attempt's result)
     }
     if (hf[0]) {
       // a full hash table: retry once at load <= 1/2 of every possible group
@@ -930,74 +1012,29 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
       P.order = order2;
     }
     HIP_TRY(hipEventRecord(ctx->evp[2], ctx->stream));
-    // outputs: one pool block
-    dfr_result *r = new dfr_result();
-    r->ctx = ctx;
-    r->groups = (int64_t)G;
-    r->key_dtypes = C.key_dtypes;
-    r->agg_dtypes = C.agg_dtypes;
-    const uint64_t vwords = (G + 63) / 64 + 1;
-    size_t bytes = 0;
-    auto col_bytes = [&](int dtype) {
-      size_t b = (size_t)G * ((dtype == DFR_INT32 || dtype == DFR_DICT) ? 4 : 8);
-      return (b + 255) / 256 * 256 + vwords * 8;
-    };
-    for (int32_t d : C.key_dtypes) bytes += col_bytes(d);
-    for (int32_t d : C.agg_dtypes) bytes += col_bytes(d);
-    if (C.maintain_order) bytes += (size_t)G * 8 + 256;
-    if ((st = ctx->pool.alloc(bytes, &r->buffer, error))) {
-      delete r;
-      return st;
-    }
-    r->bytes = bytes;
-    char *cur = (char *)r->buffer;
-    auto take = [&](int dtype, void **vals, uint64_t **valid) {
-      size_t b = (size_t)G * ((dtype == DFR_INT32 || dtype == DFR_DICT) ? 4 : 8);
-      *vals = cur;
-      cur += (b + 255) / 256 * 256;
-      *valid = (uint64_t *)cur;
-      cur += vwords * 8;
-    };
-    for (size_t k = 0; k < C.key_dtypes.size(); ++k) {
-      void *v;
-      uint64_t *m;
-      take(C.key_dtypes[k], &v, &m);
-      r->key_values.push_back(v);
-      r->key_valid.push_back(m);
-      P.out_keys[k] = v;
-      P.out_key_valid[k] = m;
-    }
-    for (size_t a = 0; a < C.agg_dtypes.size(); ++a) {
-      void *v;
-      uint64_t *m;
-      take(C.agg_dtypes[a], &v, &m);
-      r->agg_values.push_back(v);
-      r->agg_valid.push_back(m);
-      P.aggs[a].out_values = v;
-      P.aggs[a].out_valid = m;
-    }
-    if (C.maintain_order) {
-      r->first_rows = (uint64_t *)cur;
-      P.out_first = r->first_rows;
+    if (!fused) {
+      if ((st = alloc_result(G))) return st;
+      memcpy(ctx->staging, &P, sizeof P);
+      if (G > 0) {
+        HIP_TRY(hipMemcpyAsync(dplan, ctx->staging, sizeof P, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_finalize, dim3((unsigned)((G + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           ctx->stream, dplan);
+        ctx->obs.launches++;
+      }
     } else {
-      P.out_first = nullptr;
-    }
-    memcpy(ctx->staging, &P, sizeof P);
-    if (G > 0) {
-      HIP_TRY(hipMemcpyAsync(dplan, ctx->staging, sizeof P, hipMemcpyHostToDevice, ctx->stream));
-      hipLaunchKernelGGL(k_finalize, dim3((unsigned)((G + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                         ctx->stream, dplan);
-      ctx->obs.launches++;
+      if (!r && (st = alloc_result(0))) return st;  // no rows: no pass C ran
+      r->groups = (int64_t)G;
+      if (G > 0) {
+        hipLaunchKernelGGL(k_clear_tail, dim3(1), dim3(64), 0, ctx->stream, dplan, G);
+        ctx->obs.launches++;
+      }
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     uint32_t *hf2 = (uint32_t *)(ctx->staging + sizeof(DevPlan) + 64);
     HIP_TRY(hipMemcpyAsync(hf2, fault, 16, hipMemcpyDeviceToHost, ctx->stream));
     hipError_t e = hipStreamSynchronize(ctx->stream);
     ctx->obs.waits++;
-    if (e != hipSuccess) {
-      dfr_result_release(r);
-      return dfr_fail(error, DFR_DEVICE, "group-by kernels: %s", hipGetErrorString(e));
-    }
+    if (e != hipSuccess) return dfr_fail(error, DFR_DEVICE, "group-by kernels: %s", hipGetErrorString(e));
     ctx->obs.completed = ctx->obs.launches;
     ctx->obs.queries++;
     float ms = 0;
@@ -1009,15 +1046,10 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
       hipEventElapsedTime(&t, marks[ph], marks[ph + 1]);
       r->phase_ms[ph] = t;
     }
-    if (hf2[1]) {
-      dfr_result_release(r);
-      return dfr_fail(error, DFR_DATA_FAULT, "Int64 expression sum overflow");
-    }
-    if (hf2[2]) {
-      dfr_result_release(r);
-      return dfr_fail(error, DFR_DATA_FAULT, "int32 expression sum overflow");
-    }
+    if (hf2[1]) return dfr_fail(error, DFR_DATA_FAULT, "Int64 expression sum overflow");
+    if (hf2[2]) return dfr_fail(error, DFR_DATA_FAULT, "int32 expression sum overflow");
     *out = r;
+    r = nullptr;
     return DFR_OK;
   }
 }

@@ -78,10 +78,21 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF) {
       << "].fshift2;\n";
   for (int k = 0; k < P.n_keys; ++k)
     o << "  L.keys[" << k << "].minv = R.keys[" << k << "].minv; L.keys[" << k << "].radix = R.keys[" << k
-      << "].radix; L.keys[" << k << "].stride = R.keys[" << k << "].stride;\n";
+      << "].radix; L.keys[" << k << "].stride = R.keys[" << k << "].stride; L.keys[" << k
+      << "].inv_radix = R.keys[" << k << "].inv_radix;\n";
   for (int l = 0; l < P.n_leaves; ++l)
     o << "  L.leaves[" << l << "].lit = R.leaves[" << l << "].lit; L.leaves[" << l << "].mod = R.leaves[" << l
       << "].mod;\n";
+  // outputs (a fused pass C writes them)
+  for (int k = 0; k < P.n_keys; ++k)
+    o << "  L.out_keys[" << k << "] = R.out_keys[" << k << "]; L.out_key_valid[" << k << "] = R.out_key_valid[" << k
+      << "];\n";
+  for (int a = 0; a < P.n_aggs; ++a)
+    o << "  L.aggs[" << a << "].ddof = R.aggs[" << a << "].ddof; L.aggs[" << a << "].min_count = R.aggs[" << a
+      << "].min_count; L.aggs[" << a << "].out_values = R.aggs[" << a << "].out_values; L.aggs[" << a
+      << "].out_valid = R.aggs[" << a << "].out_valid;\n"
+      << "  L.aggs[" << a << "].op = " << P.aggs[a].op << "; L.aggs[" << a << "].col = " << P.aggs[a].col
+      << "; L.aggs[" << a << "].out_dtype = " << P.aggs[a].out_dtype << ";\n";
   // the query's shape: literals
   o << "  L.n_cols = " << P.n_cols << "; L.n_keys = " << P.n_keys << "; L.n_leaves = " << P.n_leaves
     << "; L.words = " << P.words << "; L.lds_words = " << P.lds_words << "; L.lds_mode = " << P.lds_mode
@@ -89,7 +100,7 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF) {
     << ";\n";
   o << "  L.rw = " << P.rw << "; L.n_vals = " << P.n_vals << "; L.rec_row = " << P.rec_row
     << "; L.rec_meta = " << P.rec_meta << "; L.rec_val0 = " << P.rec_val0 << "; L.vmask_in_key = " << P.vmask_in_key
-    << "; L.lds_stride = " << P.lds_stride << ";\n";
+    << "; L.lds_stride = " << P.lds_stride << "; L.n_aggs = " << P.n_aggs << "; L.fused = " << P.fused << ";\n";
   for (int i = 0; i < 8; ++i) o << "  L.truth[" << i << "] = " << P.truth[i] << "u;\n";
   for (int c = 0; c < P.n_cols; ++c) {
     const DevColAgg &A = P.cagg[c];
@@ -127,7 +138,7 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF) {
   o << "extern \"C\" __global__ void __launch_bounds__(1024) dfr_part_jit(const DevPlan *__restrict__ plan) {\n"
     << "  const DevPlan L = make_plan(plan);\n  part_aggregate_body(L);\n}\n";
   o << "extern \"C\" __global__ void __launch_bounds__(512) dfr_sub_jit(const DevPlan *__restrict__ plan) {\n"
-    << "  const DevPlan L = make_plan(plan);\n  sub_aggregate_body(L);\n}\n";
+    << "  const DevPlan L = make_plan(plan);\n  sub_aggregate_body<" << P.n_aggs << ">(L);\n}\n";
   return o.str();
 }
 

@@ -60,6 +60,18 @@ __device__ __forceinline__ T cget(const T &x) {
   return x;
 }
 
+// A register copy of the plan (the runtime-compiled kernels) is indexed
+// only at compile-time positions, so it stays in registers: loops over its
+// arrays are unrolled to their maximum length there.
+template <class PV>
+struct RegPlan {
+  static constexpr bool value = false;
+};
+template <>
+struct RegPlan<DevPlan> {
+  static constexpr bool value = true;
+};
+
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
@@ -209,6 +221,168 @@ __device__ __forceinline__ void add128(uint64_t *p, uint64_t lo, uint64_t hi) {
   uint64_t carry = ((uint64_t)old + lo) < (uint64_t)old ? 1 : 0;
   uint64_t h = hi + carry;
   if (h) atomicAdd((unsigned long long *)(p + 1), (unsigned long long)h);
+}
+
+// ------------------------------------------------------------ finalize
+
+struct AggOut {
+  uint64_t bits;  // int64 value, or double bits
+  bool valid;
+};
+
+__device__ __forceinline__ double finish_fsum(const DevColAgg &A, const uint64_t *b, uint64_t flags) {
+  if ((flags & FL_NAN) || ((flags & FL_PINF) && (flags & FL_NINF))) return __builtin_nan("");
+  if (flags & FL_PINF) return __builtin_inf();
+  if (flags & FL_NINF) return -__builtin_inf();
+  return i128_to_f64(b[A.o_fsum], b[A.o_fsum + 1], A.fshift);
+}
+
+__device__ __forceinline__ double finish_var(const DevColAgg &A, const uint64_t *b, uint64_t n,
+                                             int ddof, uint64_t flags) {
+  if (flags & (FL_NAN | FL_PINF | FL_NINF)) return __builtin_nan("");
+  // n * sumsq * 2^(2S - S2) - sum^2, exactly, in units of 2^-2S
+  uint64_t qlo = b[A.o_sq], qhi = b[A.o_sq + 1];
+  uint64_t tlo = b[A.o_fsum], thi = b[A.o_fsum + 1];
+  if ((int64_t)thi < 0) {
+    tlo = ~tlo + 1;
+    thi = ~thi + (tlo == 0 ? 1 : 0);
+  }
+  U256 nq = mul128(qlo, qhi, n, 0);
+  U256 a = shl256(nq, 2 * A.fshift - A.fshift2);
+  U256 t2 = mul128(tlo, thi, tlo, thi);
+  U256 r;
+  sub256(a, t2, r);
+  double num = u256_to_f64(r, 2 * A.fshift);
+  double den = (double)n * (double)(n - (uint64_t)ddof);
+  return num / den;
+}
+
+// Column `col`'s aggregate block and whether it is Float64. A register
+// copy of the plan is indexed by a static loop only (an index read from the
+// plan itself would keep the whole copy in scratch memory).
+template <class PV>
+__device__ __forceinline__ void col_agg_of(const PV &P, int col, DevColAgg &A, bool &is_float) {
+  if constexpr (RegPlan<PV>::value) {
+    // field by field: a select of whole records becomes a copy through a
+    // selected address, which also keeps the plan copy in memory
+    A = DevColAgg{};
+    is_float = false;
+#pragma unroll
+    for (int c = 0; c < kMaxCols; ++c) {
+      const bool hit = c == col;
+      A.mask = hit ? P.cagg[c].mask : A.mask;
+      A.gbase = hit ? P.cagg[c].gbase : A.gbase;
+      A.lbase = hit ? P.cagg[c].lbase : A.lbase;
+      A.o_count = hit ? P.cagg[c].o_count : A.o_count;
+      A.o_isum = hit ? P.cagg[c].o_isum : A.o_isum;
+      A.o_fsum = hit ? P.cagg[c].o_fsum : A.o_fsum;
+      A.o_sq = hit ? P.cagg[c].o_sq : A.o_sq;
+      A.o_min = hit ? P.cagg[c].o_min : A.o_min;
+      A.o_max = hit ? P.cagg[c].o_max : A.o_max;
+      A.o_flags = hit ? P.cagg[c].o_flags : A.o_flags;
+      A.o_zfirst = hit ? P.cagg[c].o_zfirst : A.o_zfirst;
+      A.fshift = hit ? P.cagg[c].fshift : A.fshift;
+      A.fshift2 = hit ? P.cagg[c].fshift2 : A.fshift2;
+      A.int_as_float = hit ? P.cagg[c].int_as_float : A.int_as_float;
+      is_float = hit ? (P.cols[c].is_float != 0) : is_float;
+    }
+  } else {
+    A = cget(P.cagg[col]);
+    is_float = P.cols[col].is_float;
+  }
+}
+
+template <class PV>
+__device__ __forceinline__ AggOut finish_agg(const PV &P, const DevAgg &G, const uint64_t *slot) {
+  AggOut o = {0, true};
+  uint64_t rows = slot[0];
+  if (G.op == 0) {  // LEN
+    o.bits = rows;
+    return o;
+  }
+  DevColAgg A;
+  bool is_float;
+  col_agg_of(P, G.col, A, is_float);
+  const uint64_t *b = slot + A.gbase;
+  uint64_t n = (A.mask & M_VCOUNT) ? b[A.o_count] : rows;
+  uint64_t flags = (A.mask & M_FLAGS) ? b[A.o_flags] : 0;
+  switch (G.op) {
+    case 1:  // COUNT
+      o.bits = n;
+      return o;
+    case 2: {  // SUM
+      o.valid = (int64_t)n >= G.min_count;
+      if (is_float) {
+        o.bits = o.valid ? f64_bits(finish_fsum(A, b, flags)) : 0;
+      } else if (A.mask & M_ISUM64) {
+        int64_t t = (int64_t)b[A.o_isum];
+        if (!o.valid) t = 0;
+        if (G.out_dtype == T_INT32 && (t > 2147483647ll || t < -2147483648ll)) atomicOr(&P.fault[2], 1u);
+        o.bits = (uint64_t)t;
+      } else {
+        uint64_t lo = b[A.o_isum], hi = b[A.o_isum + 1];
+        bool fits = (hi == 0 && (int64_t)lo >= 0) || (hi == ~0ull && (int64_t)lo < 0);
+        if (!o.valid) lo = 0;
+        else if (!fits) atomicOr(&P.fault[G.out_dtype == T_INT32 ? 2 : 1], 1u);
+        int64_t t = (int64_t)lo;
+        if (o.valid && G.out_dtype == T_INT32 && (t > 2147483647ll || t < -2147483648ll))
+          atomicOr(&P.fault[2], 1u);
+        o.bits = (uint64_t)t;
+      }
+      return o;
+    }
+    case 3:
+    case 4: {  // MIN / MAX
+      const bool is_min = G.op == 3;
+      if (is_float) {
+        uint64_t k = b[is_min ? A.o_min : A.o_max];
+        bool seen = is_min ? (k != kEmpty) : (k != 0);
+        if (!seen) {
+          o.valid = (flags & FL_NAN) != 0;  // only NaN values: the result is NaN
+          o.bits = o.valid ? f64_bits(__builtin_nan("")) : 0;
+          return o;
+        }
+        uint64_t v = ord_key_inverse(k);
+        if ((v << 1) == 0 && (A.mask & M_ZFIRST)) {
+          uint64_t z = b[A.o_zfirst];
+          if (z != kEmpty) v = (z & 1) << 63;  // the first zero's sign
+        }
+        o.bits = v;
+      } else {
+        o.valid = n > 0;
+        o.bits = o.valid ? b[is_min ? A.o_min : A.o_max] : 0;
+      }
+      return o;
+    }
+    case 5: {  // MEAN
+      o.valid = n > 0;
+      if (!o.valid) return o;
+      double total;
+      if (A.mask & M_ISUM64)
+        total = (double)(int64_t)b[A.o_isum];
+      else if (A.mask & M_ISUM128)
+        total = i128_to_f64(b[A.o_isum], b[A.o_isum + 1], 0);
+      else
+        total = finish_fsum(A, b, flags);
+      o.bits = f64_bits(total / (double)n);
+      return o;
+    }
+    default: {  // VAR / STD
+      o.valid = (int64_t)n - (int64_t)G.ddof > 0;
+      if (!o.valid) return o;
+      double v = finish_var(A, b, n, G.ddof, flags);
+      if (G.op == 7) v = sqrt(v);
+      o.bits = f64_bits(v);
+      return o;
+    }
+  }
+}
+
+__device__ __forceinline__ void store_value(void *dst, int dtype, uint64_t i, uint64_t bits) {
+  if (dtype == T_INT32 || dtype == T_DICT)
+    ((uint32_t *)dst)[i] = (uint32_t)bits;
+  else
+    ((uint64_t *)dst)[i] = bits;
 }
 
 #if !DFR_JIT
@@ -559,131 +733,6 @@ __global__ void k_rank_first(const DevPlan *__restrict__ plan, const uint64_t *o
   order_out[rank_of(rowbits, rowprefix, r)] = s;
 }
 
-// ------------------------------------------------------------ finalize
-
-struct AggOut {
-  uint64_t bits;  // int64 value, or double bits
-  bool valid;
-};
-
-__device__ __forceinline__ double finish_fsum(const DevColAgg &A, const uint64_t *b, uint64_t flags) {
-  if ((flags & FL_NAN) || ((flags & FL_PINF) && (flags & FL_NINF))) return __builtin_nan("");
-  if (flags & FL_PINF) return __builtin_inf();
-  if (flags & FL_NINF) return -__builtin_inf();
-  return i128_to_f64(b[A.o_fsum], b[A.o_fsum + 1], A.fshift);
-}
-
-__device__ __forceinline__ double finish_var(const DevColAgg &A, const uint64_t *b, uint64_t n,
-                                             int ddof, uint64_t flags) {
-  if (flags & (FL_NAN | FL_PINF | FL_NINF)) return __builtin_nan("");
-  // n * sumsq * 2^(2S - S2) - sum^2, exactly, in units of 2^-2S
-  uint64_t qlo = b[A.o_sq], qhi = b[A.o_sq + 1];
-  uint64_t tlo = b[A.o_fsum], thi = b[A.o_fsum + 1];
-  if ((int64_t)thi < 0) {
-    tlo = ~tlo + 1;
-    thi = ~thi + (tlo == 0 ? 1 : 0);
-  }
-  U256 nq = mul128(qlo, qhi, n, 0);
-  U256 a = shl256(nq, 2 * A.fshift - A.fshift2);
-  U256 t2 = mul128(tlo, thi, tlo, thi);
-  U256 r;
-  sub256(a, t2, r);
-  double num = u256_to_f64(r, 2 * A.fshift);
-  double den = (double)n * (double)(n - (uint64_t)ddof);
-  return num / den;
-}
-
-__device__ AggOut finish_agg(PlanRef P, const DevAgg &G, const uint64_t *slot) {
-  AggOut o = {0, true};
-  uint64_t rows = slot[0];
-  if (G.op == 0) {  // LEN
-    o.bits = rows;
-    return o;
-  }
-  const DevColAgg A = cget(P.cagg[G.col]);
-  const uint64_t *b = slot + A.gbase;
-  uint64_t n = (A.mask & M_VCOUNT) ? b[A.o_count] : rows;
-  uint64_t flags = (A.mask & M_FLAGS) ? b[A.o_flags] : 0;
-  const bool is_float = P.cols[G.col].is_float;
-  switch (G.op) {
-    case 1:  // COUNT
-      o.bits = n;
-      return o;
-    case 2: {  // SUM
-      o.valid = (int64_t)n >= G.min_count;
-      if (is_float) {
-        o.bits = o.valid ? f64_bits(finish_fsum(A, b, flags)) : 0;
-      } else if (A.mask & M_ISUM64) {
-        int64_t t = (int64_t)b[A.o_isum];
-        if (!o.valid) t = 0;
-        if (G.out_dtype == T_INT32 && (t > 2147483647ll || t < -2147483648ll)) atomicOr(&P.fault[2], 1u);
-        o.bits = (uint64_t)t;
-      } else {
-        uint64_t lo = b[A.o_isum], hi = b[A.o_isum + 1];
-        bool fits = (hi == 0 && (int64_t)lo >= 0) || (hi == ~0ull && (int64_t)lo < 0);
-        if (!o.valid) lo = 0;
-        else if (!fits) atomicOr(&P.fault[G.out_dtype == T_INT32 ? 2 : 1], 1u);
-        int64_t t = (int64_t)lo;
-        if (o.valid && G.out_dtype == T_INT32 && (t > 2147483647ll || t < -2147483648ll))
-          atomicOr(&P.fault[2], 1u);
-        o.bits = (uint64_t)t;
-      }
-      return o;
-    }
-    case 3:
-    case 4: {  // MIN / MAX
-      const bool is_min = G.op == 3;
-      if (is_float) {
-        uint64_t k = b[is_min ? A.o_min : A.o_max];
-        bool seen = is_min ? (k != kEmpty) : (k != 0);
-        if (!seen) {
-          o.valid = (flags & FL_NAN) != 0;  // only NaN values: the result is NaN
-          o.bits = o.valid ? f64_bits(__builtin_nan("")) : 0;
-          return o;
-        }
-        uint64_t v = ord_key_inverse(k);
-        if ((v << 1) == 0 && (A.mask & M_ZFIRST)) {
-          uint64_t z = b[A.o_zfirst];
-          if (z != kEmpty) v = (z & 1) << 63;  // the first zero's sign
-        }
-        o.bits = v;
-      } else {
-        o.valid = n > 0;
-        o.bits = o.valid ? b[is_min ? A.o_min : A.o_max] : 0;
-      }
-      return o;
-    }
-    case 5: {  // MEAN
-      o.valid = n > 0;
-      if (!o.valid) return o;
-      double total;
-      if (A.mask & M_ISUM64)
-        total = (double)(int64_t)b[A.o_isum];
-      else if (A.mask & M_ISUM128)
-        total = i128_to_f64(b[A.o_isum], b[A.o_isum + 1], 0);
-      else
-        total = finish_fsum(A, b, flags);
-      o.bits = f64_bits(total / (double)n);
-      return o;
-    }
-    default: {  // VAR / STD
-      o.valid = (int64_t)n - (int64_t)G.ddof > 0;
-      if (!o.valid) return o;
-      double v = finish_var(A, b, n, G.ddof, flags);
-      if (G.op == 7) v = sqrt(v);
-      o.bits = f64_bits(v);
-      return o;
-    }
-  }
-}
-
-__device__ __forceinline__ void store_value(void *dst, int dtype, uint64_t i, uint64_t bits) {
-  if (dtype == T_INT32 || dtype == T_DICT)
-    ((uint32_t *)dst)[i] = (uint32_t)bits;
-  else
-    ((uint64_t *)dst)[i] = bits;
-}
-
 // Output group i (of G) from table slot s. The 64 lanes of a wave hold
 // groups 64k .. 64k + 63 of one result (inactive lanes past G), so lane 0
 // writes each validity word whole.
@@ -694,6 +743,7 @@ __device__ __forceinline__ void finalize_group(PlanRef P, uint64_t i, uint64_t s
   uint64_t packed = 0;
   if (active) packed = (P.mode == MODE_DIRECT) ? s : P.keys_tab[s];
   if (active && P.out_first) P.out_first[i] = slot[1];
+  uint64_t rest = packed;  // digits from the first key up
   for (int k = 0; k < P.n_keys; ++k) {
     const DevKey K = cget(P.keys[k]);
     bool valid;
@@ -703,7 +753,8 @@ __device__ __forceinline__ void finalize_group(PlanRef P, uint64_t i, uint64_t s
       valid = packed != kNullFloatKey;
       v = (active && valid) ? ((const uint64_t *)P.cols[k].values)[slot[1]] : 0;
     } else {
-      uint64_t digit = (packed / K.stride) % K.radix;
+      uint64_t digit;
+      divmod_u64(rest, K.radix, K.inv_radix, rest, digit);
       valid = !(K.nullable && digit == K.radix - 1);
       v = valid ? (uint64_t)K.minv + digit : 0;
     }

@@ -28,18 +28,6 @@ constexpr int kMaxPartEntries = 16384;  // entries of a pass C LDS table, at mos
 
 __device__ __forceinline__ uint32_t key32(uint64_t key) { return (uint32_t)key ^ (uint32_t)(key >> 32); }
 
-// A register copy of the plan (the runtime-compiled pass A) is indexed only
-// at compile-time positions, so it stays in registers: loops over its arrays
-// are unrolled to their maximum length there.
-template <class PV>
-struct RegPlan {
-  static constexpr bool value = false;
-};
-template <>
-struct RegPlan<DevPlan> {
-  static constexpr bool value = true;
-};
-
 template <class PV>
 __device__ __forceinline__ uint32_t part_of(const PV &P, uint64_t key) {
   return P.part_log2 ? ((key32(key) * 0x9E3779B1u) >> (32 - P.part_log2)) : 0u;
@@ -969,11 +957,54 @@ __global__ void __launch_bounds__(1024) k_sub_scatter(const DevPlan *__restrict_
 }
 #endif  // !DFR_JIT
 
+// Output row i of a group: its key columns from the packed key, its
+// aggregates from its state words `slot` (in LDS). Validity words start all
+// ones: a null key or aggregate clears its bit (workgroups' output runs
+// share words, so with an atomic; such values are rare).
+//
+// NA > 0 (a register copy of the plan) is the exact number of aggregates,
+// so their loop unrolls completely.
+template <int NA, class PV>
+__device__ __forceinline__ void emit_group(const PV &P, uint64_t i, const uint64_t *slot, uint64_t packed) {
+  const uint64_t clear = ~(1ull << (i & 63));
+  uint64_t rest = packed;  // digits from the first key up
+#pragma unroll
+  for (int k = 0; k < kMaxKeys; ++k) {
+    if (k >= P.n_keys) continue;
+    const DevKey K = cget(P.keys[k]);
+    bool valid;
+    uint64_t v;
+    if (K.is_float) {
+      // the group's first row's own value (FIRST is tracked for such keys)
+      valid = packed != kNullFloatKey;
+      v = valid ? gp((const uint64_t *)P.cols[k].values)[slot[1]] : 0;
+    } else {
+      uint64_t digit;
+      divmod_u64(rest, K.radix, K.inv_radix, rest, digit);
+      valid = !(K.nullable && digit == K.radix - 1);
+      v = valid ? (uint64_t)K.minv + digit : 0;
+    }
+    store_value(P.out_keys[k], P.cols[k].dtype, i, v);
+    if (!valid) atomicAnd((unsigned long long *)&P.out_key_valid[k][i >> 6], (unsigned long long)clear);
+  }
+  auto one = [&](const DevAgg &G) {
+    const AggOut o = finish_agg(P, G, slot);
+    store_value(G.out_values, G.out_dtype, i, o.bits);
+    if (!o.valid) atomicAnd((unsigned long long *)&G.out_valid[i >> 6], (unsigned long long)clear);
+  };
+  if constexpr (RegPlan<PV>::value) {
+#pragma unroll
+    for (int a = 0; a < NA; ++a) one(P.aggs[a]);
+  } else {
+    for (int a = 0; a < P.n_aggs; ++a) one(cget(P.aggs[a]));
+  }
+}
+
 // Pass C over sub-partition x = p * n_sub + sub: folds its raw
 // records and merges its partial states in an LDS table, then gives each
 // group the next free slot of the dense HBM tables (one device atomic per
 // workgroup) and stores its key and state there.
-template <class PV>
+template <int NA = 0, class PV>
 __device__ __forceinline__ void sub_aggregate_body(const PV &P) {
   extern __shared__ uint64_t smem[];
   __shared__ uint32_t wsum[kMaxPartEntries / 64];
@@ -1038,6 +1069,10 @@ __device__ __forceinline__ void sub_aggregate_body(const PV &P) {
     const uint64_t ball = __ballot(key != kEmpty);
     if (key == kEmpty) continue;
     const uint64_t slot = gbase + wsum[c] + (uint64_t)__popcll(ball & lt);
+    if (P.fused) {
+      emit_group<NA>(P, slot, lst + (int64_t)e * S, key);
+      continue;
+    }
     P.keys_tab[slot] = key;
     uint64_t *d = P.state + slot * (uint64_t)W;
     const uint64_t *s = lst + (int64_t)e * S;
@@ -1048,6 +1083,16 @@ __device__ __forceinline__ void sub_aggregate_body(const PV &P) {
 #if !DFR_JIT
 __global__ void __launch_bounds__(512) k_sub_aggregate(const DevPlan *__restrict__ plan) {
   sub_aggregate_body(plan_ref(plan));
+}
+
+// After a fused pass C: the validity bits of rows G.. (set by the all-ones
+// fill) cleared in each output column's last word.
+__global__ void k_clear_tail(const DevPlan *__restrict__ plan, uint64_t G) {
+  PlanRef P = plan_ref(plan);
+  if (threadIdx.x != 0 || (G & 63) == 0) return;
+  const uint64_t keep = (1ull << (G & 63)) - 1;
+  for (int k = 0; k < P.n_keys; ++k) P.out_key_valid[k][G >> 6] &= keep;
+  for (int a = 0; a < P.n_aggs; ++a) P.aggs[a].out_valid[G >> 6] &= keep;
 }
 #endif
 

@@ -92,7 +92,8 @@ struct DevCol {
 struct DevKey {
   int64_t minv;
   uint64_t radix;
-  uint64_t stride;
+  uint64_t stride;   // product of the radixes of the keys before this one
+  double inv_radix;  // 1 / radix, for decoding digits without integer division
   int32_t nullable;
   int32_t is_float;  // a Float64 key (the only key): packed as its canonical word
 };
@@ -211,6 +212,9 @@ struct DevPlan {
   // pass A's hashed LDS table: 4-entry buckets, any count (a key's bucket
   // is the high half of its 32-bit hash times the count)
   int32_t lds_buckets;
+  // second level, unordered output: pass C writes each group's output row
+  // (keys and aggregates) at its dense slot, with no state or key tables
+  int32_t fused;
   uint32_t *hist2_raw;
   uint32_t *hist2_par;
   uint64_t *base2_raw;

@@ -40,7 +40,7 @@ struct Tuning {
   int scan_unroll = 1;         // row pairs a scan thread loads at once (1, 2, 4)
   int max_parts = 4096;        // first-level key partitions, at most (a power of two)
   int sub_lds_bytes = 49152;   // LDS bytes of a second-level pass C table
-  int sub_block = 256;         // threads of a second-level pass C workgroup
+  int sub_block = 512;         // threads of a second-level pass C workgroup
   int two_level = -1;          // second partition level: -1 when groups may outgrow one level, 0 never, 1 always
   int batch_tile = 131072;     // rows of a batch work item
   int batch_graph = 0;         // 1: a batch's launch sequence runs as one captured HIP graph
@@ -49,6 +49,7 @@ struct Tuning {
   int scan_prefetch = 1;       // runtime-compiled pass A: 1 overlaps the next tile's loads with this tile's fold
   int lds_odd_stride = 0;      // partitioned path: LDS table entries at an odd number of words
   int part_block = 1024;       // threads of a first-level pass C workgroup
+  int two_level_parts = 2048;  // first-level partitions of a two-level query (a power of two)
 };
 
 }  // namespace dfr

@@ -131,7 +131,9 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * next tile's loads before it folds the current tile; 0: load, then fold),
  * "lds_odd_stride" (1: the partitioned path's LDS table entries are an odd
  * number of 8-byte words apart; 0: exactly the state's words),
- * "part_block" (threads of a first-level partition table's workgroup). */
+ * "part_block" (threads of a first-level partition table's workgroup),
+ * "two_level_parts" (first-level partitions of a two-level query, a power
+ * of two up to "max_parts"). */
 int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
 
 /* ---- device columns (the column hand-off) ------------------------------ */

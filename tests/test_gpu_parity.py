@@ -176,9 +176,9 @@ def test_random_queries(seed, fi, ki):
 
 TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26, "hash_slots": 0, "block": 512,
                  "wg_per_cu": 2, "lds_probe": 4, "scan_unroll": 1, "agg_mode": 1, "two_level": -1,
-                 "max_parts": 4096, "part_lds_bytes": 131072, "sub_lds_bytes": 49152, "sub_block": 256, "jit": 1,
+                 "max_parts": 4096, "part_lds_bytes": 131072, "sub_lds_bytes": 49152, "sub_block": 512, "jit": 1,
                  "scan_prefetch": 1, "lds_odd_stride": 0, "part_block": 1024, "scan_block": 1024,
-                 "scan_wg_per_cu": 1, "scan_lds_bytes": 159744}
+                 "scan_wg_per_cu": 1, "scan_lds_bytes": 159744, "two_level_parts": 2048}
 
 
 @pytest.mark.parametrize("setting", [
@@ -190,6 +190,7 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
     # one first-level partition, many small second-level tables
     {"two_level": 1, "max_parts": 1, "sub_lds_bytes": 4096, "sub_block": 64},
     {"two_level": 1, "sub_lds_bytes": 98304, "sub_block": 512},
+    {"two_level": 1, "two_level_parts": 64}, {"two_level": 1, "two_level_parts": 2, "jit": 2},
     # pass A compiled at run time for the query's shape
     {"jit": 2}, {"jit": 2, "lds_mode": 0}, {"jit": 2, "two_level": 1}, {"jit": 2, "scan_unroll": 4},
     # its loads pipelined (the default) or not, over ranges of odd and even tile counts
