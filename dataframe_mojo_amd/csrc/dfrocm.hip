@@ -303,6 +303,10 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.batch_tile = (int)value;
   else if (k == "batch_graph" && (value == 0 || value == 1))
     t.batch_graph = (int)value;
+  else if (k == "scatter_slices" && value >= 1 && value <= 16 && (value & (value - 1)) == 0)
+    t.scatter_slices = (int)value;
+  else if (k == "scatter_chunk" && (value == 8192 || value == 16384))
+    t.scatter_chunk = (int)value;
   else if (k == "two_level_parts" && value >= 1 && value <= 4096 && (value & (value - 1)) == 0)
     t.two_level_parts = (int)value;
   else if (k == "part_block" && value >= 64 && value <= 1024 && value % 64 == 0)

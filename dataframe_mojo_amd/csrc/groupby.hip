@@ -65,6 +65,8 @@ Tuning default_tuning(dfr_context *ctx) {
   t.lds_odd_stride = env_int("DFR_LDS_ODD_STRIDE", 0) ? 1 : 0;
   t.part_block = env_int("DFR_PART_BLOCK", 1024);
   t.two_level_parts = env_int("DFR_TWO_LEVEL_PARTS", 2048);
+  t.scatter_slices = env_int("DFR_SCATTER_SLICES", 1);
+  t.scatter_chunk = env_int("DFR_SCATTER_CHUNK", 16384) == 8192 ? 8192 : 16384;
   t.scan_block = env_int("DFR_SCAN_BLOCK", 1024);
   t.scan_wg_per_cu = env_int("DFR_SCAN_WG_PER_CU", 1);
   t.scan_lds_bytes = std::min(env_int("DFR_SCAN_LDS_BYTES", 159744), ctx->lds_per_block);
@@ -503,8 +505,12 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
       P.mode = MODE_HASH;
       P.cap = std::max<uint64_t>(1, std::min<uint64_t>(D, (uint64_t)std::max<int64_t>(rows, 1)));
     }
-    // the scan's LDS table (full state words) next to its histogram
-    const int hist_bytes = (P.n_parts + 2) * 4;
+    // the scan's LDS table (full state words) next to its histograms
+    const int waves = std::max(64, std::min(1024, T.scan_block)) / 64;
+    int H = 1;  // a power of two that divides the waves
+    while (H * 2 <= std::min(T.scatter_slices, waves) && waves % (H * 2) == 0) H *= 2;
+    P.seg_slices = H;
+    const int hist_bytes = (H * P.n_parts + 2) * 4;
     const int avail = std::max(0, std::min(T.scan_lds_bytes, ctx->lds_per_block) - hist_bytes);
     const uint64_t direct_e = (uint64_t)avail / (8ull * S);
     int m = T.lds_mode;
@@ -556,7 +562,7 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
     const int S = P.lds_stride;
     C.lds_bytes = (P.lds_mode == LDS_NONE ? 0
                                           : (size_t)P.lds_entries * 8 * (S + (P.lds_mode == LDS_HASHED ? 1 : 0))) +
-                  (size_t)(P.n_parts + 2) * 4;
+                  (size_t)(P.seg_slices * P.n_parts + 2) * 4;
     C.c_lds_bytes = (size_t)P.c_entries * 8 * (S + 1);
   }
 }
@@ -694,9 +700,16 @@ int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan
   hipLaunchKernelGGL(k_part_base, dim3(1), dim3(kBlock), 0, ctx->stream, dplan);
   // k_part_scatter: 64-bit write cursors per partition and kind, then its
   // chunk sort's counters and record indices
-  const size_t cur_bytes = part_scatter_lds(P.n_parts);
-  if ((st = set_lds(k_part_scatter, cur_bytes, error))) return st;
-  hipLaunchKernelGGL(k_part_scatter, dim3(P.n_segs), dim3(kScatterThreads), cur_bytes, ctx->stream, dplan);
+  const dim3 sgrid((unsigned)(P.n_segs * P.seg_slices));
+  if (ctx->tune.scatter_chunk == 8192) {
+    const size_t cur_bytes = part_scatter_lds(P.n_parts, 8192);
+    if ((st = set_lds(k_part_scatter<8192>, cur_bytes, error))) return st;
+    hipLaunchKernelGGL(k_part_scatter<8192>, sgrid, dim3(kScatterThreads), cur_bytes, ctx->stream, dplan);
+  } else {
+    const size_t cur_bytes = part_scatter_lds(P.n_parts, kScatterChunk);
+    if ((st = set_lds(k_part_scatter<kScatterChunk>, cur_bytes, error))) return st;
+    hipLaunchKernelGGL(k_part_scatter<kScatterChunk>, sgrid, dim3(kScatterThreads), cur_bytes, ctx->stream, dplan);
+  }
   ctx->obs.launches += 3;
   if (!P.two_level) {
     void *args[] = {(void *)&dplan};
@@ -893,7 +906,8 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
       const uint64_t par_words = S_ * (uint64_t)P.seg_partials * (1 + W);
       if ((st = S.get(raw_words, &P.spill, error)) || (st = S.get(raw_words, &P.part_raw, error)) ||
           (st = S.get(par_words, &P.partials, error)) || (st = S.get(par_words, &P.part_par, error)) ||
-          (st = S.get(S_ * NP, &P.hist_raw, error)) || (st = S.get(S_ * NP, &P.hist_par, error)) ||
+          (st = S.get(S_ * (uint64_t)P.seg_slices * NP, &P.hist_raw, error)) ||
+          (st = S.get(S_ * NP, &P.hist_par, error)) ||
           (st = S.get(S_ * (uint64_t)P.seg_waves, &P.cnt_raw, error)) || (st = S.get(S_, &P.cnt_par, error)) ||
           (st = S.get(NP + 1, &P.base_raw, error)) || (st = S.get(NP + 1, &P.base_par, error)))
         return st;

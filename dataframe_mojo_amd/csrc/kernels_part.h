@@ -413,11 +413,13 @@ __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + (LDSM == LDS_HASHED ? E : 0);
   const int S = P.lds_stride;
-  // one per-partition histogram: raw records during the scan, then (once
-  // written out) the flushed partial states
+  // per-partition histograms: raw records during the scan, one per slice
+  // of the workgroup's waves; then (once written out) the first one counts
+  // the flushed partial states
+  const int H = P.seg_slices;
   uint32_t *hraw = (uint32_t *)(lst + (LDSM == LDS_NONE ? 0 : (size_t)E * S));
   uint32_t *hpar = hraw;
-  uint32_t *cnt = hraw + NP;  // [0] raw records, [1] partial records
+  uint32_t *cnt = hraw + H * NP;  // [0] raw records, [1] partial records
   if (LDSM != LDS_NONE) {
     for (int i = threadIdx.x; i < E; i += blockDim.x) {
       if (LDSM == LDS_HASHED) lkeys[i] = kEmpty;
@@ -430,12 +432,13 @@ __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
       }
     }
   }
-  for (int i = threadIdx.x; i < NP + 2; i += blockDim.x) hraw[i] = 0;
+  for (int i = threadIdx.x; i < H * NP + 2; i += blockDim.x) hraw[i] = 0;
   __syncthreads();
   const int s = blockIdx.x;
   // wave segment: the rows a wave sees, seg_rows records at most
   const int wave = threadIdx.x >> 6;
   const uint64_t wseg = (uint64_t)s * (blockDim.x >> 6) + wave;
+  uint32_t *hw = hraw + (wave / ((blockDim.x >> 6) / H)) * NP;  // this wave's slice
   uint64_t *seg = P.spill + wseg * (uint64_t)P.seg_rows * P.rw;
   uint32_t wcur = 0;
   const int64_t b0 = (int64_t)s * chunk;
@@ -453,30 +456,30 @@ __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
     // (a tile past the range loads the range's last pair: in bounds)
     for (int64_t base = b0; base < b1e;) {
       load_tile<NC, U>(P, tb, base + step, b1e);
-      fold_tile<LDSM, NC, U, false, TL>(P, ta, lkeys, lst, seg, wcur, hraw, base);
+      fold_tile<LDSM, NC, U, false, TL>(P, ta, lkeys, lst, seg, wcur, hw, base);
       base += step;
       if (base >= b1e) break;
       load_tile<NC, U>(P, ta, base + step, b1e);
-      fold_tile<LDSM, NC, U, false, TL>(P, tb, lkeys, lst, seg, wcur, hraw, base);
+      fold_tile<LDSM, NC, U, false, TL>(P, tb, lkeys, lst, seg, wcur, hw, base);
       base += step;
     }
   } else {
     for (int64_t base = b0; base < b1e; base += step) {
       ScanTile<NC, U> t;
       load_tile<NC, U>(P, t, base, b1e);
-      fold_tile<LDSM, NC, U, true>(P, t, lkeys, lst, seg, wcur, hraw, base);
+      fold_tile<LDSM, NC, U, true>(P, t, lkeys, lst, seg, wcur, hw, base);
     }
   }
   if (b1e < b1) {
     ScanTile<NC, 1> t;
     load_last_row<NC>(P, t, b1e);
-    fold_tile<LDSM, NC, 1, true>(P, t, lkeys, lst, seg, wcur, hraw, b1e);
+    fold_tile<LDSM, NC, 1, true>(P, t, lkeys, lst, seg, wcur, hw, b1e);
   }
   if ((threadIdx.x & 63) == 0) P.cnt_raw[wseg] = wcur;
   __syncthreads();
-  for (int p = threadIdx.x; p < NP; p += blockDim.x) {
-    P.hist_raw[(uint64_t)s * NP + p] = hraw[p];
-    hpar[p] = 0;
+  for (int i = threadIdx.x; i < H * NP; i += blockDim.x) {
+    P.hist_raw[(uint64_t)s * H * NP + i] = hraw[i];
+    if (i < NP) hpar[i] = 0;
   }
   __syncthreads();
   if (LDSM != LDS_NONE) {
@@ -515,8 +518,10 @@ __global__ void __launch_bounds__(1024) k_scan_spill(const DevPlan *__restrict__
 // histograms (in place), partition totals into base_*[p].
 __global__ void k_part_offsets(const DevPlan *__restrict__ plan) {
   PlanRef P = plan_ref(plan);
-  const int p = blockIdx.x, NP = P.n_parts, S = P.n_segs;
+  const int p = blockIdx.x, NP = P.n_parts;
   for (int which = 0; which < 2; ++which) {
+    // raw records: a histogram per workgroup slice; partial states: per workgroup
+    const int S = which ? P.n_segs : P.n_segs * P.seg_slices;
     uint32_t *h = which ? P.hist_par : P.hist_raw;
     uint64_t carry = 0;
     for (int s0 = 0; s0 < S; s0 += kBlock) {
@@ -592,10 +597,38 @@ __device__ __forceinline__ void block_scan_u32(uint32_t *a, int n, uint32_t *wsu
   __syncthreads();
 }
 
+// N record words moved as 16-byte pieces (records are 8-byte aligned;
+// global loads and stores of 16 bytes need no more).
+typedef uint64_t u64x2a8 __attribute__((ext_vector_type(2), aligned(8)));
+
+template <int N>
+__device__ __forceinline__ void load_words(const DFR_GLOBAL uint64_t *p, uint64_t *w) {
+#pragma unroll
+  for (int k = 0; k + 1 < N; k += 2) {
+    const u64x2a8 q = *(const DFR_GLOBAL u64x2a8 *)(p + k);
+    w[k] = q.x;
+    w[k + 1] = q.y;
+  }
+  if constexpr (N % 2) w[N - 1] = p[N - 1];
+}
+
+template <int N>
+__device__ __forceinline__ void store_words(DFR_GLOBAL uint64_t *p, const uint64_t *w) {
+#pragma unroll
+  for (int k = 0; k + 1 < N; k += 2) {
+    u64x2a8 q;
+    q.x = w[k];
+    q.y = w[k + 1];
+    *(DFR_GLOBAL u64x2a8 *)(p + k) = q;
+  }
+  if constexpr (N % 2) p[N - 1] = w[N - 1];
+}
+
 // Moves n records of rw words from `src` to dst[cur[b] ...] in bucket order
 // (bucket_of(record word 0) < nb), CH records a round, advancing the
-// cursors. LDS: cnt[nb + 1], idx[CH], wsum[16].
-template <int RW, int CH, class BucketOf>
+// cursors. LDS: cnt[nb + 1], idx[CH], wsum[16]. EXACT: rw == RW, so each
+// record moves in 16-byte pieces.
+template <int RW, int CH, bool EXACT, class BucketOf>
 __device__ __forceinline__ void scatter_sorted(const DFR_GLOBAL uint64_t *src, uint32_t n, int rw, int nb,
                                                BucketOf bucket_of, unsigned long long *cur, uint32_t *cnt,
                                                uint32_t *idx, uint32_t *wsum, DFR_GLOBAL uint64_t *dst) {
@@ -645,16 +678,24 @@ __device__ __forceinline__ void scatter_sorted(const DFR_GLOBAL uint64_t *src, u
           const uint32_t b = v >> 16, i = v & 0xffffu;
           pos[g] = cur[b] + (slot - cnt[b]);
           const DFR_GLOBAL uint64_t *r = src + (uint64_t)(c0 + i) * rw;
+          if constexpr (EXACT) {
+            load_words<RW>(r, w[g]);
+          } else {
 #pragma unroll
-          for (int k = 0; k < RW; ++k) w[g][k] = k < rw ? r[k] : 0;
+            for (int k = 0; k < RW; ++k) w[g][k] = k < rw ? r[k] : 0;
+          }
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           if (s0 + g * kScatterThreads + threadIdx.x >= m) continue;
           DFR_GLOBAL uint64_t *d = dst + pos[g] * rw;
+          if constexpr (EXACT) {
+            store_words<RW>(d, w[g]);
+          } else {
 #pragma unroll
-          for (int k = 0; k < RW; ++k)
-            if (k < rw) d[k] = w[g][k];
+            for (int k = 0; k < RW; ++k)
+              if (k < rw) d[k] = w[g][k];
+          }
         }
       }
     } else {
@@ -673,38 +714,47 @@ __device__ __forceinline__ void scatter_sorted(const DFR_GLOBAL uint64_t *src, u
   }
 }
 
-// LDS bytes of k_part_scatter for np partitions.
-__host__ __device__ constexpr size_t part_scatter_lds(int np) {
-  return (size_t)2 * np * 8 + (size_t)(np + 1) * 4 + (size_t)kScatterChunk * 4;
+// LDS bytes of k_part_scatter for np partitions and chunks of ch records.
+__host__ __device__ constexpr size_t part_scatter_lds(int np, int ch) {
+  return (size_t)2 * np * 8 + (size_t)(np + 1) * 4 + (size_t)ch * 4;
 }
 
+// Workgroup (s, h) = blockIdx.x (s * seg_slices + h) moves the records of
+// slice h of pass A workgroup s's waves (and, for h = 0, its partial
+// states), from that slice's cursors.
+template <int CH>
 __global__ void __launch_bounds__(1024) k_part_scatter(const DevPlan *__restrict__ plan) {
   extern __shared__ unsigned long long cur[];  // [2 * n_parts], then cnt and idx
   __shared__ uint32_t wsum[kScatterThreads / 64];
   PlanRef P = plan_ref(plan);
-  const int s = blockIdx.x, NP = P.n_parts, W = P.words;
+  const int H = P.seg_slices, s = blockIdx.x / H, h = blockIdx.x % H, NP = P.n_parts, W = P.words;
   uint32_t *cnt = (uint32_t *)(cur + 2 * NP);
   uint32_t *idx = cnt + NP + 1;
   for (int p = threadIdx.x; p < NP; p += blockDim.x) {
-    cur[p] = P.base_raw[p] + P.hist_raw[(uint64_t)s * NP + p];
-    cur[NP + p] = P.base_par[p] + P.hist_par[(uint64_t)s * NP + p];
+    cur[p] = P.base_raw[p] + P.hist_raw[((uint64_t)s * H + h) * NP + p];
+    if (h == 0) cur[NP + p] = P.base_par[p] + P.hist_par[(uint64_t)s * NP + p];
   }
   __syncthreads();
   const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
   auto bucket = [&](uint64_t w0) { return part_of(P, w0 & keymask); };
-  for (int w = 0; w < P.seg_waves; ++w) {
+  const int per = P.seg_waves / H;
+  for (int w = h * per; w < (h + 1) * per; ++w) {
     const uint64_t wseg = (uint64_t)s * P.seg_waves + w;
     const DFR_GLOBAL uint64_t *rseg = gp(P.spill) + wseg * (uint64_t)P.seg_rows * P.rw;
     const uint32_t n = P.cnt_raw[wseg];
-    if (P.rw <= 2)
-      scatter_sorted<2, kScatterChunk>(rseg, n, P.rw, NP, bucket, cur, cnt, idx, wsum, gpw(P.part_raw));
-    else if (P.rw <= 4)
-      scatter_sorted<4, kScatterChunk>(rseg, n, P.rw, NP, bucket, cur, cnt, idx, wsum, gpw(P.part_raw));
+    DFR_GLOBAL uint64_t *dst = gpw(P.part_raw);
+    if (P.rw == 2)
+      scatter_sorted<2, CH, true>(rseg, n, 2, NP, bucket, cur, cnt, idx, wsum, dst);
+    else if (P.rw == 3)
+      scatter_sorted<3, CH, true>(rseg, n, 3, NP, bucket, cur, cnt, idx, wsum, dst);
+    else if (P.rw == 4)
+      scatter_sorted<4, CH, true>(rseg, n, 4, NP, bucket, cur, cnt, idx, wsum, dst);
     else if (P.rw <= 8)
-      scatter_sorted<8, kScatterChunk>(rseg, n, P.rw, NP, bucket, cur, cnt, idx, wsum, gpw(P.part_raw));
+      scatter_sorted<8, CH, false>(rseg, n, P.rw, NP, bucket, cur, cnt, idx, wsum, dst);
     else
-      scatter_sorted<kMaxCols + 3, kScatterChunk>(rseg, n, P.rw, NP, bucket, cur, cnt, idx, wsum, gpw(P.part_raw));
+      scatter_sorted<kMaxCols + 3, CH, false>(rseg, n, P.rw, NP, bucket, cur, cnt, idx, wsum, dst);
   }
+  if (h != 0) return;
   const DFR_GLOBAL uint64_t *pseg = gp(P.partials) + (uint64_t)s * (uint64_t)P.seg_partials * (1 + W);
   const uint32_t n = P.cnt_par[s];
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
@@ -946,14 +996,18 @@ __global__ void __launch_bounds__(1024) k_sub_scatter(const DevPlan *__restrict_
   auto bucket = [&](uint64_t w0) { return sub_of(P, w0 & keymask); };
   const DFR_GLOBAL uint64_t *s0 = src + lo * rw;
   const uint32_t n = (uint32_t)(hi - lo);
-  if (rw <= 2)
-    scatter_sorted<2, kSubScatterChunk>(s0, n, rw, NS, bucket, cur2, cnt, idx, wsum, dst);
-  else if (rw <= 4)
-    scatter_sorted<4, kSubScatterChunk>(s0, n, rw, NS, bucket, cur2, cnt, idx, wsum, dst);
+  if (rw == 2)
+    scatter_sorted<2, kSubScatterChunk, true>(s0, n, 2, NS, bucket, cur2, cnt, idx, wsum, dst);
+  else if (rw == 3)
+    scatter_sorted<3, kSubScatterChunk, true>(s0, n, 3, NS, bucket, cur2, cnt, idx, wsum, dst);
+  else if (rw == 4)
+    scatter_sorted<4, kSubScatterChunk, true>(s0, n, 4, NS, bucket, cur2, cnt, idx, wsum, dst);
+  else if (rw == 5)
+    scatter_sorted<5, kSubScatterChunk, true>(s0, n, 5, NS, bucket, cur2, cnt, idx, wsum, dst);
   else if (rw <= 8)
-    scatter_sorted<8, kSubScatterChunk>(s0, n, rw, NS, bucket, cur2, cnt, idx, wsum, dst);
+    scatter_sorted<8, kSubScatterChunk, false>(s0, n, rw, NS, bucket, cur2, cnt, idx, wsum, dst);
   else
-    scatter_sorted<kMaxWords + 1, kSubScatterChunk>(s0, n, rw, NS, bucket, cur2, cnt, idx, wsum, dst);
+    scatter_sorted<kMaxWords + 1, kSubScatterChunk, false>(s0, n, rw, NS, bucket, cur2, cnt, idx, wsum, dst);
 }
 #endif  // !DFR_JIT
 

@@ -183,6 +183,8 @@ struct DevPlan {
   int32_t c_log2;
   int32_t n_segs;        // workgroups of pass A
   int32_t seg_waves;     // waves per workgroup: one spill segment each
+  int32_t seg_slices;    // raw-record histograms per workgroup, one per run of
+                         // seg_waves / seg_slices waves (scattered separately)
   int64_t seg_rows;      // record capacity of a wave's spill segment
   int64_t seg_partials;  // partial-state capacity of a segment
   uint64_t *spill;       // n_segs * seg_rows * rw

@@ -133,7 +133,11 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * number of 8-byte words apart; 0: exactly the state's words),
  * "part_block" (threads of a first-level partition table's workgroup),
  * "two_level_parts" (first-level partitions of a two-level query, a power
- * of two up to "max_parts"). */
+ * of two up to "max_parts"), "scatter_slices" (the partitioned pass A's
+ * workgroups count their spilled records in this many histograms, one per
+ * run of their waves, and each is moved by its own scatter workgroup),
+ * "scatter_chunk" (records a scatter workgroup sorts at once: 8192 or
+ * 16384). */
 int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
 
 /* ---- device columns (the column hand-off) ------------------------------ */

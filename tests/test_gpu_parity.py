@@ -178,7 +178,8 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
                  "wg_per_cu": 2, "lds_probe": 4, "scan_unroll": 1, "agg_mode": 1, "two_level": -1,
                  "max_parts": 4096, "part_lds_bytes": 131072, "sub_lds_bytes": 49152, "sub_block": 512, "jit": 1,
                  "scan_prefetch": 1, "lds_odd_stride": 0, "part_block": 1024, "scan_block": 1024,
-                 "scan_wg_per_cu": 1, "scan_lds_bytes": 159744, "two_level_parts": 2048}
+                 "scan_wg_per_cu": 1, "scan_lds_bytes": 159744, "two_level_parts": 2048,
+                 "scatter_slices": 1, "scatter_chunk": 16384}
 
 
 @pytest.mark.parametrize("setting", [
@@ -202,6 +203,9 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
     {"scan_block": 256, "scan_wg_per_cu": 2, "scan_lds_bytes": 40000},
     {"scan_block": 512, "scan_wg_per_cu": 4, "scan_lds_bytes": 65536, "jit": 2},
     {"scan_block": 64, "scan_lds_bytes": 12000, "lds_mode": 2},
+    # the scatter's work split and chunk (slices that must divide 5 waves: one)
+    {"scatter_slices": 2}, {"scatter_slices": 16, "scatter_chunk": 8192},
+    {"scatter_slices": 4, "scan_block": 320, "two_level": 1},
 ])
 def test_table_modes_give_identical_results(setting, gpu_ctx):
     df = random_frame(30000, 5, key_card=400)
