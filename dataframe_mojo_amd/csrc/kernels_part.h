@@ -173,20 +173,6 @@ __device__ __forceinline__ int bucket_find(const PV &P, uint64_t *lkeys, uint64_
   return -1;
 }
 
-// Whether truth-table index m keeps its row. The truth words are uniform;
-// up to five leaves need only the first.
-template <class PV>
-__device__ __forceinline__ bool truth_keeps(const PV &P, uint32_t m) {
-  uint32_t w = P.truth[0];
-  if (P.n_leaves > 5) {
-    const uint32_t q = m >> 5;
-#pragma unroll
-    for (int i = 1; i < 8; ++i)
-      if (q == (uint32_t)i) w = P.truth[i];
-  }
-  return ((w >> (m & 31)) & 1u) != 0;
-}
-
 // One kept-or-not row of pass A: predicate already applied (`kept`); folds
 // into the LDS table or spills a record. Every lane of the wave calls it.
 template <int LDSM, int NC, class PV>
@@ -235,171 +221,6 @@ __device__ __forceinline__ void spill_row(const PV &P, uint64_t *lkeys, uint64_t
   }
 }
 
-// The raw loads of U row pairs per thread: values as loaded (32-bit
-// columns not yet widened) and the validity words holding each pair's
-// bits. Nothing here reads a loaded register, so a tile's loads stay in
-// flight until fold_tile uses them.
-template <int NC, int U>
-struct ScanTile {
-  uint64_t v[U][2][NC];
-  uint64_t vw[U][NC];  // validity word of row r (all ones without a bitmap)
-  uint32_t vx[U][NC];  // low half of the word after it (row r+1's bit when r's is bit 63)
-  bool any[U], two[U];
-};
-
-// Tile rows base + u * 2 * blockDim + 2 * thread (+1) of the even-length
-// range [b0, end), end - b0 >= 2. A pair past the range loads the range's
-// last pair instead (its rows are not folded), so every load is issued
-// unconditionally: a branch around a load, or between two load forms,
-// makes the compiler wait for the load where the paths merge.
-template <int NC, int U, class PV>
-__device__ __forceinline__ void load_tile(const PV &P, ScanTile<NC, U> &t, int64_t base, int64_t end) {
-  const int64_t stride = 2 * (int64_t)blockDim.x;
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int64_t r = base + u * stride + 2 * (int64_t)threadIdx.x;
-    t.any[u] = t.two[u] = r < end;
-    const int64_t rc = min(r, end - 2);
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      t.v[u][0][c] = t.v[u][1][c] = 0;
-      t.vw[u][c] = ~0ull;
-      t.vx[u][c] = ~0u;
-      if (c >= P.n_cols) continue;
-      const DevCol C = cget(P.cols[c]);
-      if (C.dtype == T_INT64 || C.dtype == T_FLOAT64) {
-        const DFR_GLOBAL uint64_t *p = gp((const uint64_t *)C.values) + rc;
-        if (P.vec_ok) {
-          const u64x2 q = __builtin_nontemporal_load((const DFR_GLOBAL u64x2 *)p);
-          t.v[u][0][c] = q.x;
-          t.v[u][1][c] = q.y;
-        } else {
-          t.v[u][0][c] = __builtin_nontemporal_load(p);
-          t.v[u][1][c] = __builtin_nontemporal_load(p + 1);
-        }
-      } else {
-        const DFR_GLOBAL uint32_t *p = gp((const uint32_t *)C.values) + rc;
-        if (P.vec_ok) {
-          const u32x2 q = __builtin_nontemporal_load((const DFR_GLOBAL u32x2 *)p);
-          t.v[u][0][c] = q.x;
-          t.v[u][1][c] = q.y;
-        } else {
-          t.v[u][0][c] = __builtin_nontemporal_load(p);
-          t.v[u][1][c] = __builtin_nontemporal_load(p + 1);
-        }
-      }
-      if (C.valid) {
-        const DFR_GLOBAL uint64_t *vp = gp(C.valid);
-        const int64_t w = (C.vbit + rc) >> 6, wlast = (C.vbit + end - 1) >> 6;
-        t.vw[u][c] = vp[w];
-        t.vx[u][c] = ((const DFR_GLOBAL uint32_t *)vp)[2 * min(w + 1, wlast)];
-      }
-    }
-  }
-}
-
-// The last row of a range of odd length: thread 0's row pair 0 holds it.
-template <int NC, class PV>
-__device__ __forceinline__ void load_last_row(const PV &P, ScanTile<NC, 1> &t, int64_t r) {
-  t.any[0] = threadIdx.x == 0;
-  t.two[0] = false;
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    t.v[0][0][c] = t.v[0][1][c] = 0;
-    t.vw[0][c] = ~0ull;
-    t.vx[0][c] = ~0u;
-    if (c >= P.n_cols || !t.any[0]) continue;
-    const DevCol C = cget(P.cols[c]);
-    if (C.dtype == T_INT64 || C.dtype == T_FLOAT64)
-      t.v[0][0][c] = gp((const uint64_t *)C.values)[r];
-    else
-      t.v[0][0][c] = gp((const uint32_t *)C.values)[r];
-    if (C.valid) t.vw[0][c] = gp(C.valid)[(C.vbit + r) >> 6];
-  }
-}
-
-// A raw 32-bit value as the column's 64-bit value (Int32 sign-extends).
-__device__ __forceinline__ uint64_t widen(const DevCol C, uint64_t x) {
-  if (C.dtype == T_INT32) return (uint64_t)(int64_t)(int32_t)(uint32_t)x;
-  if (C.dtype == T_INT64 || C.dtype == T_FLOAT64) return x;
-  return (uint32_t)x;
-}
-
-// Folds a loaded tile's 2U rows: predicate, then the LDS table or a spilled
-// record. Rolled (ROLL), one loop over the rows keeps a single copy of the
-// row code and picks the row's values from the tile registers with
-// selects. The pipelined scan unrolls it: a loop that issues no loads but
-// reads registers loaded before it makes the compiler wait for every load
-// in flight at its entry, the next tile's included.
-//
-// TL > 0 is the number of loads the next tile has in flight behind this
-// one: the fold first waits until only those remain, so this tile's
-// registers are complete on every path through it (otherwise a register
-// whose load a branch skipped waiting for is still pending where it is
-// next written, and that write waits for every load in flight). The
-// compiler still inserts any wait a use needs: the count only saves waits.
-template <int LDSM, int NC, int U, bool ROLL, int TL = 0, class PV>
-__device__ __forceinline__ void fold_tile(const PV &P, const ScanTile<NC, U> &t, uint64_t *lkeys, uint64_t *lst,
-                                          uint64_t *seg, uint32_t &wcur, uint32_t *hraw, int64_t base) {
-  const int64_t stride = 2 * (int64_t)blockDim.x;
-  if constexpr (TL > 0 && TL < 64)  // gfx9 s_waitcnt: vmcnt in bits 3:0 and 15:14; expcnt and lgkmcnt not waited
-    __builtin_amdgcn_s_waitcnt((TL & 15) | (7 << 4) | (15 << 8) | ((TL >> 4) << 14));
-#pragma unroll(ROLL ? 1 : 2 * U)
-  for (int i = 0; i < 2 * U; ++i) {
-    uint64_t vals[NC];
-    uint32_t vbits = 0;
-    bool in = false;
-    const int64_t row = base + (i >> 1) * stride + 2 * (int64_t)threadIdx.x + (i & 1);
-#pragma unroll
-    for (int c = 0; c < NC; ++c) vals[c] = 0;
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        if (i != 2 * u + j) continue;
-        in = j ? t.two[u] : t.any[u];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          if (c >= P.n_cols) continue;
-          const DevCol C = cget(P.cols[c]);
-          vals[c] = widen(C, t.v[u][j][c]);
-          uint32_t bitv;
-          if (!C.valid) {
-            bitv = 1u;
-          } else {
-            const int sh = (int)((C.vbit + row - j) & 63) + j;
-            bitv = sh < 64 ? (uint32_t)(t.vw[u][c] >> sh) & 1u : t.vx[u][c] & 1u;
-          }
-          vbits |= bitv << c;
-        }
-      }
-    bool kept = in;
-    if (in && P.n_leaves) {
-      uint32_t m = 0;
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        if (c >= P.n_cols || !((P.leaf_cols >> c) & 1)) continue;
-        const bool fl = P.cols[c].is_float;
-        if constexpr (RegPlan<PV>::value) {
-#pragma unroll
-          for (int l = 0; l < kMaxLeaves; ++l) {
-            if (l < P.leaf_begin[c] || l >= P.leaf_begin[c + 1]) continue;
-            const DevLeaf L = P.leaves[l];
-            m |= (uint32_t)leaf_true(L, fl, vals[c], (vbits >> c) & 1u) << L.bit;
-          }
-        } else {
-          for (int l = P.leaf_begin[c]; l < P.leaf_begin[c + 1]; ++l) {
-            const DevLeaf L = cget(P.leaves[l]);
-            m |= (uint32_t)leaf_true(L, fl, vals[c], (vbits >> c) & 1u) << L.bit;
-          }
-        }
-      }
-      kept = truth_keeps(P, m);
-    }
-    spill_row<LDSM, NC>(P, lkeys, lst, seg, wcur, hraw, kept, row, vals, vbits);
-  }
-}
-
 // Pass A. Workgroup s folds rows [s * chunk, (s + 1) * chunk); NC bounds
 // the number of input columns, U the row pairs a thread loads at once, PF
 // whether the next tile's loads overlap this tile's fold, TL the loads one
@@ -441,6 +262,9 @@ __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
   uint32_t *hw = hraw + (wave / ((blockDim.x >> 6) / H)) * NP;  // this wave's slice
   uint64_t *seg = P.spill + wseg * (uint64_t)P.seg_rows * P.rw;
   uint32_t wcur = 0;
+  auto spill = [&](bool kept, int64_t row, const uint64_t(&vals)[NC], uint32_t vbits) {
+    spill_row<LDSM, NC>(P, lkeys, lst, seg, wcur, hw, kept, row, vals, vbits);
+  };
   const int64_t b0 = (int64_t)s * chunk;
   const int64_t b1 = min(b0 + chunk, P.rows);
   const int64_t step = 2 * (int64_t)blockDim.x * U;
@@ -456,24 +280,24 @@ __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
     // (a tile past the range loads the range's last pair: in bounds)
     for (int64_t base = b0; base < b1e;) {
       load_tile<NC, U>(P, tb, base + step, b1e);
-      fold_tile<LDSM, NC, U, false, TL>(P, ta, lkeys, lst, seg, wcur, hw, base);
+      fold_tile<NC, U, false, TL>(P, ta, base, spill);
       base += step;
       if (base >= b1e) break;
       load_tile<NC, U>(P, ta, base + step, b1e);
-      fold_tile<LDSM, NC, U, false, TL>(P, tb, lkeys, lst, seg, wcur, hw, base);
+      fold_tile<NC, U, false, TL>(P, tb, base, spill);
       base += step;
     }
   } else {
     for (int64_t base = b0; base < b1e; base += step) {
       ScanTile<NC, U> t;
       load_tile<NC, U>(P, t, base, b1e);
-      fold_tile<LDSM, NC, U, true>(P, t, lkeys, lst, seg, wcur, hw, base);
+      fold_tile<NC, U, true>(P, t, base, spill);
     }
   }
   if (b1e < b1) {
     ScanTile<NC, 1> t;
     load_last_row<NC>(P, t, b1e);
-    fold_tile<LDSM, NC, 1, true>(P, t, lkeys, lst, seg, wcur, hw, b1e);
+    fold_tile<NC, 1, true>(P, t, b1e, spill);
   }
   if ((threadIdx.x & 63) == 0) P.cnt_raw[wseg] = wcur;
   __syncthreads();

@@ -32,6 +32,7 @@
  * per-group float error.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -186,11 +187,215 @@ static void set_err(char *err, int errlen, const char *msg) {
     }
 }
 
+/* Folds row r into group state s[0..n_aggs) (LEN is counted by the caller). */
+static void fold_row(state_t *s0, int n_aggs, const dfo_agg *aggs, const dfo_col *cols, int64_t r) {
+    for (int a = 0; a < n_aggs; ++a) {
+        const dfo_agg *A = &aggs[a];
+        state_t *s = &s0[a];
+        if (A->op == AGG_LEN) continue;
+        const dfo_col *c = &cols[A->col];
+        if (!col_valid(c, r)) continue;
+        s->count++;
+        const int fl = c->dtype == DFO_FLOAT64;
+        switch (A->op) {
+            case AGG_COUNT: break;
+            case AGG_SUM:
+            case AGG_MEAN:
+                if (fl)
+                    s->fsum += col_f64(c, r);
+                else
+                    s->isum += (i128)col_int(c, r);
+                break;
+            case AGG_MIN:
+            case AGG_MAX: {
+                const int is_max = A->op == AGG_MAX;
+                if (fl) {
+                    double x = col_f64(c, r);
+                    if (isnan(x)) {
+                        s->flags |= 2;
+                        break;
+                    }
+                    int better = is_max ? (x > s->fbest) : (x < s->fbest);
+                    if (!(s->flags & 1) || better) {
+                        s->fbest = x;
+                        s->flags |= 1;
+                    }
+                } else {
+                    int64_t x = col_int(c, r);
+                    int better = is_max ? (x > s->ibest) : (x < s->ibest);
+                    if (!(s->flags & 1) || better) {
+                        s->ibest = x;
+                        s->flags |= 1;
+                    }
+                }
+                break;
+            }
+            default: { /* VAR / STD: Welford, as VarState.add */
+                double x = col_f64(c, r);
+                s->mcount++;
+                double delta = x - s->mean;
+                s->mean += delta / (double)s->mcount;
+                s->m2 += delta * (x - s->mean);
+                break;
+            }
+        }
+    }
+}
+
+/* Finishes aggregate a of one group into agg_bits/agg_valid[a * n + g]. */
+static int finish_agg(const state_t *s, int64_t rows, const dfo_agg *A, const dfo_col *cols, int a, int64_t n,
+                      int64_t g, uint64_t *agg_bits, uint8_t *agg_valid, const char **errmsg) {
+    const dfo_col *c = A->op == AGG_LEN ? NULL : &cols[A->col];
+    uint64_t bits = 0;
+    int valid = 1, status = 0;
+    switch (A->op) {
+        case AGG_LEN: bits = (uint64_t)rows; break;
+        case AGG_COUNT: bits = (uint64_t)s->count; break;
+        case AGG_SUM:
+            valid = s->count >= A->min_count;
+            if (c->dtype == DFO_FLOAT64) {
+                double v = valid ? s->fsum : 0.0;
+                memcpy(&bits, &v, 8);
+            } else {
+                i128 tot = valid ? s->isum : 0;
+                if (c->dtype == DFO_INT32) {
+                    if (tot > 2147483647 || tot < -2147483647 - 1) {
+                        *errmsg = "int32 expression sum overflow";
+                        status = 2;
+                    }
+                } else if (tot > (i128)INT64_MAX || tot < (i128)INT64_MIN) {
+                    *errmsg = "Int64 expression sum overflow";
+                    status = 2;
+                }
+                bits = (uint64_t)(int64_t)tot;
+            }
+            break;
+        case AGG_MIN:
+        case AGG_MAX:
+            if (c->dtype == DFO_FLOAT64) {
+                valid = (s->flags & 3) != 0;
+                double v = (s->flags & 1) ? s->fbest : ((s->flags & 2) ? NAN : 0.0);
+                memcpy(&bits, &v, 8);
+            } else {
+                valid = (s->flags & 1) != 0;
+                bits = valid ? (uint64_t)s->ibest : 0;
+            }
+            break;
+        case AGG_MEAN: {
+            valid = s->count > 0;
+            double tot = c->dtype == DFO_FLOAT64 ? s->fsum : (double)s->isum;
+            double v = valid ? tot / (double)s->count : 0.0;
+            memcpy(&bits, &v, 8);
+            break;
+        }
+        default: {
+            valid = s->mcount - (int64_t)A->ddof > 0;
+            double v = valid ? s->m2 / (double)(s->mcount - A->ddof) : 0.0;
+            if (A->op == AGG_STD && valid) v = sqrt(v);
+            memcpy(&bits, &v, 8);
+            break;
+        }
+    }
+    agg_bits[(int64_t)a * n + g] = bits;
+    agg_valid[(int64_t)a * n + g] = (uint8_t)valid;
+    return status;
+}
+
+/* Writes key k of group g (table slot tg) to key_vals/key_valid[k * n + g]. */
+static void emit_key(const table_t *t, int64_t tg, const dfo_col *keys, int k, int64_t first_row, int64_t n, int64_t g,
+                     int64_t *key_vals, uint8_t *key_valid) {
+    const int nk = t->n_keys;
+    key_valid[k * n + g] = !t->keynull[tg * nk + k];
+    key_vals[k * n + g] = t->keyvals[tg * nk + k];
+    /* a Float64 key reports the group's first row's own value, as the
+     * reference's first() of the key (group_by.mojo:319-329) */
+    if (keys[k].dtype == DFO_FLOAT64 && !t->keynull[tg * nk + k])
+        memcpy(&key_vals[k * n + g], &((const double *)keys[k].values)[first_row], 8);
+}
+
+/* The key tuple of row r. */
+static void row_key(int n_keys, const dfo_col *keys, int64_t r, int64_t *kv, uint8_t *kn) {
+    for (int k = 0; k < n_keys; ++k) {
+        kn[k] = !col_valid(&keys[k], r);
+        kv[k] = kn[k] ? 0 : key_word(&keys[k], r);
+    }
+}
+
+/* The groups of one run of rows, folded in the given (ascending) order. */
+typedef struct {
+    table_t t;
+    state_t *st;
+    int64_t *rows_of, *first_rows;
+    int64_t scap;
+} groups_t;
+
+static void groups_init(groups_t *G, int n_keys, int n_aggs) {
+    memset(G, 0, sizeof *G);
+    G->t.n_keys = n_keys;
+    table_grow(&G->t);
+    G->scap = 1024;
+    G->st = calloc((size_t)G->scap * (n_aggs ? n_aggs : 1), sizeof(state_t));
+    G->rows_of = malloc((size_t)G->scap * sizeof(int64_t));
+    G->first_rows = malloc((size_t)G->scap * sizeof(int64_t));
+}
+
+static void groups_add(groups_t *G, int n_keys, const dfo_col *keys, int n_aggs, const dfo_agg *aggs,
+                       const dfo_col *cols, int64_t r) {
+    int64_t kv[8];
+    uint8_t kn[8];
+    row_key(n_keys, keys, r, kv, kn);
+    int created;
+    const int na = n_aggs ? n_aggs : 1;
+    int64_t g = table_find(&G->t, kv, kn, &created);
+    if (created) {
+        if (g >= G->scap) {
+            int64_t ncap = G->scap * 2;
+            G->st = realloc(G->st, (size_t)ncap * na * sizeof(state_t));
+            memset(G->st + G->scap * na, 0, (size_t)(ncap - G->scap) * na * sizeof(state_t));
+            G->rows_of = realloc(G->rows_of, (size_t)ncap * sizeof(int64_t));
+            G->first_rows = realloc(G->first_rows, (size_t)ncap * sizeof(int64_t));
+            G->scap = ncap;
+        }
+        G->first_rows[g] = r;
+        G->rows_of[g] = 0;
+    }
+    G->rows_of[g]++;
+    fold_row(&G->st[g * n_aggs], n_aggs, aggs, cols, r);
+}
+
+static void groups_free(groups_t *G) {
+    free(G->st);
+    free(G->rows_of);
+    free(G->first_rows);
+    free(G->t.slot_group);
+    free(G->t.keyvals);
+    free(G->t.keynull);
+}
+
+/* Writes the groups of G at output rows off.. (keys, first rows, aggregates). */
+static int groups_emit(const groups_t *G, int n_keys, const dfo_col *keys, int n_aggs, const dfo_agg *aggs,
+                       const dfo_col *cols, int64_t n, int64_t off, int64_t *first_rows, int64_t *key_vals,
+                       uint8_t *key_valid, uint64_t *agg_bits, uint8_t *agg_valid, const char **errmsg) {
+    int status = 0;
+    for (int64_t g = 0; g < G->t.groups; ++g) {
+        first_rows[off + g] = G->first_rows[g];
+        for (int k = 0; k < n_keys; ++k)
+            emit_key(&G->t, g, keys, k, G->first_rows[g], n, off + g, key_vals, key_valid);
+        for (int a = 0; a < n_aggs; ++a) {
+            int st = finish_agg(&G->st[g * n_aggs + a], G->rows_of[g], &aggs[a], cols, a, n, off + g, agg_bits,
+                                agg_valid, errmsg);
+            if (st) status = st;
+        }
+    }
+    return status;
+}
+
 /*
  * Groups the kept rows (kept[r] != 0) by the key columns and finishes each
  * aggregate. Outputs are sized for `n` groups by the caller:
  *   first_rows[g], key_vals[k * n + g], key_valid[k * n + g],
  *   agg_bits[a * n + g] (int64 value or double bits), agg_valid[a * n + g].
+ * Groups come in order of their first kept row.
  * Returns 0, or 2 with `err` set to the reference's error text.
  */
 int dfo_groupby(int64_t n, int n_keys, const dfo_col *keys, const uint8_t *kept, int n_cols,
@@ -198,163 +403,97 @@ int dfo_groupby(int64_t n, int n_keys, const dfo_col *keys, const uint8_t *kept,
                 int64_t *first_rows, int64_t *key_vals, uint8_t *key_valid, uint64_t *agg_bits,
                 uint8_t *agg_valid, char *err, int errlen) {
     (void)n_cols;
-    table_t t;
-    memset(&t, 0, sizeof t);
-    t.n_keys = n_keys;
-    table_grow(&t);
-    int64_t scap = 1024;
-    state_t *st = calloc((size_t)scap * (n_aggs ? n_aggs : 1), sizeof(state_t));
-    int64_t *rows_of = malloc((size_t)scap * sizeof(int64_t));
-    int64_t kv[8];
-    uint8_t kn[8];
-    for (int64_t r = 0; r < n; ++r) {
-        if (!kept[r]) continue;
-        for (int k = 0; k < n_keys; ++k) {
-            kn[k] = !col_valid(&keys[k], r);
-            kv[k] = kn[k] ? 0 : key_word(&keys[k], r);
-        }
-        int created;
-        int64_t g = table_find(&t, kv, kn, &created);
-        if (created) {
-            if (g >= scap) {
-                int64_t ncap = scap * 2;
-                st = realloc(st, (size_t)ncap * (n_aggs ? n_aggs : 1) * sizeof(state_t));
-                memset(st + scap * (n_aggs ? n_aggs : 1), 0, (size_t)(ncap - scap) * (n_aggs ? n_aggs : 1) * sizeof(state_t));
-                rows_of = realloc(rows_of, (size_t)ncap * sizeof(int64_t));
-                scap = ncap;
+    groups_t G;
+    groups_init(&G, n_keys, n_aggs);
+    for (int64_t r = 0; r < n; ++r)
+        if (kept[r]) groups_add(&G, n_keys, keys, n_aggs, aggs, cols, r);
+    *out_groups = G.t.groups;
+    const char *msg = NULL;
+    int status = groups_emit(&G, n_keys, keys, n_aggs, aggs, cols, n, 0, first_rows, key_vals, key_valid, agg_bits,
+                             agg_valid, &msg);
+    if (status) set_err(err, errlen, msg);
+    groups_free(&G);
+    return status;
+}
+
+/*
+ * The same grouping by `threads` workers, as the reference CPU engine
+ * organises it (morsel-parallel workers, per-worker radix partitions by key
+ * hash, then partition-parallel aggregation and finish; hash_table.mojo:1-49,
+ * 2850 _FinishJob). Worker t takes the contiguous rows
+ * [n * t / threads, n * (t + 1) / threads) and appends each kept row to its
+ * list for the row's key partition; each partition is then folded by one
+ * worker over the lists in worker order, i.e. in ascending row order, so
+ * every group sees its rows in the sequential order and every state (float
+ * sums and moments included) is bit-identical to dfo_groupby's. Groups come
+ * partition by partition (not in first-row order); first_rows tells the
+ * caller the order dfo_groupby gives.
+ */
+typedef struct {
+    int64_t *v;
+    int64_t n, cap;
+} rowvec_t;
+
+int dfo_groupby_parallel(int threads, int64_t n, int n_keys, const dfo_col *keys, const uint8_t *kept, int n_cols,
+                         const dfo_col *cols, int n_aggs, const dfo_agg *aggs, int64_t *out_groups,
+                         int64_t *first_rows, int64_t *key_vals, uint8_t *key_valid, uint64_t *agg_bits,
+                         uint8_t *agg_valid, char *err, int errlen) {
+    (void)n_cols;
+    enum { LOG_P = 8, NP = 1 << LOG_P };
+    if (threads < 1) threads = 1;
+    const int T = threads;
+    rowvec_t *lists = calloc((size_t)T * NP, sizeof(rowvec_t));
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        const int64_t r0 = n * t / T, r1 = n * (t + 1) / T;
+        int64_t kv[8];
+        uint8_t kn[8];
+        for (int64_t r = r0; r < r1; ++r) {
+            if (!kept[r]) continue;
+            row_key(n_keys, keys, r, kv, kn);
+            const int p = (int)(key_hash(n_keys, kv, kn) >> (64 - LOG_P));
+            rowvec_t *L = &lists[(size_t)t * NP + p];
+            if (L->n == L->cap) {
+                L->cap = L->cap ? L->cap * 2 : 256;
+                L->v = realloc(L->v, (size_t)L->cap * sizeof(int64_t));
             }
-            first_rows[g] = r;
-            rows_of[g] = 0;
-        }
-        rows_of[g]++;
-        for (int a = 0; a < n_aggs; ++a) {
-            const dfo_agg *A = &aggs[a];
-            state_t *s = &st[g * n_aggs + a];
-            if (A->op == AGG_LEN) continue;
-            const dfo_col *c = &cols[A->col];
-            if (!col_valid(c, r)) continue;
-            s->count++;
-            const int fl = c->dtype == DFO_FLOAT64;
-            switch (A->op) {
-                case AGG_COUNT: break;
-                case AGG_SUM:
-                case AGG_MEAN:
-                    if (fl)
-                        s->fsum += col_f64(c, r);
-                    else
-                        s->isum += (i128)col_int(c, r);
-                    break;
-                case AGG_MIN:
-                case AGG_MAX: {
-                    const int is_max = A->op == AGG_MAX;
-                    if (fl) {
-                        double x = col_f64(c, r);
-                        if (isnan(x)) {
-                            s->flags |= 2;
-                            break;
-                        }
-                        int better = is_max ? (x > s->fbest) : (x < s->fbest);
-                        if (!(s->flags & 1) || better) {
-                            s->fbest = x;
-                            s->flags |= 1;
-                        }
-                    } else {
-                        int64_t x = col_int(c, r);
-                        int better = is_max ? (x > s->ibest) : (x < s->ibest);
-                        if (!(s->flags & 1) || better) {
-                            s->ibest = x;
-                            s->flags |= 1;
-                        }
-                    }
-                    break;
-                }
-                default: { /* VAR / STD: Welford, as VarState.add */
-                    double x = col_f64(c, r);
-                    s->mcount++;
-                    double delta = x - s->mean;
-                    s->mean += delta / (double)s->mcount;
-                    s->m2 += delta * (x - s->mean);
-                    break;
-                }
-            }
+            L->v[L->n++] = r;
         }
     }
-    const int64_t G = t.groups;
-    *out_groups = G;
+    groups_t *parts = calloc(NP, sizeof(groups_t));
+#pragma omp parallel for schedule(dynamic) num_threads(T)
+    for (int p = 0; p < NP; ++p) {
+        groups_init(&parts[p], n_keys, n_aggs);
+        for (int t = 0; t < T; ++t) {
+            const rowvec_t *L = &lists[(size_t)t * NP + p];
+            for (int64_t i = 0; i < L->n; ++i) groups_add(&parts[p], n_keys, keys, n_aggs, aggs, cols, L->v[i]);
+        }
+    }
+    int64_t off[NP + 1];
+    off[0] = 0;
+    for (int p = 0; p < NP; ++p) off[p + 1] = off[p] + parts[p].t.groups;
+    *out_groups = off[NP];
     int status = 0;
-    for (int k = 0; k < n_keys; ++k)
-        for (int64_t g = 0; g < G; ++g) {
-            key_valid[k * n + g] = !t.keynull[g * n_keys + k];
-            key_vals[k * n + g] = t.keyvals[g * n_keys + k];
-            /* a Float64 key reports the group's first row's own value, as
-             * the reference's first() of the key (group_by.mojo:319-329) */
-            if (keys[k].dtype == DFO_FLOAT64 && !t.keynull[g * n_keys + k])
-                memcpy(&key_vals[k * n + g], &((const double *)keys[k].values)[first_rows[g]], 8);
-        }
-    for (int a = 0; a < n_aggs && status == 0; ++a) {
-        const dfo_agg *A = &aggs[a];
-        const dfo_col *c = A->op == AGG_LEN ? NULL : &cols[A->col];
-        for (int64_t g = 0; g < G; ++g) {
-            state_t *s = &st[g * n_aggs + a];
-            uint64_t bits = 0;
-            int valid = 1;
-            switch (A->op) {
-                case AGG_LEN: bits = (uint64_t)rows_of[g]; break;
-                case AGG_COUNT: bits = (uint64_t)s->count; break;
-                case AGG_SUM:
-                    valid = s->count >= A->min_count;
-                    if (c->dtype == DFO_FLOAT64) {
-                        double v = valid ? s->fsum : 0.0;
-                        memcpy(&bits, &v, 8);
-                    } else {
-                        i128 tot = valid ? s->isum : 0;
-                        if (c->dtype == DFO_INT32) {
-                            if (tot > 2147483647 || tot < -2147483647 - 1) {
-                                set_err(err, errlen, "int32 expression sum overflow");
-                                status = 2;
-                            }
-                        } else if (tot > (i128)INT64_MAX || tot < (i128)INT64_MIN) {
-                            set_err(err, errlen, "Int64 expression sum overflow");
-                            status = 2;
-                        }
-                        bits = (uint64_t)(int64_t)tot;
-                    }
-                    break;
-                case AGG_MIN:
-                case AGG_MAX:
-                    if (c->dtype == DFO_FLOAT64) {
-                        valid = (s->flags & 3) != 0;
-                        double v = (s->flags & 1) ? s->fbest : ((s->flags & 2) ? NAN : 0.0);
-                        memcpy(&bits, &v, 8);
-                    } else {
-                        valid = (s->flags & 1) != 0;
-                        bits = valid ? (uint64_t)s->ibest : 0;
-                    }
-                    break;
-                case AGG_MEAN: {
-                    valid = s->count > 0;
-                    double tot = c->dtype == DFO_FLOAT64 ? s->fsum : (double)s->isum;
-                    double v = valid ? tot / (double)s->count : 0.0;
-                    memcpy(&bits, &v, 8);
-                    break;
-                }
-                default: {
-                    valid = s->mcount - (int64_t)A->ddof > 0;
-                    double v = valid ? s->m2 / (double)(s->mcount - A->ddof) : 0.0;
-                    if (A->op == AGG_STD && valid) v = sqrt(v);
-                    memcpy(&bits, &v, 8);
-                    break;
-                }
+    const char *msg = NULL;
+#pragma omp parallel for schedule(dynamic) num_threads(T)
+    for (int p = 0; p < NP; ++p) {
+        const char *m = NULL;
+        int st = groups_emit(&parts[p], n_keys, keys, n_aggs, aggs, cols, n, off[p], first_rows, key_vals, key_valid,
+                             agg_bits, agg_valid, &m);
+        if (st) {
+#pragma omp critical
+            {
+                status = st;
+                msg = m;
             }
-            agg_bits[(int64_t)a * n + g] = bits;
-            agg_valid[(int64_t)a * n + g] = (uint8_t)valid;
         }
     }
-    free(st);
-    free(rows_of);
-    free(t.slot_group);
-    free(t.keyvals);
-    free(t.keynull);
+    if (status) set_err(err, errlen, msg);
+    for (int p = 0; p < NP; ++p) groups_free(&parts[p]);
+    free(parts);
+    for (int i = 0; i < T * NP; ++i) free(lists[i].v);
+    free(lists);
     return status;
 }
 

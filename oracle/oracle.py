@@ -62,6 +62,8 @@ def lib():
         vp, i64 = ctypes.c_void_p, ctypes.c_int64
         L.dfo_groupby.argtypes = [i64, ctypes.c_int, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp, ctypes.c_char_p, ctypes.c_int]
         L.dfo_groupby.restype = ctypes.c_int
+        L.dfo_groupby_parallel.argtypes = [ctypes.c_int] + L.dfo_groupby.argtypes
+        L.dfo_groupby_parallel.restype = ctypes.c_int
         L.dfo_dd_moments.argtypes = [i64, vp, vp, vp, i64, ctypes.c_int, vp, vp, vp, vp]
         _lib = L
     return _lib
@@ -168,9 +170,12 @@ def _out_dtype(op: str, dt: str) -> str:
     return dt
 
 
-def groupby(frame, keep: np.ndarray, keys: list, aggs: list, maintain_order: bool = True):
+def groupby(frame, keep: np.ndarray, keys: list, aggs: list, maintain_order: bool = True, threads: int = 1):
     """Runs the C oracle. `aggs`: (op, column name or None, ddof, min_count).
-    Returns (DataFrame pieces as (name, Column) lists, first_rows)."""
+    Returns (DataFrame pieces as (name, Column) lists, first_rows). With
+    threads > 1 the parallel restatement runs (dfo_groupby_parallel: the
+    same states, bit for bit); its groups are put in first-row order when
+    `maintain_order` asks for it."""
     n = frame.height()
     hold: list = []
     kcols = (_Col * max(1, len(keys)))(*[_ocol(frame.column(k).column, hold) for k in keys])
@@ -194,14 +199,21 @@ def groupby(frame, keep: np.ndarray, keys: list, aggs: list, maintain_order: boo
     ab = np.empty(max(1, len(aggs)) * cap, dtype=np.uint64)
     aval = np.empty(max(1, len(aggs)) * cap, dtype=np.uint8)
     err = ctypes.create_string_buffer(256)
-    rc = lib().dfo_groupby(
+    args = (
         n, len(keys), ctypes.addressof(kcols), keep8.ctypes.data, len(names), ctypes.addressof(vcols), len(aggs),
         ctypes.addressof(oaggs), ctypes.addressof(groups), first.ctypes.data, kv.ctypes.data, kval.ctypes.data,
         ab.ctypes.data, aval.ctypes.data, err, len(err),
     )
+    rc = lib().dfo_groupby_parallel(threads, *args) if threads > 1 else lib().dfo_groupby(*args)
     if rc != 0:
         raise OracleError(err.value.decode())
     G = groups.value
+    if threads > 1 and maintain_order:
+        order = np.argsort(first[:G], kind="stable")
+        first[:G] = first[:G][order]
+        for arr, count in ((kv, len(keys)), (kval, len(keys)), (ab, len(aggs)), (aval, len(aggs))):
+            for i in range(count):
+                arr[i * cap : i * cap + G] = arr[i * cap : i * cap + G][order]
     out = []
     for i, k in enumerate(keys):
         src = frame.column(k).column
@@ -228,8 +240,9 @@ def groupby(frame, keep: np.ndarray, keys: list, aggs: list, maintain_order: boo
     return out, agg_cols, first[:G].copy()
 
 
-def collect(plan):
-    """The oracle's answer for a LazyFrame: filter -> group_by -> agg."""
+def collect(plan, threads: int = 1):
+    """The oracle's answer for a LazyFrame: filter -> group_by -> agg (on
+    `threads` workers when more than one)."""
     frame = plan.source
     keep = kept_rows(frame, plan.filters)
     specs, names = [], []
@@ -238,7 +251,7 @@ def collect(plan):
         kw = dict(kw)
         specs.append((op, None if target is None else target.args[0], int(kw.get("ddof", 1)), int(kw.get("min_count", 0))))
         names.append(e.output_name())
-    key_cols, agg_cols, first = groupby(frame, keep, list(plan.keys), specs, plan.maintain_order)
+    key_cols, agg_cols, first = groupby(frame, keep, list(plan.keys), specs, plan.maintain_order, threads)
     series = [Series(k, c) for k, c in key_cols] + [Series(nm, c) for nm, c in zip(names, agg_cols)]
     return DataFrame(series), first
 

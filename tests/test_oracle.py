@@ -271,3 +271,47 @@ def test_float_keys_canonical_groups_keep_first_value():
     assert np.signbit(k.values[0]) and k.values[0] == 0.0  # the first row's -0.0
     assert math.isnan(k.values[1]) and k.values[2] == 1.5 and not k.is_valid()[3]
     assert out.to_dict()["v"] == [9, 8, 11, 8] and out.to_dict()["len"] == [3, 2, 2, 1]
+
+
+@pytest.mark.parametrize("threads", [2, 3, 8])
+@pytest.mark.parametrize("keys", [["k"], ["k", "s"], ["f"]])
+def test_parallel_restatement_equals_sequential(threads, keys):
+    """dfo_groupby_parallel (workers over contiguous row ranges, per-worker
+    partitions by key hash, partition-parallel folding and finish) gives
+    every group the same state as the sequential dfo_groupby, bit for bit,
+    Float64 sums and moments included, because each partition folds its
+    rows in ascending row order; in first-row order when ordered."""
+    rng = np.random.default_rng(threads * 7 + len(keys))
+    n = 60000
+    df = DataFrame([
+        Series("k", Column.int64(rng.integers(-300, 300, n), rng.random(n) > 0.02)),
+        Series("s", Column.int32(rng.integers(0, 7, n))),
+        Series("f", Column.float64(rng.choice([0.0, -0.0, 1.5, np.nan, -2.0], n), rng.random(n) > 0.05)),
+        Series("v", Column.int64(rng.integers(-10**12, 10**12, n), rng.random(n) > 0.1)),
+        Series("x", Column.float64(rng.normal(size=n) * 1e3, rng.random(n) > 0.05)),
+    ])
+    aggs = [len_(), col("v").sum().alias("v_sum"), col("v").min().alias("v_min"), col("x").sum().alias("x_sum"),
+            col("x").mean().alias("x_mean"), col("x").var().alias("x_var"), col("x").max().alias("x_max"),
+            col("v").count().alias("v_count")]
+    for mo in (True, False):
+        lf = df.filter(col("v") > -5 * 10**11).group_by(*keys, maintain_order=mo).agg(aggs)
+        seq, fs = oracle.collect(lf)
+        par, fp = oracle.collect(lf, threads=threads)
+        if not mo:
+            assert sorted(fp) == sorted(fs)
+            o = np.argsort(fp, kind="stable")
+            fp = fp[o]
+            par = DataFrame([Series(s.name, Column(s.column.dtype, s.column.values[o], s.column.is_valid()[o],
+                                                   s.column.labels)) for s in par._series])
+        assert np.array_equal(fs, fp)
+        for name in seq.columns:
+            a, b = seq.column(name).column, par.column(name).column
+            assert np.array_equal(a.is_valid(), b.is_valid()), name
+            assert np.array_equal(a.values.view(np.uint8), b.values.view(np.uint8)), name
+
+
+def test_parallel_restatement_raises_the_same_overflow():
+    df = DataFrame([Series("k", Column.int64([1, 1, 2])), Series("v", Column.int64([INT64_MAX, 1, 5]))])
+    lf = df.lazy().group_by("k").agg(col("v").sum())
+    with pytest.raises(oracle.OracleError, match="Int64 expression sum overflow"):
+        oracle.collect(lf, threads=4)
