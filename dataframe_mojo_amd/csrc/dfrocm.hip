@@ -305,6 +305,8 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.batch_graph = (int)value;
   else if (k == "scatter_slices" && value >= 1 && value <= 16 && (value & (value - 1)) == 0)
     t.scatter_slices = (int)value;
+  else if (k == "scatter_stage" && (value == 0 || value == 1))
+    t.scatter_stage = (int)value;
   else if (k == "scatter_chunk" && (value == 8192 || value == 16384))
     t.scatter_chunk = (int)value;
   else if (k == "two_level_parts" && value >= 1 && value <= 4096 && (value & (value - 1)) == 0)

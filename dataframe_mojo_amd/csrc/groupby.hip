@@ -67,6 +67,7 @@ Tuning default_tuning(dfr_context *ctx) {
   t.two_level_parts = env_int("DFR_TWO_LEVEL_PARTS", 2048);
   t.scatter_slices = env_int("DFR_SCATTER_SLICES", 1);
   t.scatter_chunk = env_int("DFR_SCATTER_CHUNK", 16384) == 8192 ? 8192 : 16384;
+  t.scatter_stage = env_int("DFR_SCATTER_STAGE", 1) ? 1 : 0;
   t.scan_block = env_int("DFR_SCAN_BLOCK", 1024);
   t.scan_wg_per_cu = env_int("DFR_SCAN_WG_PER_CU", 1);
   t.scan_lds_bytes = std::min(env_int("DFR_SCAN_LDS_BYTES", 159744), ctx->lds_per_block);
@@ -701,7 +702,11 @@ int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan
   // k_part_scatter: 64-bit write cursors per partition and kind, then its
   // chunk sort's counters and record indices
   const dim3 sgrid((unsigned)(P.n_segs * P.seg_slices));
-  if (ctx->tune.scatter_chunk == 8192) {
+  const size_t staged_bytes = part_scatter_staged_lds(P.n_parts, P.rw);
+  if (ctx->tune.scatter_stage && staged_it(P.rw) > 0 && staged_bytes + 1024 <= (size_t)ctx->lds_per_block) {
+    if ((st = set_lds(k_part_scatter_staged, staged_bytes, error))) return st;
+    hipLaunchKernelGGL(k_part_scatter_staged, sgrid, dim3(kScatterThreads), staged_bytes, ctx->stream, dplan);
+  } else if (ctx->tune.scatter_chunk == 8192) {
     const size_t cur_bytes = part_scatter_lds(P.n_parts, 8192);
     if ((st = set_lds(k_part_scatter<8192>, cur_bytes, error))) return st;
     hipLaunchKernelGGL(k_part_scatter<8192>, sgrid, dim3(kScatterThreads), cur_bytes, ctx->stream, dplan);
@@ -764,9 +769,12 @@ int launch_second_level(dfr_context *ctx, const Compiled &C, const DevPlan *dpla
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, ctx->stream, bs, nb);
     hipLaunchKernelGGL(k_count_apply, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, h, N2, bs, b2);
   }
-  const size_t sc_bytes = sub_scatter_lds((int)NS);
+  // raw records staged through LDS when they fit (partial states gather)
+  const size_t staged = sub_scatter_staged_lds((int)NS, P.rw);
+  const int stage = ctx->tune.scatter_stage && staged_it(P.rw) > 0 && staged + 1024 <= (size_t)ctx->lds_per_block;
+  const size_t sc_bytes = std::max(sub_scatter_lds((int)NS), stage ? staged : (size_t)0);
   if ((st = set_lds(k_sub_scatter, sc_bytes, error))) return st;
-  hipLaunchKernelGGL(k_sub_scatter, grid2, dim3(kScatterThreads), sc_bytes, ctx->stream, dplan);
+  hipLaunchKernelGGL(k_sub_scatter, grid2, dim3(kScatterThreads), sc_bytes, ctx->stream, dplan, stage);
   void *args[] = {(void *)&dplan};
   const unsigned n_sub_parts = (unsigned)((uint64_t)P.n_parts * NS);
   if (!jit_launch(ctx, C, JIT_SUB, n_sub_parts, ctx->tune.sub_block, C.c_lds_bytes, args)) {

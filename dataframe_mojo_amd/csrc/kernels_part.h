@@ -538,9 +538,91 @@ __device__ __forceinline__ void scatter_sorted(const DFR_GLOBAL uint64_t *src, u
   }
 }
 
+// Moves n records of exactly RW words from `src` to dst[cur[b] ...] in
+// bucket order, IT * 1024 records a round. Each thread loads its records
+// whole (consecutive records across the lanes, so every line is read once,
+// coalesced); the workgroup counts and ranks them by bucket, places them in
+// LDS at their sorted positions, then streams them out in that order, so
+// nothing is read twice. LDS: cnt[nb + 1], stage[IT * 1024 * RW], wsum[16].
+template <int RW, int IT, class BucketOf>
+__device__ __forceinline__ void scatter_staged(const DFR_GLOBAL uint64_t *src, uint32_t n, int nb, BucketOf bucket_of,
+                                               unsigned long long *cur, uint32_t *cnt, uint64_t *stage,
+                                               uint32_t *wsum, DFR_GLOBAL uint64_t *dst) {
+  constexpr int CH = IT * kScatterThreads;
+  static_assert(CH <= 65536, "ranks fit 16 bits");
+  for (uint32_t c0 = 0; c0 < n; c0 += CH) {
+    const uint32_t m = min((uint32_t)CH, n - c0);
+    for (int b = threadIdx.x; b <= nb; b += kScatterThreads) cnt[b] = 0;
+    uint64_t w[IT][RW];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const uint32_t i = min((uint32_t)(it * kScatterThreads + threadIdx.x), m - 1);
+      load_words<RW>(src + (uint64_t)(c0 + i) * RW, w[it]);
+    }
+    __syncthreads();
+    uint32_t br[IT];  // bucket << 16 | rank within the bucket
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const uint32_t i = it * kScatterThreads + threadIdx.x;
+      br[it] = 0;
+      if (i < m) {
+        const uint32_t b = bucket_of(w[it][0]);
+        br[it] = (b << 16) | atomicAdd(&cnt[b], 1u);
+      }
+    }
+    __syncthreads();
+    block_scan_u32(cnt, nb, wsum);
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const uint32_t i = it * kScatterThreads + threadIdx.x;
+      if (i < m) {
+        uint64_t *d = stage + (uint64_t)(cnt[br[it] >> 16] + (br[it] & 0xffffu)) * RW;
+#pragma unroll
+        for (int k = 0; k < RW; ++k) d[k] = w[it][k];
+      }
+    }
+    __syncthreads();
+    for (uint32_t slot = threadIdx.x; slot < m; slot += kScatterThreads) {
+      uint64_t x[RW];
+#pragma unroll
+      for (int k = 0; k < RW; ++k) x[k] = stage[(uint64_t)slot * RW + k];
+      const uint32_t b = bucket_of(x[0]);
+      store_words<RW>(dst + (cur[b] + (slot - cnt[b])) * RW, x);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += kScatterThreads) cur[b] += cnt[b + 1] - cnt[b];
+    __syncthreads();
+  }
+}
+
+// Records per thread and round of the staged scatter, by record words (0:
+// not staged).
+__host__ __device__ constexpr int staged_it(int rw) { return rw == 2 ? 6 : rw == 3 ? 4 : rw == 4 ? 3 : 0; }
+
+// LDS bytes of k_part_scatter_staged: cursors, counters, the staging area.
+__host__ __device__ constexpr size_t part_scatter_staged_lds(int np, int rw) {
+  return (size_t)2 * np * 8 + (((size_t)(np + 1) * 4 + 7) & ~(size_t)7) +
+         (size_t)staged_it(rw) * kScatterThreads * rw * 8;
+}
+
 // LDS bytes of k_part_scatter for np partitions and chunks of ch records.
 __host__ __device__ constexpr size_t part_scatter_lds(int np, int ch) {
   return (size_t)2 * np * 8 + (size_t)(np + 1) * 4 + (size_t)ch * 4;
+}
+
+// Pass A workgroup s's flushed partial states into partition order, from
+// the cursors curp[n_parts].
+__device__ __forceinline__ void scatter_partials(PlanRef P, int s, unsigned long long *curp) {
+  const int W = P.words;
+  const DFR_GLOBAL uint64_t *pseg = gp(P.partials) + (uint64_t)s * (uint64_t)P.seg_partials * (1 + W);
+  const uint32_t n = P.cnt_par[s];
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const DFR_GLOBAL uint64_t *rec = pseg + (uint64_t)i * (1 + W);
+    const uint32_t p = part_of(P, rec[0]);
+    const uint64_t pos = atomicAdd(&curp[p], 1ull);
+    DFR_GLOBAL uint64_t *d = gpw(P.part_par) + pos * (1 + W);
+    for (int w = 0; w < 1 + W; ++w) d[w] = rec[w];
+  }
 }
 
 // Workgroup (s, h) = blockIdx.x (s * seg_slices + h) moves the records of
@@ -551,7 +633,7 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const DevPlan *__restrict
   extern __shared__ unsigned long long cur[];  // [2 * n_parts], then cnt and idx
   __shared__ uint32_t wsum[kScatterThreads / 64];
   PlanRef P = plan_ref(plan);
-  const int H = P.seg_slices, s = blockIdx.x / H, h = blockIdx.x % H, NP = P.n_parts, W = P.words;
+  const int H = P.seg_slices, s = blockIdx.x / H, h = blockIdx.x % H, NP = P.n_parts;
   uint32_t *cnt = (uint32_t *)(cur + 2 * NP);
   uint32_t *idx = cnt + NP + 1;
   for (int p = threadIdx.x; p < NP; p += blockDim.x) {
@@ -579,15 +661,40 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const DevPlan *__restrict
       scatter_sorted<kMaxCols + 3, CH, false>(rseg, n, P.rw, NP, bucket, cur, cnt, idx, wsum, dst);
   }
   if (h != 0) return;
-  const DFR_GLOBAL uint64_t *pseg = gp(P.partials) + (uint64_t)s * (uint64_t)P.seg_partials * (1 + W);
-  const uint32_t n = P.cnt_par[s];
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const DFR_GLOBAL uint64_t *rec = pseg + (uint64_t)i * (1 + W);
-    const uint32_t p = part_of(P, rec[0]);
-    const uint64_t pos = atomicAdd(&cur[NP + p], 1ull);
-    DFR_GLOBAL uint64_t *d = gpw(P.part_par) + pos * (1 + W);
-    for (int w = 0; w < 1 + W; ++w) d[w] = rec[w];
+  scatter_partials(P, s, cur + NP);
+}
+
+// k_part_scatter with staged chunks (scatter_staged) for records of 2 to 4
+// words.
+__global__ void __launch_bounds__(1024) k_part_scatter_staged(const DevPlan *__restrict__ plan) {
+  extern __shared__ unsigned long long cur[];  // [2 * n_parts], then cnt and the staging area
+  __shared__ uint32_t wsum[kScatterThreads / 64];
+  PlanRef P = plan_ref(plan);
+  const int H = P.seg_slices, s = blockIdx.x / H, h = blockIdx.x % H, NP = P.n_parts;
+  uint32_t *cnt = (uint32_t *)(cur + 2 * NP);
+  uint64_t *stage = (uint64_t *)((char *)cnt + (((size_t)(NP + 1) * 4 + 7) & ~(size_t)7));
+  for (int p = threadIdx.x; p < NP; p += blockDim.x) {
+    cur[p] = P.base_raw[p] + P.hist_raw[((uint64_t)s * H + h) * NP + p];
+    if (h == 0) cur[NP + p] = P.base_par[p] + P.hist_par[(uint64_t)s * NP + p];
   }
+  __syncthreads();
+  const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
+  auto bucket = [&](uint64_t w0) { return part_of(P, w0 & keymask); };
+  const int per = P.seg_waves / H;
+  DFR_GLOBAL uint64_t *dst = gpw(P.part_raw);
+  for (int w = h * per; w < (h + 1) * per; ++w) {
+    const uint64_t wseg = (uint64_t)s * P.seg_waves + w;
+    const DFR_GLOBAL uint64_t *rseg = gp(P.spill) + wseg * (uint64_t)P.seg_rows * P.rw;
+    const uint32_t n = P.cnt_raw[wseg];
+    if (P.rw == 2)
+      scatter_staged<2, staged_it(2)>(rseg, n, NP, bucket, cur, cnt, stage, wsum, dst);
+    else if (P.rw == 3)
+      scatter_staged<3, staged_it(3)>(rseg, n, NP, bucket, cur, cnt, stage, wsum, dst);
+    else
+      scatter_staged<4, staged_it(4)>(rseg, n, NP, bucket, cur, cnt, stage, wsum, dst);
+  }
+  if (h != 0) return;
+  scatter_partials(P, s, cur + NP);
 }
 #endif  // !DFR_JIT
 
@@ -800,8 +907,16 @@ __host__ __device__ constexpr size_t sub_scatter_lds(int ns) {
   return (size_t)ns * 8 + (size_t)(ns + 1) * 4 + (size_t)kSubScatterChunk * 4;
 }
 
-__global__ void __launch_bounds__(1024) k_sub_scatter(const DevPlan *__restrict__ plan) {
-  extern __shared__ unsigned long long cur2[];  // [n_sub], then cnt and idx
+// LDS bytes of k_sub_scatter staging raw records of rw words (staged_it(rw)
+// records per thread and round) for ns sub-partitions.
+__host__ __device__ constexpr size_t sub_scatter_staged_lds(int ns, int rw) {
+  return (size_t)ns * 8 + (((size_t)(ns + 1) * 4 + 7) & ~(size_t)7) + (size_t)staged_it(rw) * kScatterThreads * rw * 8;
+}
+
+// stage: raw records of 2 to 4 words move through scatter_staged (the LDS
+// then holds sub_scatter_staged_lds bytes); partial states always gather.
+__global__ void __launch_bounds__(1024) k_sub_scatter(const DevPlan *__restrict__ plan, int stage) {
+  extern __shared__ unsigned long long cur2[];  // [n_sub], then cnt and idx (or the staging area)
   __shared__ uint32_t wsum[kScatterThreads / 64];
   PlanRef P = plan_ref(plan);
   const int j = blockIdx.x, p = blockIdx.y, which = blockIdx.z;
@@ -820,6 +935,16 @@ __global__ void __launch_bounds__(1024) k_sub_scatter(const DevPlan *__restrict_
   auto bucket = [&](uint64_t w0) { return sub_of(P, w0 & keymask); };
   const DFR_GLOBAL uint64_t *s0 = src + lo * rw;
   const uint32_t n = (uint32_t)(hi - lo);
+  if (stage && !which) {
+    uint64_t *st = (uint64_t *)((char *)cnt + (((size_t)(NS + 1) * 4 + 7) & ~(size_t)7));
+    if (rw == 2)
+      scatter_staged<2, staged_it(2)>(s0, n, NS, bucket, cur2, cnt, st, wsum, dst);
+    else if (rw == 3)
+      scatter_staged<3, staged_it(3)>(s0, n, NS, bucket, cur2, cnt, st, wsum, dst);
+    else
+      scatter_staged<4, staged_it(4)>(s0, n, NS, bucket, cur2, cnt, st, wsum, dst);
+    return;
+  }
   if (rw == 2)
     scatter_sorted<2, kSubScatterChunk, true>(s0, n, 2, NS, bucket, cur2, cnt, idx, wsum, dst);
   else if (rw == 3)

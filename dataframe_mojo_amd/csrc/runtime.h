@@ -52,6 +52,7 @@ struct Tuning {
   int two_level_parts = 2048;  // first-level partitions of a two-level query (a power of two)
   int scatter_slices = 1;      // pass A workgroup's waves split into this many scatter workgroups
   int scatter_chunk = 16384;   // records a scatter workgroup sorts at once (8192 or 16384)
+  int scatter_stage = 1;       // 1: the scatters stage whole records of 2 to 4 words in LDS
 };
 
 }  // namespace dfr

@@ -137,7 +137,9 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * workgroups count their spilled records in this many histograms, one per
  * run of their waves, and each is moved by its own scatter workgroup),
  * "scatter_chunk" (records a scatter workgroup sorts at once: 8192 or
- * 16384). */
+ * 16384), "scatter_stage" (1, the default: the scatters load records of 2
+ * to 4 words whole, sort them through LDS and stream them out; 0: they sort
+ * indices and gather the records again). */
 int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
 
 /* ---- device columns (the column hand-off) ------------------------------ */

@@ -179,7 +179,7 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
                  "max_parts": 4096, "part_lds_bytes": 131072, "sub_lds_bytes": 49152, "sub_block": 512, "jit": 1,
                  "scan_prefetch": 1, "lds_odd_stride": 0, "part_block": 1024, "scan_block": 1024,
                  "scan_wg_per_cu": 1, "scan_lds_bytes": 159744, "two_level_parts": 2048,
-                 "scatter_slices": 1, "scatter_chunk": 16384}
+                 "scatter_slices": 1, "scatter_chunk": 16384, "scatter_stage": 1}
 
 
 @pytest.mark.parametrize("setting", [
@@ -206,6 +206,8 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
     # the scatter's work split and chunk (slices that must divide 5 waves: one)
     {"scatter_slices": 2}, {"scatter_slices": 16, "scatter_chunk": 8192},
     {"scatter_slices": 4, "scan_block": 320, "two_level": 1},
+    {"scatter_stage": 0}, {"scatter_stage": 0, "two_level": 1, "scatter_slices": 4},
+    {"scatter_stage": 1, "jit": 2, "scan_block": 128, "two_level": 1},
 ])
 def test_table_modes_give_identical_results(setting, gpu_ctx):
     df = random_frame(30000, 5, key_card=400)
@@ -392,6 +394,37 @@ def test_jit_scan_matches_precompiled(ki, fi, gpu_ctx):
         assert np.array_equal(a.is_valid(), b.is_valid()), name
     exp, _ = oracle.collect(lf)
     assert_same(got, exp, True, ("x_min", "x_max"), len(KEYS[ki]))
+
+
+@pytest.mark.parametrize("two_level", [-1, 1])
+@pytest.mark.parametrize("width", [2, 3, 4])
+def test_staged_scatter_record_widths(width, two_level, gpu_ctx):
+    """The staged scatters (records of 2, 3 and 4 words sorted through LDS,
+    at the first level and, with two levels, the second) give the gathering
+    scatters' results bit for bit, and the oracle's. One-wave pass A
+    workgroups make every spill segment span several staging rounds."""
+    from test_gpu_batch import frames_equal_bitwise
+    rng = np.random.default_rng(width)
+    n = 3_000_000
+    df = DataFrame([Series("k", Column.int64(rng.integers(0, 300000, n))),
+                    Series("v", Column.int64(rng.integers(-1000, 1000, n))),
+                    Series("x", Column.float64(rng.normal(size=n), rng.random(n) > 0.1)),
+                    Series("y", Column.int32(rng.integers(-50, 50, n)))])
+    aggs = [col("v").sum(), col("x").sum().alias("x_sum"), col("y").min().alias("y_min")][: width - 1]
+    lf = df.filter(col("v") > -900).group_by("k").agg(aggs)
+    got = []
+    try:
+        gpu_ctx.tune("scan_block", 64)
+        gpu_ctx.tune("two_level", two_level)
+        for stage in (0, 1):
+            gpu_ctx.tune("scatter_stage", stage)
+            got.append(lf.collect(engine="accel"))
+    finally:
+        for k, v in TUNE_DEFAULTS.items():
+            gpu_ctx.tune(k, v)
+    frames_equal_bitwise(got[0], got[1], 1)
+    exp, _ = oracle.collect(lf)
+    assert_same(got[1], exp, False, (), 1)
 
 
 @pytest.mark.parametrize("mo", [True, False])
