@@ -34,9 +34,11 @@ namespace {
 
 struct JitEntry {
   hipModule_t module = nullptr;
-  hipFunction_t fn[3] = {nullptr, nullptr, nullptr};  // JIT_SCAN, JIT_PART, JIT_SUB
+  hipFunction_t fn = nullptr;
   bool failed = false;
 };
+
+const char *const kJitNames[3] = {"dfr_scan_jit", "dfr_part_jit", "dfr_sub_jit"};  // JIT_SCAN, JIT_PART, JIT_SUB
 
 std::mutex g_mu;
 std::unordered_map<std::string, JitEntry> g_cache;
@@ -50,8 +52,9 @@ std::string header_dir() {
   return (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/csrc";
 }
 
-// The kernel source for this plan's shape.
-std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF) {
+// The source of one kernel (JIT_SCAN, JIT_PART or JIT_SUB) for this plan's
+// shape.
+std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF, int which) {
   std::ostringstream o;
   o << "#define DFR_JIT 1\n#include \"kernels_part.h\"\nusing namespace dfr;\n"
     << "__device__ __forceinline__ DevPlan make_plan(const DevPlan *plan) {\n"
@@ -130,20 +133,23 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF) {
   int tile_loads = 0;
   for (int c = 0; c < P.n_cols; ++c) tile_loads += (P.vec_ok ? 1 : 2) + (P.cols[c].valid ? 2 : 0);
   tile_loads *= U;
-  // pass A, pass C (one level), pass C (second level)
-  o << "extern \"C\" __global__ void __launch_bounds__(1024) dfr_scan_jit(const DevPlan *__restrict__ plan, "
-       "int64_t chunk) {\n"
-    << "  const DevPlan L = make_plan(plan);\n"
-    << "  scan_spill_body<" << LDSM << ", " << NC << ", " << U << ", " << (PF ? "true" : "false") << ", "
-    << tile_loads << ">(L, chunk);\n}\n";
-  o << "extern \"C\" __global__ void __launch_bounds__(1024) dfr_part_jit(const DevPlan *__restrict__ plan) {\n"
-    << "  const DevPlan L = make_plan(plan);\n  part_aggregate_body(L);\n}\n";
-  o << "extern \"C\" __global__ void __launch_bounds__(512) dfr_sub_jit(const DevPlan *__restrict__ plan) {\n"
-    << "  const DevPlan L = make_plan(plan);\n  sub_aggregate_body<" << P.n_aggs << ">(L);\n}\n";
+  // pass A, pass C (one level) or pass C (second level)
+  if (which == JIT_SCAN)
+    o << "extern \"C\" __global__ void __launch_bounds__(1024) dfr_scan_jit(const DevPlan *__restrict__ plan, "
+         "int64_t chunk) {\n"
+      << "  const DevPlan L = make_plan(plan);\n"
+      << "  scan_spill_body<" << LDSM << ", " << NC << ", " << U << ", " << (PF ? "true" : "false") << ", "
+      << tile_loads << ">(L, chunk);\n}\n";
+  else if (which == JIT_PART)
+    o << "extern \"C\" __global__ void __launch_bounds__(1024) dfr_part_jit(const DevPlan *__restrict__ plan) {\n"
+      << "  const DevPlan L = make_plan(plan);\n  part_aggregate_body(L);\n}\n";
+  else
+    o << "extern \"C\" __global__ void __launch_bounds__(512) dfr_sub_jit(const DevPlan *__restrict__ plan) {\n"
+      << "  const DevPlan L = make_plan(plan);\n  sub_aggregate_body<" << P.n_aggs << ">(L);\n}\n";
   return o.str();
 }
 
-JitEntry compile(const std::string &src, int device) {
+JitEntry compile(const std::string &src, int device, int which) {
   JitEntry e;
   e.failed = true;
   hiprtcProgram prog;
@@ -164,9 +170,7 @@ JitEntry compile(const std::string &src, int device) {
     std::vector<char> code(code_size);
     if (hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS &&
         hipModuleLoadData(&e.module, code.data()) == hipSuccess &&
-        hipModuleGetFunction(&e.fn[JIT_SCAN], e.module, "dfr_scan_jit") == hipSuccess &&
-        hipModuleGetFunction(&e.fn[JIT_PART], e.module, "dfr_part_jit") == hipSuccess &&
-        hipModuleGetFunction(&e.fn[JIT_SUB], e.module, "dfr_sub_jit") == hipSuccess)
+        hipModuleGetFunction(&e.fn, e.module, kJitNames[which]) == hipSuccess)
       e.failed = false;
   }
   hiprtcDestroyProgram(&prog);
@@ -179,7 +183,7 @@ JitEntry compile(const std::string &src, int device) {
 // shape on the context's device, or null when it cannot be built here (then
 // the precompiled kernel runs).
 hipFunction_t jit_function(dfr_context *ctx, const DevPlan &P, int LDSM, int NC, int U, int which) {
-  const std::string src = shape_source(P, LDSM, NC, U, ctx->tune.scan_prefetch);
+  const std::string src = shape_source(P, LDSM, NC, U, ctx->tune.scan_prefetch, which);
   const std::string key = std::to_string(ctx->device) + "\n" + src;
   std::lock_guard<std::mutex> g(g_mu);
   auto it = g_cache.find(key);
@@ -187,19 +191,20 @@ hipFunction_t jit_function(dfr_context *ctx, const DevPlan &P, int LDSM, int NC,
     hipSetDevice(ctx->device);
     if (const char *dir = getenv("DFR_JIT_DUMP")) {
       // the generated source, for inspection
-      const std::string path = std::string(dir) + "/scan_" + std::to_string(std::hash<std::string>()(src)) + ".hip";
+      const std::string path =
+          std::string(dir) + "/" + kJitNames[which] + "_" + std::to_string(std::hash<std::string>()(src)) + ".hip";
       if (FILE *f = fopen(path.c_str(), "w")) {
         fputs(src.c_str(), f);
         fclose(f);
       }
     }
-    it = g_cache.emplace(key, compile(src, ctx->device)).first;
+    it = g_cache.emplace(key, compile(src, ctx->device, which)).first;
     ctx->obs.jit_compiles++;
   }
-  return it->second.failed ? nullptr : it->second.fn[which];
+  return it->second.failed ? nullptr : it->second.fn;
 }
 
 // The generated source of a plan's pass A (for tests and inspection).
 std::string jit_scan_source(const DevPlan &P, int LDSM, int NC, int U, int PF) {
-  return shape_source(P, LDSM, NC, U, PF);
+  return shape_source(P, LDSM, NC, U, PF, JIT_SCAN);
 }
