@@ -740,11 +740,11 @@ void second_level_layout(dfr_context *ctx, Compiled &C, uint64_t records, int bo
   // within 64 KiB)
   ns = std::min<uint64_t>(std::max<uint64_t>(ns << boost, 2), 8192);
   P.n_sub = (int32_t)ns;
-  // chunks of about 8192 records, at most 64 per partition and 2^26
-  // counters in all
+  // chunks of about 8192 records, at most 64 per partition, and a
+  // partition's (sub-partition, chunk) counters within 64 KiB of LDS
   uint64_t ch = next_pow2(std::max<uint64_t>(1, records / (NP * 8192)));
   ch = std::min<uint64_t>(ch, 64);
-  while (ch > 1 && NP * ns * ch > (1ull << 26)) ch >>= 1;
+  while (ch > 1 && ns * ch > 16384) ch >>= 1;
   P.sub_chunks = (int32_t)ch;
   (void)ctx;
 }
@@ -758,8 +758,9 @@ int launch_second_level(dfr_context *ctx, const Compiled &C, const DevPlan *dpla
   const uint64_t NS = (uint64_t)P.n_sub;
   const uint64_t N2 = (uint64_t)P.n_parts * NS * (uint64_t)P.sub_chunks;
   const dim3 grid2(P.sub_chunks, P.n_parts, 2);
-  if ((st = set_lds(k_sub_hist, NS * 4, error))) return st;
-  hipLaunchKernelGGL(k_sub_hist, grid2, dim3(256), NS * 4, ctx->stream, dplan);
+  const size_t h2_bytes = NS * (uint64_t)P.sub_chunks * 4;
+  if ((st = set_lds(k_sub_hist, h2_bytes, error))) return st;
+  hipLaunchKernelGGL(k_sub_hist, dim3(P.n_parts, 2), dim3(1024), h2_bytes, ctx->stream, dplan);
   const uint64_t nb = (N2 + kScanChunk - 1) / kScanChunk;
   for (int which = 0; which < 2; ++which) {
     uint64_t *bs = which ? C.scan2_par : C.scan2_raw;

@@ -842,33 +842,37 @@ __device__ __forceinline__ void chunk_range(const uint64_t *base, int p, int j, 
   hi = b0 + n * (uint64_t)(j + 1) / (uint64_t)nch;
 }
 
-// Grid (sub_chunks, n_parts, 2): chunk x of partition y counts its records
-// per sub-partition; z = 0 raw records, z = 1 partial states.
-__global__ void __launch_bounds__(256) k_sub_hist(const DevPlan *__restrict__ plan) {
-  extern __shared__ uint32_t h2[];  // [n_sub]
+// Grid (n_parts, 2): partition p's records (y = 0 raw, y = 1 partial
+// states), counted per (sub-partition, chunk) in LDS and written as one
+// contiguous block of the [p][sub][chunk] histogram (n_sub * sub_chunks
+// counters, at most 16384).
+__global__ void __launch_bounds__(1024) k_sub_hist(const DevPlan *__restrict__ plan) {
+  extern __shared__ uint32_t h2[];  // [n_sub][sub_chunks]
   PlanRef P = plan_ref(plan);
-  const int j = blockIdx.x, p = blockIdx.y, which = blockIdx.z;
-  const int NS = P.n_sub;
-  for (int i = threadIdx.x; i < NS; i += blockDim.x) h2[i] = 0;
+  const int p = blockIdx.x, which = blockIdx.y;
+  const int NS = P.n_sub, CH = P.sub_chunks;
+  for (int i = threadIdx.x; i < NS * CH; i += blockDim.x) h2[i] = 0;
   __syncthreads();
   const DFR_GLOBAL uint64_t *recs = gp(which ? P.part_par : P.part_raw);
   const uint64_t rw = which ? 1 + P.words : P.rw;
   const uint64_t keymask = (!which && P.vmask_in_key) ? ((1ull << P.key_bits) - 1) : ~0ull;
-  uint64_t lo, hi;
-  chunk_range(which ? P.base_par : P.base_raw, p, j, P.sub_chunks, lo, hi);
   constexpr int K = 8;  // records per thread at once, loads first
-  for (uint64_t i0 = lo; i0 < hi; i0 += (uint64_t)K * blockDim.x) {
-    uint64_t w0[K];
+  for (int j = 0; j < CH; ++j) {
+    uint64_t lo, hi;
+    chunk_range(which ? P.base_par : P.base_raw, p, j, CH, lo, hi);
+    for (uint64_t i0 = lo; i0 < hi; i0 += (uint64_t)K * blockDim.x) {
+      uint64_t w0[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) w0[k] = recs[min(i0 + (uint64_t)k * blockDim.x + threadIdx.x, hi - 1) * rw];
+      for (int k = 0; k < K; ++k) w0[k] = recs[min(i0 + (uint64_t)k * blockDim.x + threadIdx.x, hi - 1) * rw];
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-      if (i0 + (uint64_t)k * blockDim.x + threadIdx.x < hi) atomicAdd(&h2[sub_of(P, w0[k] & keymask)], 1u);
+      for (int k = 0; k < K; ++k)
+        if (i0 + (uint64_t)k * blockDim.x + threadIdx.x < hi)
+          atomicAdd(&h2[sub_of(P, w0[k] & keymask) * CH + j], 1u);
+    }
   }
   __syncthreads();
-  uint32_t *out = which ? P.hist2_par : P.hist2_raw;
-  for (int i = threadIdx.x; i < NS; i += blockDim.x)
-    out[((uint64_t)p * NS + i) * P.sub_chunks + j] = h2[i];
+  uint32_t *out = (which ? P.hist2_par : P.hist2_raw) + (uint64_t)p * NS * CH;
+  for (int i = threadIdx.x; i < NS * CH; i += blockDim.x) out[i] = h2[i];
 }
 
 // Exclusive scan of n 32-bit counts into 64-bit offsets, out[n] = total:
