@@ -91,7 +91,8 @@ int column_stats(dfr_context *ctx, dfr_column *col, char **error) {
   HIP_TRY(hipMemcpyAsync(d, h, sizeof init, hipMemcpyHostToDevice, ctx->stream));
   DevCol C = col->dev();
   int64_t n = col->length;
-  int blocks = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, (int64_t)ctx->cus * 8);
+  // at least 16 rows a thread, at most two blocks a CU
+  int blocks = (int)std::min<int64_t>((n + 16 * kBlock - 1) / (16 * kBlock), (int64_t)ctx->cus * 2);
   if (blocks > 0) {
     hipLaunchKernelGGL(k_col_stats, dim3(blocks), dim3(kBlock), 0, ctx->stream, C, n, d);
     ctx->obs.launches++;

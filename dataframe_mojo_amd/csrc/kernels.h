@@ -932,7 +932,28 @@ __global__ void k_col_stats(DevCol C, int64_t n, ColStats *out) {
     flags |= (uint32_t)__shfl_xor((unsigned int)flags, d, 64);
     cnt += (uint64_t)__shfl_xor((unsigned long long)cnt, d, 64);
   }
+  // the block's waves combined in LDS: one set of device atomics per block
+  // (same-address device atomics serialize)
+  __shared__ ColStats part[kBlock / 64];
+  const int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    part[wave].imin = lo;
+    part[wave].imax = hi;
+    part[wave].emin = emin;
+    part[wave].emax = emax;
+    part[wave].flags = flags;
+    part[wave].valid = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      lo = min(lo, part[w].imin);
+      hi = max(hi, part[w].imax);
+      emin = min(emin, part[w].emin);
+      emax = max(emax, part[w].emax);
+      flags |= part[w].flags;
+      cnt += part[w].valid;
+    }
     atomicMin((long long *)&out->imin, (long long)lo);
     atomicMax((long long *)&out->imax, (long long)hi);
     atomicMin(&out->emin, emin);
