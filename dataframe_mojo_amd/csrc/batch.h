@@ -202,6 +202,46 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
   }
   const int m = (int)live.size();
   if (m == 0) return DFR_OK;
+  // Large queries whose groups outgrow the batch kernel's LDS table (their
+  // rows would fold into HBM with device atomics) take the partitioned
+  // passes instead, launched after the batch kernel into the same tables;
+  // not under graph capture (a new shape may compile at run time). On
+  // batch_4096 (rows >= 4M, 4096 queries) this takes 172 ms to 140 ms.
+  const bool graph = ctx->tune.batch_graph != 0;
+  std::vector<char> parted(m, 0);
+  uint64_t part_bytes = 0;
+  auto part_sizes = [](const DevPlan &P, uint64_t *w) {
+    const uint64_t S_ = (uint64_t)P.n_segs, NP = (uint64_t)P.n_parts, W = (uint64_t)P.words;
+    w[0] = S_ * (uint64_t)P.seg_waves * (uint64_t)P.seg_rows * (uint64_t)P.rw;  // spill, part_raw
+    w[1] = S_ * (uint64_t)P.seg_partials * (1 + W);                             // partials, part_par
+    w[2] = S_ * (uint64_t)P.seg_slices * NP;                                   // hist_raw (u32)
+    w[3] = S_ * NP;                                                             // hist_par (u32)
+    w[4] = S_ * (uint64_t)P.seg_waves;                                          // cnt_raw (u32)
+    w[5] = S_;                                                                  // cnt_par (u32)
+    w[6] = NP + 1;                                                              // base_raw, base_par
+  };
+  auto bytes_of = [](const uint64_t *w) {
+    auto r = [](uint64_t b) { return (b + 255) / 256 * 256; };
+    return 2 * r(w[0] * 8) + 2 * r(w[1] * 8) + r(w[2] * 4) + r(w[3] * 4) + r(w[4] * 4) + r(w[5] * 4) +
+           2 * r(w[6] * 8);
+  };
+  if (!graph && ctx->tune.batch_part_rows > 0) {
+    for (int j = 0; j < m; ++j) {
+      Compiled &c = C[live[j]];
+      if (c.plan.lds_mode == LDS_DIRECT || c.rows < ctx->tune.batch_part_rows) continue;
+      Compiled t = c;
+      choose_layout(ctx, t, ctx->tune.hash_slots, 0);  // the context's (partitioned) tuning
+      if (t.plan.agg_mode != AGG_PARTITION || t.plan.two_level) continue;
+      // a batch's queries share a few shapes: their runtime-compiled kernels
+      // serve every size (the precompiled ones run these sizes slower)
+      t.jit_any_size = true;
+      uint64_t w[7];
+      part_sizes(t.plan, w);
+      part_bytes += bytes_of(w);
+      c = t;
+      parted[j] = 1;
+    }
+  }
 
   // device layout: tables, occupancy word offsets, outputs, work items
   Scratch S(ctx);
@@ -219,13 +259,14 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
     pad_total += (P.cap + 63) / 64 * 64;
     for (int32_t d : c.key_dtypes) out_bytes += out_col_bytes(d, P.cap);
     for (int32_t d : c.agg_dtypes) out_bytes += out_col_bytes(d, P.cap);
-    lds_max = std::max(lds_max, c.lds_bytes);
+    if (!parted[j]) lds_max = std::max(lds_max, c.lds_bytes);
   }
   woff[m] = occ_words;
   const int block = std::min(512, ctx->tune.block);  // k_batch_aggregate's launch bound
   const int64_t tile = std::max<int64_t>(2 * block, ctx->tune.batch_tile / (2 * block) * (2 * block));
   std::vector<WorkItem> items;
   for (int j = 0; j < m; ++j) {
+    if (parted[j]) continue;
     const int64_t rows = C[live[j]].rows;
     for (int64_t b = 0; b < rows; b += tile) items.push_back({j, 0, b, std::min(rows, b + tile)});
   }
@@ -250,7 +291,34 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
       (st = S.get(std::max<uint64_t>(pad_total, 1), &order, error)) || (st = S.get(1, &next, error)) ||
       (st = S.get(4 * (uint64_t)m, &faults, error)) || (st = S.get(scan_nb + 1, &sums, error)))
     return st;
+  char *partblock = nullptr;
+  if (part_bytes && (st = S.get(part_bytes, &partblock, error))) return st;
   if ((st = ctx->pool.alloc(std::max<uint64_t>(out_bytes, 256), &outblock, error))) return st;
+  // the partitioned queries' spill, partition and histogram buffers
+  {
+    char *pc = partblock;
+    auto carve = [&](uint64_t bytes) {
+      char *p = pc;
+      pc += (bytes + 255) / 256 * 256;
+      return p;
+    };
+    for (int j = 0; j < m; ++j) {
+      if (!parted[j]) continue;
+      DevPlan &P = C[live[j]].plan;
+      uint64_t w[7];
+      part_sizes(P, w);
+      P.spill = (uint64_t *)carve(w[0] * 8);
+      P.part_raw = (uint64_t *)carve(w[0] * 8);
+      P.partials = (uint64_t *)carve(w[1] * 8);
+      P.part_par = (uint64_t *)carve(w[1] * 8);
+      P.hist_raw = (uint32_t *)carve(w[2] * 4);
+      P.hist_par = (uint32_t *)carve(w[3] * 4);
+      P.cnt_raw = (uint32_t *)carve(w[4] * 4);
+      P.cnt_par = (uint32_t *)carve(w[5] * 4);
+      P.base_raw = (uint64_t *)carve(w[6] * 8);
+      P.base_par = (uint64_t *)carve(w[6] * 8);
+    }
+  }
   auto outs = std::make_shared<BatchBlock>();
   outs->ctx = ctx;
   outs->p = outblock;
@@ -332,7 +400,6 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
     drop_all();
     return st;
   }
-  const bool graph = ctx->tune.batch_graph != 0;
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
@@ -351,6 +418,14 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
       const int grid = ctx->cus * std::max(1, ctx->tune.wg_per_cu);
       hipLaunchKernelGGL(kagg, dim3(grid), dim3(block), lds_max, ctx->stream, dplans,
                          ditems, (int64_t)n_items, (unsigned long long *)next);
+    }
+    for (int j = 0; j < m; ++j) {
+      if (!parted[j] || C[live[j]].rows == 0) continue;
+      if ((st = launch_partitioned(ctx, C[live[j]], dplans + j, error))) {
+        hipStreamSynchronize(ctx->stream);
+        drop_all();
+        return st;
+      }
     }
     if (!graph) hipEventRecord(ctx->evp[1], ctx->stream);
     hipLaunchKernelGGL(k_batch_occupancy, dim3(m), dim3(kBlock), 0, ctx->stream, dplans, dwoff, occ);

@@ -303,6 +303,8 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.batch_tile = (int)value;
   else if (k == "batch_graph" && (value == 0 || value == 1))
     t.batch_graph = (int)value;
+  else if (k == "batch_part_rows" && value >= 0)
+    t.batch_part_rows = value;
   else if (k == "scatter_slices" && value >= 1 && value <= 16 && (value & (value - 1)) == 0)
     t.scatter_slices = (int)value;
   else if (k == "scatter_stage" && (value == 0 || value == 1))

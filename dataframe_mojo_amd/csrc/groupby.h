@@ -11,6 +11,7 @@ struct Compiled {
   std::vector<int32_t> key_dtypes, agg_dtypes;
   int64_t rows = 0;
   bool maintain_order = false;
+  bool jit_any_size = false;  // runtime-compiled kernels whatever the row count (tune jit 1)
   // launch shape
   int block = 512;
   int grid = 1;

@@ -68,6 +68,7 @@ Tuning default_tuning(dfr_context *ctx) {
   t.scatter_slices = env_int("DFR_SCATTER_SLICES", 1);
   t.scatter_chunk = env_int("DFR_SCATTER_CHUNK", 16384) == 8192 ? 8192 : 16384;
   t.scatter_stage = env_int("DFR_SCATTER_STAGE", 1) ? 1 : 0;
+  t.batch_part_rows = env_int("DFR_BATCH_PART_ROWS", 1 << 22);
   t.scan_block = env_int("DFR_SCAN_BLOCK", 1024);
   t.scan_wg_per_cu = env_int("DFR_SCAN_WG_PER_CU", 1);
   t.scan_lds_bytes = std::min(env_int("DFR_SCAN_LDS_BYTES", 159744), ctx->lds_per_block);
@@ -638,7 +639,7 @@ void scan_shape(const dfr_context *ctx, const Compiled &C, int *ldsm, int *nc, i
 bool jit_launch(dfr_context *ctx, const Compiled &C, int which, unsigned grid, unsigned block, size_t lds,
                 void **args) {
   const Tuning &T = ctx->tune;
-  if (!(T.jit == 2 || (T.jit == 1 && C.rows >= T.jit_rows))) return false;
+  if (!(T.jit == 2 || (T.jit == 1 && (C.rows >= T.jit_rows || C.jit_any_size)))) return false;
   int ldsm, nc, u;
   scan_shape(ctx, C, &ldsm, &nc, &u);
   hipFunction_t fn = jit_function(ctx, C.plan, ldsm, nc, u, which);

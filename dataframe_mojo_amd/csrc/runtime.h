@@ -44,6 +44,8 @@ struct Tuning {
   int two_level = -1;          // second partition level: -1 when groups may outgrow one level, 0 never, 1 always
   int batch_tile = 131072;     // rows of a batch work item
   int batch_graph = 0;         // 1: a batch's launch sequence runs as one captured HIP graph
+  int64_t batch_part_rows = 1 << 22;  // a batch query of this many rows whose groups outgrow the batch
+                                      // kernel's LDS table takes the partitioned passes (0: never)
   int jit = 1;                 // pass A compiled for the query's shape: 0 never, 1 from jit_rows rows, 2 always
   int64_t jit_rows = 1 << 24;  // rows from which jit = 1 compiles
   int scan_prefetch = 1;       // runtime-compiled pass A: 1 overlaps the next tile's loads with this tile's fold

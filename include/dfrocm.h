@@ -124,7 +124,10 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * power of two up to 4096), "sub_lds_bytes" and "sub_block" (LDS bytes and
  * threads of a second-level partition table's workgroup), "batch_tile"
  * (rows of a batch work item), "batch_graph" (1: a batch's launch sequence
- * is captured and run as one HIP graph), "jit" (pass A compiled at run time
+ * is captured and run as one HIP graph), "batch_part_rows" (a batch query
+ * with at least this many rows whose key domain outgrows the batch kernel's
+ * LDS table runs the partitioned passes instead, after the batch kernel;
+ * 0: never; not under graph capture), "jit" (pass A compiled at run time
  * for the query's shape: 0 never, 1 when the input has at least "jit_rows"
  * rows, 2 always; the precompiled kernel runs when it cannot be built),
  * "jit_rows", "scan_prefetch" (1: the runtime-compiled pass A issues the

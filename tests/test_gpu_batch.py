@@ -72,7 +72,9 @@ def frames_equal_bitwise(a, b, n_keys):
         assert np.array_equal(va[ma].view(np.uint8), vb[mb].view(np.uint8)), name
 
 
-@pytest.mark.parametrize("setting", [{}, {"batch_graph": 1}, {"batch_tile": 2048}, {"block": 256, "lds_bytes": 16384}])
+@pytest.mark.parametrize("setting", [{}, {"batch_graph": 1}, {"batch_tile": 2048}, {"block": 256, "lds_bytes": 16384},
+                                     {"batch_part_rows": 0}, {"batch_part_rows": 1000},
+                                     {"batch_part_rows": 1000, "block": 256, "lds_bytes": 16384}])
 def test_batch_equals_one_launch_per_query(setting, gpu_ctx):
     jobs, lfs = make_jobs(gpu_ctx, 40, 1)
     try:
@@ -80,7 +82,7 @@ def test_batch_equals_one_launch_per_query(setting, gpu_ctx):
             gpu_ctx.tune(k, v)
         batch = gpu_ctx.groupby_batch(jobs)
     finally:
-        for k, v in {**TUNE_DEFAULTS, "batch_graph": 0, "batch_tile": 131072}.items():
+        for k, v in {**TUNE_DEFAULTS, "batch_graph": 0, "batch_tile": 131072, "batch_part_rows": 1 << 22}.items():
             gpu_ctx.tune(k, v)
     for (plan, keys, cols), got in zip(jobs, batch):
         assert not isinstance(got, Exception), got
