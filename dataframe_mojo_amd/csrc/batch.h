@@ -232,6 +232,14 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
       Compiled t = c;
       choose_layout(ctx, t, ctx->tune.hash_slots, 0);  // the context's (partitioned) tuning
       if (t.plan.agg_mode != AGG_PARTITION || t.plan.two_level) continue;
+      if (!ctx->tune.jit_wait) {
+        // no compile waits (a serving lane): routed only once the shape's
+        // kernels exist, since the precompiled ones lose to the batch kernel
+        int ldsm, nc, u;
+        scan_shape(ctx, t, &ldsm, &nc, &u);
+        if (!jit_compiled(ctx, t.plan, ldsm, nc, u, JIT_SCAN) || !jit_compiled(ctx, t.plan, ldsm, nc, u, JIT_PART))
+          continue;
+      }
       // a batch's queries share a few shapes: their runtime-compiled kernels
       // serve every size (the precompiled ones run these sizes slower)
       t.jit_any_size = true;

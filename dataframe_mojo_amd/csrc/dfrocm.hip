@@ -299,6 +299,10 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.sub_lds_bytes = (int)value;
   else if (k == "sub_block" && value >= 64 && value <= 512 && value % 64 == 0)
     t.sub_block = (int)value;
+  else if (k == "sub_fill" && value >= 10 && value <= 90)
+    t.sub_fill = (int)value;
+  else if (k == "sub_wg_per_cu" && value >= -1 && value <= 64)
+    t.sub_wg_per_cu = (int)value;
   else if (k == "batch_tile" && value >= 2048 && value <= (1 << 30))
     t.batch_tile = (int)value;
   else if (k == "batch_graph" && (value == 0 || value == 1))
@@ -323,6 +327,8 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.jit = (int)value;
   else if (k == "jit_rows" && value >= 0)
     t.jit_rows = value;
+  else if (k == "jit_wait" && (value == 0 || value == 1))
+    t.jit_wait = (int)value;
   else
     return dfr_fail(error, DFR_INVALID, "bad tuning %s=%lld", key, (long long)value);
   return DFR_OK;

@@ -32,3 +32,4 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error);
 // pass C at one or two partition levels; null when they cannot be built.
 enum { JIT_SCAN = 0, JIT_PART = 1, JIT_SUB = 2 };
 hipFunction_t jit_function(dfr_context *ctx, const dfr::DevPlan &P, int LDSM, int NC, int U, int which);
+bool jit_compiled(dfr_context *ctx, const dfr::DevPlan &P, int LDSM, int NC, int U, int which);

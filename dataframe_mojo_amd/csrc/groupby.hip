@@ -60,6 +60,8 @@ Tuning default_tuning(dfr_context *ctx) {
   t.two_level = env_int("DFR_TWO_LEVEL", -1);
   t.sub_lds_bytes = std::min(env_int("DFR_SUB_LDS_BYTES", 49152), ctx->lds_per_block - 2048);
   t.sub_block = std::max(64, std::min(512, env_int("DFR_SUB_BLOCK", 512) / 64 * 64));
+  t.sub_wg_per_cu = std::max(-1, env_int("DFR_SUB_WG_PER_CU", -1));
+  t.sub_fill = std::max(10, std::min(90, env_int("DFR_SUB_FILL", 50)));
   t.jit = env_int("DFR_JIT", 1);
   t.scan_prefetch = env_int("DFR_SCAN_PREFETCH", 1) ? 1 : 0;
   t.lds_odd_stride = env_int("DFR_LDS_ODD_STRIDE", 0) ? 1 : 0;
@@ -636,8 +638,19 @@ void scan_shape(const dfr_context *ctx, const Compiled &C, int *ldsm, int *nc, i
 // Launches the kernel `which` compiled at run time for the query's shape;
 // false when the tuning leaves it off or it cannot run here (the caller then
 // launches the precompiled kernel).
+// Workgroups of a loop over `items` work items: one per item with tune
+// per_cu 0, else per_cu per CU (-1: as many as fit on a CU at once, from the
+// kernel's occupancy `occ`), at most one per item.
+unsigned persistent_grid(const dfr_context *ctx, int per_cu, int occ, uint64_t items) {
+  if (per_cu == 0) return (unsigned)items;
+  const uint64_t k = per_cu > 0 ? (uint64_t)per_cu : (uint64_t)std::max(1, occ);
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(items, k * (uint64_t)ctx->cus));
+}
+
+// grid 0: a persistent loop over `items` work items (persistent_grid with
+// tune sub_wg_per_cu).
 bool jit_launch(dfr_context *ctx, const Compiled &C, int which, unsigned grid, unsigned block, size_t lds,
-                void **args) {
+                void **args, uint64_t items = 0) {
   const Tuning &T = ctx->tune;
   if (!(T.jit == 2 || (T.jit == 1 && (C.rows >= T.jit_rows || C.jit_any_size)))) return false;
   int ldsm, nc, u;
@@ -648,6 +661,14 @@ bool jit_launch(dfr_context *ctx, const Compiled &C, int which, unsigned grid, u
       hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
     (void)hipGetLastError();
     return false;
+  }
+  if (grid == 0) {
+    int occ = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, (int)block, lds) != hipSuccess) {
+      (void)hipGetLastError();
+      occ = 1;
+    }
+    grid = persistent_grid(ctx, T.sub_wg_per_cu, occ, items);
   }
   if (hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, (unsigned)lds, ctx->stream, args, nullptr) != hipSuccess) {
     (void)hipGetLastError();
@@ -731,12 +752,13 @@ int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan
   return DFR_OK;
 }
 
-// Sub-partitions for `records` first-level records (raw + partial): about
-// half a table's entries each on average, twice as many per retry (boost).
+// Sub-partitions for `records` first-level records (raw + partial): on
+// average sub_fill percent of a table's entries each, twice as many
+// sub-partitions per retry (boost).
 void second_level_layout(dfr_context *ctx, Compiled &C, uint64_t records, int boost) {
   DevPlan &P = C.plan;
   const uint64_t NP = (uint64_t)P.n_parts;
-  const uint64_t per_sub = std::max<uint64_t>(1, (uint64_t)P.c_entries / 2);
+  const uint64_t per_sub = std::max<uint64_t>(1, (uint64_t)P.c_entries * (uint64_t)ctx->tune.sub_fill / 100);
   uint64_t ns = (std::max<uint64_t>(records, 1) + NP * per_sub - 1) / (NP * per_sub);
   // (the second level's LDS cursors, 8 bytes per sub-partition, stay
   // within 64 KiB)
@@ -748,7 +770,6 @@ void second_level_layout(dfr_context *ctx, Compiled &C, uint64_t records, int bo
   ch = std::min<uint64_t>(ch, 64);
   while (ch > 1 && ns * ch > 16384) ch >>= 1;
   P.sub_chunks = (int32_t)ch;
-  (void)ctx;
 }
 
 // The second level after the first scatter: per-chunk sub-partition counts,
@@ -779,11 +800,17 @@ int launch_second_level(dfr_context *ctx, const Compiled &C, const DevPlan *dpla
   if ((st = set_lds(k_sub_scatter, sc_bytes, error))) return st;
   hipLaunchKernelGGL(k_sub_scatter, grid2, dim3(kScatterThreads), sc_bytes, ctx->stream, dplan, stage);
   void *args[] = {(void *)&dplan};
-  const unsigned n_sub_parts = (unsigned)((uint64_t)P.n_parts * NS);
-  if (!jit_launch(ctx, C, JIT_SUB, n_sub_parts, ctx->tune.sub_block, C.c_lds_bytes, args)) {
+  const uint64_t n_sub_parts = (uint64_t)P.n_parts * NS;
+  if (!jit_launch(ctx, C, JIT_SUB, 0, ctx->tune.sub_block, C.c_lds_bytes, args, n_sub_parts)) {
     if ((st = set_lds(k_sub_aggregate, C.c_lds_bytes, error))) return st;
-    hipLaunchKernelGGL(k_sub_aggregate, dim3(n_sub_parts), dim3(ctx->tune.sub_block), C.c_lds_bytes, ctx->stream,
-                       dplan);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sub_aggregate, ctx->tune.sub_block, C.c_lds_bytes) !=
+        hipSuccess) {
+      (void)hipGetLastError();
+      occ = 1;
+    }
+    const unsigned grid = persistent_grid(ctx, ctx->tune.sub_wg_per_cu, occ, n_sub_parts);
+    hipLaunchKernelGGL(k_sub_aggregate, dim3(grid), dim3(ctx->tune.sub_block), C.c_lds_bytes, ctx->stream, dplan);
   }
   ctx->obs.launches += 9;
   HIP_TRY(hipGetLastError());

@@ -144,7 +144,8 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF, int 
     o << "extern \"C\" __global__ void __launch_bounds__(1024) dfr_part_jit(const DevPlan *__restrict__ plan) {\n"
       << "  const DevPlan L = make_plan(plan);\n  part_aggregate_body(L);\n}\n";
   else
-    o << "extern \"C\" __global__ void __launch_bounds__(512) dfr_sub_jit(const DevPlan *__restrict__ plan) {\n"
+    o << "extern \"C\" __global__ void __launch_bounds__(512) DFR_SUB_WAVES dfr_sub_jit(const DevPlan *__restrict__ "
+         "plan) {\n"
       << "  const DevPlan L = make_plan(plan);\n  sub_aggregate_body<" << P.n_aggs << ">(L);\n}\n";
   return o.str();
 }
@@ -181,13 +182,15 @@ JitEntry compile(const std::string &src, int device, int which) {
 
 // A runtime-compiled kernel (JIT_SCAN, JIT_PART, JIT_SUB) for this plan's
 // shape on the context's device, or null when it cannot be built here (then
-// the precompiled kernel runs).
+// the precompiled kernel runs). With tune jit_wait 0, only a kernel already
+// compiled is returned: a new shape is not compiled (that takes seconds).
 hipFunction_t jit_function(dfr_context *ctx, const DevPlan &P, int LDSM, int NC, int U, int which) {
   const std::string src = shape_source(P, LDSM, NC, U, ctx->tune.scan_prefetch, which);
   const std::string key = std::to_string(ctx->device) + "\n" + src;
   std::lock_guard<std::mutex> g(g_mu);
   auto it = g_cache.find(key);
   if (it == g_cache.end()) {
+    if (!ctx->tune.jit_wait) return nullptr;
     hipSetDevice(ctx->device);
     if (const char *dir = getenv("DFR_JIT_DUMP")) {
       // the generated source, for inspection
@@ -202,6 +205,15 @@ hipFunction_t jit_function(dfr_context *ctx, const DevPlan &P, int LDSM, int NC,
     ctx->obs.jit_compiles++;
   }
   return it->second.failed ? nullptr : it->second.fn;
+}
+
+// Whether jit_function has a compiled kernel for this shape already.
+bool jit_compiled(dfr_context *ctx, const DevPlan &P, int LDSM, int NC, int U, int which) {
+  const std::string key =
+      std::to_string(ctx->device) + "\n" + shape_source(P, LDSM, NC, U, ctx->tune.scan_prefetch, which);
+  std::lock_guard<std::mutex> g(g_mu);
+  auto it = g_cache.find(key);
+  return it != g_cache.end() && !it->second.failed;
 }
 
 // The generated source of a plan's pass A (for tests and inspection).

@@ -717,53 +717,76 @@ __device__ __forceinline__ int part_find(const PV &P, uint64_t *lkeys, uint64_t 
   return -1;
 }
 
+// U records per thread of a pass C tile: their words (those past rw
+// unused) and whether each lies in the range.
+template <int RW, int U>
+struct RecTile {
+  uint64_t w[U][RW];
+  bool ok[U];
+};
+
+// Loads records i0 + u * blockDim.x + threadIdx.x (u < U) of a range that
+// ends at b1 > i0. Past the range, the range's last record is loaded (and
+// not folded): every load is unconditional, so all are in flight together.
+template <int RW, int U, class PV>
+__device__ __forceinline__ void load_recs(const PV &P, RecTile<RW, U> &t, const DFR_GLOBAL uint64_t *recs,
+                                          uint64_t i0, uint64_t b1) {
+  const int rw = P.rw;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint64_t i = i0 + (uint64_t)u * blockDim.x + threadIdx.x;
+    t.ok[u] = i < b1;
+    const uint64_t ic = min(i, b1 - 1);
+#pragma unroll
+    for (int k = 0; k < RW; ++k) t.w[u][k] = k < rw ? recs[ic * rw + k] : 0;
+  }
+}
+
+// Folds a loaded tile into the LDS table. A key the LDS table cannot place
+// goes to the HBM table, or, with DENSE (no HBM table), raises the
+// sub-partition capacity fault.
+template <int RW, int U, bool DENSE, class PV>
+__device__ __forceinline__ void fold_recs(const PV &P, uint64_t *lkeys, uint64_t *lst, uint64_t keymask,
+                                          const RecTile<RW, U> &t) {
+  const int W = P.words;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (!t.ok[u]) continue;
+    const uint64_t *rec = t.w[u];
+    // record words picked with selects (uniform indices, no scratch)
+    auto pick = [&](int k) {
+      uint64_t v = 0;
+#pragma unroll
+      for (int j = 0; j < RW; ++j)
+        if (j == k) v = rec[j];
+      return v;
+    };
+    const uint64_t key = rec[0] & keymask;
+    const uint64_t vmask = P.vmask_in_key ? (rec[0] >> P.key_bits) : (P.rec_meta >= 0 ? pick(P.rec_meta) : ~0ull);
+    const int64_t row = P.rec_row >= 0 ? (int64_t)pick(P.rec_row) : 0;
+    auto val = [&](int c) { return pick(P.rec_val0 + P.val_of[c]); };
+    auto ok = [&](int c) { return ((vmask >> P.val_of[c]) & 1ull) != 0; };
+    const int e = part_find(P, lkeys, key);
+    if (e >= 0) {
+      fold_full(P, lst + (int64_t)e * P.lds_stride, row, val, ok);
+    } else if (DENSE) {
+      atomicOr(&P.fault[3], 1u);
+    } else {
+      const uint64_t g = global_slot(P, key);
+      if (g != kEmpty) fold_full(P, P.state + g * (uint64_t)W, row, val, ok);
+    }
+  }
+}
+
 // Folds the raw records [b0, b1) of one partition, U per thread at once
-// (loads first). A key the LDS table cannot place goes to the HBM table,
-// or, with DENSE (no HBM table), raises the sub-partition capacity fault.
+// (loads first).
 template <int RW, int U, bool DENSE, class PV>
 __device__ __forceinline__ void part_fold_records(const PV &P, uint64_t *lkeys, uint64_t *lst, uint64_t keymask,
                                                   const DFR_GLOBAL uint64_t *recs, uint64_t b0, uint64_t b1) {
-  const int W = P.words, rw = P.rw;
   for (uint64_t i0 = b0; i0 < b1; i0 += (uint64_t)U * blockDim.x) {
-    uint64_t w[U][RW];
-    bool okr[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      // past the range, the range's last record is loaded (and not folded):
-      // every load is unconditional, so all are in flight together
-      const uint64_t i = i0 + (uint64_t)u * blockDim.x + threadIdx.x;
-      okr[u] = i < b1;
-      const uint64_t ic = min(i, b1 - 1);
-#pragma unroll
-      for (int k = 0; k < RW; ++k) w[u][k] = k < rw ? recs[ic * rw + k] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (!okr[u]) continue;
-      const uint64_t *rec = w[u];
-      // record words picked with selects (uniform indices, no scratch)
-      auto pick = [&](int k) {
-        uint64_t v = 0;
-#pragma unroll
-        for (int j = 0; j < RW; ++j)
-          if (j == k) v = rec[j];
-        return v;
-      };
-      const uint64_t key = rec[0] & keymask;
-      const uint64_t vmask = P.vmask_in_key ? (rec[0] >> P.key_bits) : (P.rec_meta >= 0 ? pick(P.rec_meta) : ~0ull);
-      const int64_t row = P.rec_row >= 0 ? (int64_t)pick(P.rec_row) : 0;
-      auto val = [&](int c) { return pick(P.rec_val0 + P.val_of[c]); };
-      auto ok = [&](int c) { return ((vmask >> P.val_of[c]) & 1ull) != 0; };
-      const int e = part_find(P, lkeys, key);
-      if (e >= 0) {
-        fold_full(P, lst + (int64_t)e * P.lds_stride, row, val, ok);
-      } else if (DENSE) {
-        atomicOr(&P.fault[3], 1u);
-      } else {
-        const uint64_t g = global_slot(P, key);
-        if (g != kEmpty) fold_full(P, P.state + g * (uint64_t)W, row, val, ok);
-      }
-    }
+    RecTile<RW, U> t;
+    load_recs<RW, U>(P, t, recs, i0, b1);
+    fold_recs<RW, U, DENSE>(P, lkeys, lst, keymask, t);
   }
 }
 
@@ -1007,88 +1030,156 @@ __device__ __forceinline__ void emit_group(const PV &P, uint64_t i, const uint64
   }
 }
 
-// Pass C over sub-partition x = p * n_sub + sub: folds its raw
-// records and merges its partial states in an LDS table, then gives each
-// group the next free slot of the dense HBM tables (one device atomic per
-// workgroup) and stores its key and state there.
-template <int NA = 0, class PV>
-__device__ __forceinline__ void sub_aggregate_body(const PV &P) {
+// Pass C over sub-partitions x = p * n_sub + sub, from blockIdx.x on in
+// steps of gridDim.x: folds the sub-partition's raw records and merges its
+// partial states in an LDS table, then gives each group the next free slot
+// of the dense HBM tables (one device atomic per sub-partition) and stores
+// its key and state there, or, fused, its output row. The first tile of the
+// next sub-partition is loaded while this one's groups are placed.
+template <int NA, int RW, int U, class PV>
+__device__ __forceinline__ void sub_aggregate_loop(const PV &P, uint32_t *wsum, uint64_t &gbase) {
   extern __shared__ uint64_t smem[];
-  __shared__ uint32_t wsum[kMaxPartEntries / 64];
-  __shared__ uint64_t gbase;
-  const uint64_t sp = blockIdx.x;
   const int E = P.c_entries, W = P.words, S = P.lds_stride, CH = P.sub_chunks;
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + E;
-  for (int i = threadIdx.x; i < E; i += blockDim.x) {
-    lkeys[i] = kEmpty;
-    if constexpr (RegPlan<PV>::value) {
-#pragma unroll
-      for (int w = 0; w < kMaxWords; ++w)
-        if (w < W) lst[i * S + w] = P.ident[w];
-    } else {
-      for (int w = 0; w < W; ++w) lst[i * S + w] = P.ident[w];
-    }
-  }
-  __syncthreads();
   const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;
   const DFR_GLOBAL uint64_t *recs = gp(P.part2_raw);
-  const uint64_t b0 = P.base2_raw[sp * CH], b1 = P.base2_raw[(sp + 1) * CH];
-  if (P.rw <= 4)
-    part_fold_records<4, 4, true>(P, lkeys, lst, keymask, recs, b0, b1);
-  else if (P.rw <= 8)
-    part_fold_records<8, 2, true>(P, lkeys, lst, keymask, recs, b0, b1);
-  else
-    part_fold_records<kMaxCols + 3, 1, true>(P, lkeys, lst, keymask, recs, b0, b1);
-  const uint64_t q0 = P.base2_par[sp * CH], q1 = P.base2_par[(sp + 1) * CH];
-  for (uint64_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
-    const DFR_GLOBAL uint64_t *rec = gp(P.part2_par) + i * (1 + W);
-    const int e = part_find(P, lkeys, rec[0]);
-    if (e >= 0)
-      merge_full(P, lst + (int64_t)e * S, rec + 1);
-    else
-      atomicOr(&P.fault[3], 1u);
-  }
-  __syncthreads();
-  // dense slots: per 64-entry chunk its occupied count, a prefix over the
-  // chunks, one reservation for the workgroup
+  const uint64_t nsp = (uint64_t)P.n_parts * (uint64_t)P.n_sub;
+  const uint64_t tile = (uint64_t)U * blockDim.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   const int nchunks = E >> 6;
-  for (int c = wave; c < nchunks; c += nwaves) {
-    const uint64_t ball = __ballot(lkeys[c * 64 + lane] != kEmpty);
-    if (lane == 0) wsum[c] = (uint32_t)__popcll(ball);
+  RecTile<RW, U> t;
+  // the sub-partition's raw and partial ranges, read (and its first tile
+  // loaded) one sub-partition ahead
+  uint64_t b0 = 0, b1 = 0, q0 = 0, q1 = 0;
+  uint64_t sp = blockIdx.x;
+  if (sp < nsp) {
+    b0 = P.base2_raw[sp * CH];
+    b1 = P.base2_raw[(sp + 1) * CH];
+    q0 = P.base2_par[sp * CH];
+    q1 = P.base2_par[(sp + 1) * CH];
+    if (b0 < b1) load_recs<RW, U>(P, t, recs, b0, b1);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (int c = 0; c < nchunks; ++c) {
-      const uint32_t x = wsum[c];
-      wsum[c] = run;
-      run += x;
+  // (every iteration starts with its tile in registers: no fold waits for
+  // the loads issued at the loop's top)
+  __builtin_amdgcn_s_waitcnt(0xF70);
+  for (; sp < nsp; sp += gridDim.x) {
+    // the next sub-partition's ranges, in flight during this one's fold
+    const uint64_t nx = sp + gridDim.x;
+    uint64_t nb0 = 0, nb1 = 0, nq0 = 0, nq1 = 0;
+    if (nx < nsp) {
+      nb0 = P.base2_raw[nx * CH];
+      nb1 = P.base2_raw[(nx + 1) * CH];
+      nq0 = P.base2_par[nx * CH];
+      nq1 = P.base2_par[(nx + 1) * CH];
     }
-    gbase = run ? atomicAdd((unsigned long long *)P.gcount, (unsigned long long)run) : 0ull;
-  }
-  __syncthreads();
-  const uint64_t lt = (1ull << lane) - 1ull;
-  for (int c = wave; c < nchunks; c += nwaves) {
-    const int e = c * 64 + lane;
-    const uint64_t key = lkeys[e];
-    const uint64_t ball = __ballot(key != kEmpty);
-    if (key == kEmpty) continue;
-    const uint64_t slot = gbase + wsum[c] + (uint64_t)__popcll(ball & lt);
-    if (P.fused) {
-      emit_group<NA>(P, slot, lst + (int64_t)e * S, key);
-      continue;
+    for (int i = threadIdx.x; i < E; i += blockDim.x) {
+      lkeys[i] = kEmpty;
+      if constexpr (RegPlan<PV>::value) {
+#pragma unroll
+        for (int w = 0; w < kMaxWords; ++w)
+          if (w < W) lst[i * S + w] = P.ident[w];
+      } else {
+        for (int w = 0; w < W; ++w) lst[i * S + w] = P.ident[w];
+      }
     }
-    P.keys_tab[slot] = key;
-    uint64_t *d = P.state + slot * (uint64_t)W;
-    const uint64_t *s = lst + (int64_t)e * S;
-    for (int w = 0; w < W; ++w) d[w] = s[w];
+    __syncthreads();
+    if (b0 < b1) {
+      fold_recs<RW, U, true>(P, lkeys, lst, keymask, t);
+      if (b0 + tile < b1) part_fold_records<RW, U, true>(P, lkeys, lst, keymask, recs, b0 + tile, b1);
+    }
+    for (uint64_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
+      const DFR_GLOBAL uint64_t *rec = gp(P.part2_par) + i * (1 + W);
+      const int e = part_find(P, lkeys, rec[0]);
+      if (e >= 0)
+        merge_full(P, lst + (int64_t)e * S, rec + 1);
+      else
+        atomicOr(&P.fault[3], 1u);
+    }
+    // and its first tile, in flight while this one's groups are placed
+    if (nb0 < nb1) load_recs<RW, U>(P, t, recs, nb0, nb1);
+    b0 = nb0;
+    b1 = nb1;
+    q0 = nq0;
+    q1 = nq1;
+    __syncthreads();
+    // dense slots: per 64-entry chunk its occupied count, a prefix over the
+    // chunks (wave 0), one reservation for the sub-partition
+    for (int c = wave; c < nchunks; c += nwaves) {
+      const uint64_t ball = __ballot(lkeys[c * 64 + lane] != kEmpty);
+      if (lane == 0) wsum[c] = (uint32_t)__popcll(ball);
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const int per = (nchunks + 63) >> 6;
+      uint32_t own = 0;
+      for (int j = 0; j < per; ++j) {
+        const int c = lane * per + j;
+        if (c < nchunks) own += wsum[c];
+      }
+      uint32_t inc = own;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
+      }
+      uint32_t run = inc - own;
+      for (int j = 0; j < per; ++j) {
+        const int c = lane * per + j;
+        if (c < nchunks) {
+          const uint32_t x = wsum[c];
+          wsum[c] = run;
+          run += x;
+        }
+      }
+      const uint32_t total = __shfl(inc, 63, 64);
+      if (lane == 0) gbase = total ? atomicAdd((unsigned long long *)P.gcount, (unsigned long long)total) : 0ull;
+    }
+    __syncthreads();
+    // the next tile has arrived by now: waiting for it here, before this
+    // sub-partition's stores, keeps the next fold from waiting for them
+    // (loads and stores share one in-order counter: gfx9 s_waitcnt
+    // vmcnt(0), expcnt and lgkmcnt not waited)
+    __builtin_amdgcn_s_waitcnt(0xF70);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int c = wave; c < nchunks; c += nwaves) {
+      const int e = c * 64 + lane;
+      const uint64_t key = lkeys[e];
+      const uint64_t ball = __ballot(key != kEmpty);
+      if (key == kEmpty) continue;
+      const uint64_t slot = gbase + wsum[c] + (uint64_t)__popcll(ball & lt);
+      if (P.fused) {
+        emit_group<NA>(P, slot, lst + (int64_t)e * S, key);
+        continue;
+      }
+      P.keys_tab[slot] = key;
+      uint64_t *d = P.state + slot * (uint64_t)W;
+      const uint64_t *s = lst + (int64_t)e * S;
+      for (int w = 0; w < W; ++w) d[w] = s[w];
+    }
+    __syncthreads();  // (the table is set up again for the next sub-partition)
   }
 }
 
+// The second-level pass C's registers, held for three 512-thread
+// workgroups per CU (what their default 48 KiB LDS tables allow): its loop
+// keeps the next sub-partition's first records in registers.
+#define DFR_SUB_WAVES __attribute__((amdgpu_waves_per_eu(6, 8)))
+
+template <int NA = 0, class PV>
+__device__ __forceinline__ void sub_aggregate_body(const PV &P) {
+  __shared__ uint32_t wsum[kMaxPartEntries / 64];
+  __shared__ uint64_t gbase;
+  if (P.rw <= 4)
+    sub_aggregate_loop<NA, 4, 2>(P, wsum, gbase);
+  else if (P.rw <= 8)
+    sub_aggregate_loop<NA, 8, 1>(P, wsum, gbase);
+  else
+    sub_aggregate_loop<NA, kMaxCols + 3, 1>(P, wsum, gbase);
+}
+
 #if !DFR_JIT
-__global__ void __launch_bounds__(512) k_sub_aggregate(const DevPlan *__restrict__ plan) {
+__global__ void __launch_bounds__(512) DFR_SUB_WAVES k_sub_aggregate(const DevPlan *__restrict__ plan) {
   sub_aggregate_body(plan_ref(plan));
 }
 

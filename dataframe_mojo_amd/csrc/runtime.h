@@ -41,6 +41,8 @@ struct Tuning {
   int max_parts = 4096;        // first-level key partitions, at most (a power of two)
   int sub_lds_bytes = 49152;   // LDS bytes of a second-level pass C table
   int sub_block = 512;         // threads of a second-level pass C workgroup
+  int sub_fill = 50;           // records per sub-partition, in percent of its table's entries
+  int sub_wg_per_cu = -1;      // its workgroups per CU (-1: as many as fit at once; 0: one per sub-partition)
   int two_level = -1;          // second partition level: -1 when groups may outgrow one level, 0 never, 1 always
   int batch_tile = 131072;     // rows of a batch work item
   int batch_graph = 0;         // 1: a batch's launch sequence runs as one captured HIP graph
@@ -48,6 +50,7 @@ struct Tuning {
                                       // kernel's LDS table takes the partitioned passes (0: never)
   int jit = 1;                 // pass A compiled for the query's shape: 0 never, 1 from jit_rows rows, 2 always
   int64_t jit_rows = 1 << 24;  // rows from which jit = 1 compiles
+  int jit_wait = 1;            // 0: a runtime-compiled kernel is used only once compiled (no compile waits)
   int scan_prefetch = 1;       // runtime-compiled pass A: 1 overlaps the next tile's loads with this tile's fold
   int lds_odd_stride = 0;      // partitioned path: LDS table entries at an odd number of words
   int part_block = 1024;       // threads of a first-level pass C workgroup

@@ -197,6 +197,7 @@ extern "C" int dfr_server_create(dfr_context *ctx, int32_t lanes, int32_t max_ba
       return st;
     }
     c->tune = ctx->tune;
+    c->tune.jit_wait = 0;  // a query never waits for a runtime compile (seconds)
     s->lanes.push_back(c);
   }
   for (int i = 0; i < lanes; ++i) s->workers.emplace_back([s, i] { s->worker(i); });

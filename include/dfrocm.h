@@ -122,7 +122,12 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * partition level when the groups may outgrow one level's LDS tables; 0
  * never; 1 always), "max_parts" (first-level key partitions at most, a
  * power of two up to 4096), "sub_lds_bytes" and "sub_block" (LDS bytes and
- * threads of a second-level partition table's workgroup), "batch_tile"
+ * threads of a second-level partition table's workgroup), "sub_fill"
+ * (records per sub-partition on average, in percent of its table's entries,
+ * 10 to 90; a sub-partition whose groups outgrow its table doubles the
+ * sub-partitions and runs the second level again), "sub_wg_per_cu"
+ * (those workgroups per CU, each looping over sub-partitions: -1 as many as
+ * fit on a CU at once, 0 one workgroup per sub-partition), "batch_tile"
  * (rows of a batch work item), "batch_graph" (1: a batch's launch sequence
  * is captured and run as one HIP graph), "batch_part_rows" (a batch query
  * with at least this many rows whose key domain outgrows the batch kernel's
@@ -130,7 +135,10 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * 0: never; not under graph capture), "jit" (pass A compiled at run time
  * for the query's shape: 0 never, 1 when the input has at least "jit_rows"
  * rows, 2 always; the precompiled kernel runs when it cannot be built),
- * "jit_rows", "scan_prefetch" (1: the runtime-compiled pass A issues the
+ * "jit_rows", "jit_wait" (0: a runtime-compiled kernel is used only when
+ * it was compiled already, so no call waits seconds for a compile, and a
+ * batch query is routed to the partitioned passes only then; a server's
+ * lanes run with 0), "scan_prefetch" (1: the runtime-compiled pass A issues the
  * next tile's loads before it folds the current tile; 0: load, then fold),
  * "lds_odd_stride" (1: the partitioned path's LDS table entries are an odd
  * number of 8-byte words apart; 0: exactly the state's words),

@@ -92,6 +92,27 @@ def test_batch_equals_one_launch_per_query(setting, gpu_ctx):
         one.release()
 
 
+def test_batch_never_waits_for_a_compile_with_jit_wait_0(gpu_ctx):
+    # a serving lane's setting: queries that would be routed to the
+    # partitioned passes use runtime-compiled kernels only when they exist
+    jobs, lfs = make_jobs(gpu_ctx, 40, 3)
+    try:
+        for k, v in {"batch_part_rows": 1000, "jit": 2, "jit_wait": 0}.items():
+            gpu_ctx.tune(k, v)
+        before = gpu_ctx.observe()["jit_compiles"]
+        batch = gpu_ctx.groupby_batch(jobs)
+        assert gpu_ctx.observe()["jit_compiles"] == before
+    finally:
+        for k, v in {**TUNE_DEFAULTS, "batch_part_rows": 1 << 22}.items():
+            gpu_ctx.tune(k, v)
+    for (plan, keys, cols), got in zip(jobs, batch):
+        assert not isinstance(got, Exception), got
+        one = gpu_ctx.groupby(plan, keys, cols)
+        frames_equal_bitwise(result_frame(got, plan, keys), result_frame(one, plan, keys), len(plan.keys))
+        got.release()
+        one.release()
+
+
 def test_batch_matches_oracle(gpu_ctx):
     jobs, lfs = make_jobs(gpu_ctx, 12, 2)
     for (plan, keys, cols), got, lf in zip(jobs, gpu_ctx.groupby_batch(jobs), lfs):

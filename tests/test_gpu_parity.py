@@ -179,7 +179,8 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
                  "max_parts": 4096, "part_lds_bytes": 131072, "sub_lds_bytes": 49152, "sub_block": 512, "jit": 1,
                  "scan_prefetch": 1, "lds_odd_stride": 0, "part_block": 1024, "scan_block": 1024,
                  "scan_wg_per_cu": 1, "scan_lds_bytes": 159744, "two_level_parts": 2048,
-                 "scatter_slices": 1, "scatter_chunk": 16384, "scatter_stage": 1}
+                 "scatter_slices": 1, "scatter_chunk": 16384, "scatter_stage": 1, "jit_wait": 1,
+                 "sub_wg_per_cu": -1, "sub_fill": 50}
 
 
 @pytest.mark.parametrize("setting", [
@@ -192,6 +193,10 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
     {"two_level": 1, "max_parts": 1, "sub_lds_bytes": 4096, "sub_block": 64},
     {"two_level": 1, "sub_lds_bytes": 98304, "sub_block": 512},
     {"two_level": 1, "two_level_parts": 64}, {"two_level": 1, "two_level_parts": 2, "jit": 2},
+    # second-level workgroups: one per sub-partition, or looping over many;
+    # sub-partitions nearly full (capacity retries) or sparse
+    {"two_level": 1, "sub_wg_per_cu": 0}, {"two_level": 1, "sub_wg_per_cu": 1, "jit": 2},
+    {"two_level": 1, "sub_fill": 90, "sub_lds_bytes": 4096}, {"two_level": 1, "sub_fill": 10, "sub_wg_per_cu": 3},
     # pass A compiled at run time for the query's shape
     {"jit": 2}, {"jit": 2, "lds_mode": 0}, {"jit": 2, "two_level": 1}, {"jit": 2, "scan_unroll": 4},
     # its loads pipelined (the default) or not, over ranges of odd and even tile counts
