@@ -66,7 +66,7 @@ Tuning default_tuning(dfr_context *ctx) {
   t.scan_prefetch = env_int("DFR_SCAN_PREFETCH", 1) ? 1 : 0;
   t.lds_odd_stride = env_int("DFR_LDS_ODD_STRIDE", 0) ? 1 : 0;
   t.part_block = env_int("DFR_PART_BLOCK", 1024);
-  t.two_level_parts = env_int("DFR_TWO_LEVEL_PARTS", 2048);
+  t.two_level_parts = env_int("DFR_TWO_LEVEL_PARTS", 512);
   t.scatter_slices = env_int("DFR_SCATTER_SLICES", 1);
   t.scatter_chunk = env_int("DFR_SCATTER_CHUNK", 16384) == 8192 ? 8192 : 16384;
   t.scatter_stage = env_int("DFR_SCATTER_STAGE", 1) ? 1 : 0;

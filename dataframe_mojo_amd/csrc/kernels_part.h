@@ -790,6 +790,43 @@ __device__ __forceinline__ void part_fold_records(const PV &P, uint64_t *lkeys, 
   }
 }
 
+// Merges one partial state (its key, and its W words at `s`) into the LDS
+// table; a key the table cannot place goes to the HBM table, or, with DENSE
+// (no HBM table), raises the sub-partition capacity fault.
+template <bool DENSE, class SrcPtr, class PV>
+__device__ __forceinline__ void merge_one(const PV &P, uint64_t *lkeys, uint64_t *lst, uint64_t key, SrcPtr s) {
+  const int e = part_find(P, lkeys, key);
+  if (e >= 0) {
+    merge_full(P, lst + (int64_t)e * P.lds_stride, s);
+  } else if (DENSE) {
+    atomicOr(&P.fault[3], 1u);
+  } else {
+    const uint64_t g = global_slot(P, key);
+    if (g != kEmpty) merge_full(P, P.state + g * (uint64_t)P.words, s);
+  }
+}
+
+// Merges the partial-state records [q0, q1) of `par` (key, then W words).
+// With a register copy of the plan (W known), a record's words are loaded
+// all at once first: one memory latency per record, not one per word.
+template <bool DENSE, class PV>
+__device__ __forceinline__ void merge_partials(const PV &P, uint64_t *lkeys, uint64_t *lst,
+                                               const DFR_GLOBAL uint64_t *par, uint64_t q0, uint64_t q1) {
+  const int W = P.words;
+  for (uint64_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
+    const DFR_GLOBAL uint64_t *rec = par + i * (1 + W);
+    if constexpr (RegPlan<PV>::value) {
+      uint64_t r[kMaxWords + 1];
+#pragma unroll
+      for (int w = 0; w <= kMaxWords; ++w)
+        if (w <= W) r[w] = rec[w];
+      merge_one<DENSE>(P, lkeys, lst, r[0], (const uint64_t *)(r + 1));
+    } else {
+      merge_one<DENSE>(P, lkeys, lst, rec[0], rec + 1);
+    }
+  }
+}
+
 // Pass C: partition p.
 template <class PV>
 __device__ __forceinline__ void part_aggregate_body(const PV &P) {
@@ -817,17 +854,7 @@ __device__ __forceinline__ void part_aggregate_body(const PV &P) {
     part_fold_records<8, 2, false>(P, lkeys, lst, keymask, recs, b0, b1);
   else
     part_fold_records<kMaxCols + 3, 1, false>(P, lkeys, lst, keymask, recs, b0, b1);
-  for (uint64_t i = P.base_par[p] + threadIdx.x; i < P.base_par[p + 1]; i += blockDim.x) {
-    const DFR_GLOBAL uint64_t *rec = gp(P.part_par) + i * (1 + W);
-    const uint64_t key = rec[0];
-    const int e = part_find(P, lkeys, key);
-    if (e >= 0) {
-      merge_full(P, lst + (int64_t)e * S, rec + 1);
-    } else {
-      const uint64_t g = global_slot(P, key);
-      if (g != kEmpty) merge_full(P, P.state + g * (uint64_t)W, rec + 1);
-    }
-  }
+  merge_partials<false>(P, lkeys, lst, gp(P.part_par), P.base_par[p], P.base_par[p + 1]);
   __syncthreads();
   // every key of this partition lives only here: plain stores
   for (int e = threadIdx.x; e < E; e += blockDim.x) {
@@ -1088,14 +1115,7 @@ __device__ __forceinline__ void sub_aggregate_loop(const PV &P, uint32_t *wsum, 
       fold_recs<RW, U, true>(P, lkeys, lst, keymask, t);
       if (b0 + tile < b1) part_fold_records<RW, U, true>(P, lkeys, lst, keymask, recs, b0 + tile, b1);
     }
-    for (uint64_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
-      const DFR_GLOBAL uint64_t *rec = gp(P.part2_par) + i * (1 + W);
-      const int e = part_find(P, lkeys, rec[0]);
-      if (e >= 0)
-        merge_full(P, lst + (int64_t)e * S, rec + 1);
-      else
-        atomicOr(&P.fault[3], 1u);
-    }
+    merge_partials<true>(P, lkeys, lst, gp(P.part2_par), q0, q1);
     // and its first tile, in flight while this one's groups are placed
     if (nb0 < nb1) load_recs<RW, U>(P, t, recs, nb0, nb1);
     b0 = nb0;

@@ -54,7 +54,7 @@ struct Tuning {
   int scan_prefetch = 1;       // runtime-compiled pass A: 1 overlaps the next tile's loads with this tile's fold
   int lds_odd_stride = 0;      // partitioned path: LDS table entries at an odd number of words
   int part_block = 1024;       // threads of a first-level pass C workgroup
-  int two_level_parts = 2048;  // first-level partitions of a two-level query (a power of two)
+  int two_level_parts = 512;   // first-level partitions of a two-level query (a power of two)
   int scatter_slices = 1;      // pass A workgroup's waves split into this many scatter workgroups
   int scatter_chunk = 16384;   // records a scatter workgroup sorts at once (8192 or 16384)
   int scatter_stage = 1;       // 1: the scatters stage whole records of 2 to 4 words in LDS
