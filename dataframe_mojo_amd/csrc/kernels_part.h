@@ -39,8 +39,21 @@ __device__ __forceinline__ int part_slot(const PV &P, uint64_t key) {
   return (int)((key32(key) * 0x85EBCA77u) >> (32 - P.c_log2));
 }
 
+// A 128-bit add in two steps: the low word's atomic, whose returned old
+// value add128_finish needs for the carry into the high word. Other updates
+// can be issued between the two, so the wave waits for the value once.
+__device__ __forceinline__ uint64_t add128_issue(uint64_t *p, uint64_t lo) {
+  return atomicAdd((unsigned long long *)p, (unsigned long long)lo);
+}
+__device__ __forceinline__ void add128_finish(uint64_t *p, uint64_t old, uint64_t lo, uint64_t hi) {
+  const uint64_t h = hi + ((old + lo < old) ? 1ull : 0ull);
+  if (h) atomicAdd((unsigned long long *)(p + 1), (unsigned long long)h);
+}
+
 // Folds one kept row into a full-layout state (every word, FLAGS and ZFIRST
-// included). `val(c)` and `ok(c)` give column c's value and validity.
+// included). `val(c)` and `ok(c)` give column c's value and validity. Per
+// column, the 128-bit sums' low-word atomics are issued first and their
+// carries last.
 template <int NC = kMaxCols, class Val, class Ok, class PV>
 __device__ __forceinline__ void fold_full(const PV &P, uint64_t *st, int64_t row, Val val, Ok ok) {
   atomicAdd((unsigned long long *)&st[0], 1ull);
@@ -53,11 +66,10 @@ __device__ __forceinline__ void fold_full(const PV &P, uint64_t *st, int64_t row
     if (!m || !ok(c)) continue;
     uint64_t *b = st + A.gbase;
     const uint64_t v = val(c);
-    if (m & M_VCOUNT) atomicAdd((unsigned long long *)&b[A.o_count], 1ull);
-    if (m & M_ISUM64) atomicAdd((unsigned long long *)&b[A.o_isum], (unsigned long long)v);
-    if (m & M_ISUM128) add128(&b[A.o_isum], v, ((int64_t)v < 0) ? ~0ull : 0ull);
-    if (m & M_IMIN) atomicMin((long long *)&b[A.o_min], (long long)v);
-    if (m & M_IMAX) atomicMax((long long *)&b[A.o_max], (long long)v);
+    const uint64_t vhi = ((int64_t)v < 0) ? ~0ull : 0ull;
+    uint64_t i_old = 0, f_old = 0, q_old = 0, flo = 0, fhi = 0, qlo = 0, qhi = 0;
+    bool finite = false;
+    if (m & M_ISUM128) i_old = add128_issue(&b[A.o_isum], v);
     if (m & (M_FSUM | M_FSUMSQ | M_FMIN | M_FMAX | M_FLAGS | M_ZFIRST)) {
       const uint64_t bits = A.int_as_float ? f64_bits((double)(int64_t)v) : v;
       const uint32_t ex = (uint32_t)(bits >> 52) & 0x7ffu;
@@ -68,15 +80,14 @@ __device__ __forceinline__ void fold_full(const PV &P, uint64_t *st, int64_t row
           atomicOr((unsigned long long *)&b[A.o_flags],
                    (unsigned long long)(nan ? FL_NAN : ((bits >> 63) ? FL_NINF : FL_PINF)));
       } else {
+        finite = true;
         if (m & M_FSUM) {
-          uint64_t lo, hi;
-          fix_of(bits, A.fshift, lo, hi);
-          add128(&b[A.o_fsum], lo, hi);
+          fix_of(bits, A.fshift, flo, fhi);
+          f_old = add128_issue(&b[A.o_fsum], flo);
         }
         if (m & M_FSUMSQ) {
-          uint64_t lo, hi;
-          fixsq_of(bits, A.fshift2, lo, hi);
-          add128(&b[A.o_sq], lo, hi);
+          fixsq_of(bits, A.fshift2, qlo, qhi);
+          q_old = add128_issue(&b[A.o_sq], qlo);
         }
         if ((m & M_ZFIRST) && (bits << 1) == 0)
           atomicMin((unsigned long long *)&b[A.o_zfirst],
@@ -88,6 +99,13 @@ __device__ __forceinline__ void fold_full(const PV &P, uint64_t *st, int64_t row
         if (m & M_FMAX) atomicMax((unsigned long long *)&b[A.o_max], (unsigned long long)k);
       }
     }
+    if (m & M_VCOUNT) atomicAdd((unsigned long long *)&b[A.o_count], 1ull);
+    if (m & M_ISUM64) atomicAdd((unsigned long long *)&b[A.o_isum], (unsigned long long)v);
+    if (m & M_IMIN) atomicMin((long long *)&b[A.o_min], (long long)v);
+    if (m & M_IMAX) atomicMax((long long *)&b[A.o_max], (long long)v);
+    if (m & M_ISUM128) add128_finish(&b[A.o_isum], i_old, v, vhi);
+    if (finite && (m & M_FSUM)) add128_finish(&b[A.o_fsum], f_old, flo, fhi);
+    if (finite && (m & M_FSUMSQ)) add128_finish(&b[A.o_sq], q_old, qlo, qhi);
   }
 }
 
