@@ -1,8 +1,8 @@
 """Expressions for the filter -> group_by -> agg path (expr.mojo:2869-3006).
 
 Nodes: col, lit, comparisons (== != < <= > >=), %, & | ~, is_null,
-is_not_null, and the reductions sum(min_count), count, len, min, max, mean,
-var(ddof), std(ddof), with alias.
+is_not_null, is_in, is_between, and the reductions sum(min_count), count,
+len, min, max, mean, var(ddof), std(ddof), with alias.
 """
 from __future__ import annotations
 
@@ -82,6 +82,20 @@ class Expr:
 
     def is_not_null(self) -> "Expr":
         return Expr("is_not_null", (self,))
+
+    def is_in(self, values) -> "Expr":
+        """True when the value equals any candidate (integers and floats
+        compare by value); a null candidate never matches, a null input stays
+        null, and an empty list gives false (docs/expressions.md:445-448)."""
+        return Expr("is_in", (self, tuple(values)))
+
+    def is_between(self, lower, upper, closed: str = "both") -> "Expr":
+        """lower <= value <= upper (closed "both", "left", "right" or
+        "none" picks which bounds are inclusive), as the Kleene AND of the
+        two comparisons (docs/expressions.md:448-450)."""
+        if closed not in ("both", "left", "right", "none"):
+            raise ValueError(f"closed must be both, left, right or none, not {closed!r}")
+        return Expr("between", (self, _wrap(lower), _wrap(upper), closed))
 
     # reductions ----------------------------------------------------------
     def _agg(self, op: str, **kw) -> "Expr":

@@ -110,9 +110,55 @@ class _Compiler:
             return self.leaf(Leaf(target.args[0], op))
         if k == "cmp":
             return self.comparison(e, neg)
+        if k == "between":
+            x, lo, hi, closed = e.args
+            op_lo = "ge" if closed in ("both", "left") else "gt"
+            op_hi = "le" if closed in ("both", "right") else "lt"
+            return self.formula(Expr("and", (Expr("cmp", (op_lo, x, lo)), Expr("cmp", (op_hi, x, hi)))), neg)
+        if k == "is_in":
+            return self.membership(e, neg)
         if k == "col" and self.dtype(e.args[0]) not in (INT32, INT64, FLOAT64, STRING):
             raise Unsupported("boolean columns are not on the device path")
         raise Unsupported(f"predicate {e!r} is not on the device path")
+
+    def membership(self, e: Expr, neg: bool):
+        """is_in as an OR of equality leaves, one per distinct non-null
+        candidate that can equal a value of the column (an integer column
+        never equals a fractional candidate); a null input makes every leaf
+        null, so the OR stays null. With no candidate left the result is
+        false, or null for a null input (negated: is_not_null)."""
+        x, values = e.args
+        if x.kind != "col":
+            raise Unsupported("is_in of an expression")
+        dt = self.dtype(x.args[0])
+        cands = []
+        for v in values:
+            if v is None:
+                continue  # never matches
+            if isinstance(v, bool):
+                raise Unsupported("is_in with a boolean candidate")
+            if dt == STRING:
+                if not isinstance(v, str):
+                    raise Unsupported("dictionary columns take string candidates")
+            elif isinstance(v, str):
+                raise Unsupported(f"numeric column is_in with {v!r}")
+            elif dt != FLOAT64 and isinstance(v, float):
+                if not math.isfinite(v) or v != int(v):
+                    continue  # no integer equals it
+                v = int(v)
+            elif dt == FLOAT64:
+                v = float(v)
+            if v not in cands:
+                cands.append(v)
+        if not cands:
+            if neg:
+                return self.leaf(Leaf(x.args[0], _lib.IS_NOT_NULL))
+            return ("const", False)
+        f = None
+        for v in cands:
+            term = Expr("cmp", ("eq", x, Expr("lit", (v,))))
+            f = term if f is None else Expr("or", (f, term))
+        return self.formula(f, neg)
 
     def leaf(self, leaf: Leaf):
         self.leaves.append(leaf)

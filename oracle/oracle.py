@@ -22,6 +22,7 @@ import os
 
 import numpy as np
 
+from dataframe_mojo_amd.expr import Expr
 from dataframe_mojo_amd.frame import FLOAT64, INT32, INT64, STRING, Column, DataFrame, Series
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -137,6 +138,32 @@ def evaluate(frame, e):
     if k == "is_not_null":
         _, am, _ = evaluate(frame, e.args[0])
         return am.copy(), np.ones(n, dtype=bool), "bool"
+    if k == "is_in":
+        # EQ per candidate with nulls filled false, OR, then the input's nulls
+        # kept (docs/design/scalar-membership-chunks.md:49-51;
+        # docs/expressions.md:445-448)
+        av, am, at = evaluate(frame, e.args[0])
+        hit = np.zeros(n, dtype=bool)
+        for c in e.args[1]:
+            if c is None:
+                continue
+            if at == STRING:
+                hit |= av == c
+            elif at == FLOAT64:
+                with np.errstate(invalid="ignore"):
+                    hit |= av == float(c)
+            elif isinstance(c, float):
+                # an integer equals a float candidate only when it is integral
+                if np.isfinite(c) and c == int(c):
+                    hit |= av.astype(np.int64) == int(c)
+            else:
+                hit |= av.astype(np.int64) == int(c)
+        return hit & am, am.copy(), "bool"
+    if k == "between":
+        x, lo, hi, closed = e.args
+        op_lo = "ge" if closed in ("both", "left") else "gt"
+        op_hi = "le" if closed in ("both", "right") else "lt"
+        return evaluate(frame, Expr("and", (Expr("cmp", (op_lo, x, lo)), Expr("cmp", (op_hi, x, hi)))))
     raise OracleError(f"oracle cannot evaluate {e!r}")
 
 

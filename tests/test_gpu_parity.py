@@ -156,6 +156,7 @@ FILTERS = [
     [(col("b") % 3 != 0) | col("x").is_null()],
     [~((col("x") > 0.25) & (col("s") == "lbl3"))],
     [col("a").is_not_null(), col("x") <= 1.0],
+    [col("k").is_in([3, 17, None, 40.0]) | col("s").is_in(["lbl3", "lbl5"]), ~col("x").is_between(-1.0, 0.0)],
 ]
 
 KEYS = [["k"], ["k32"], ["s"], ["k", "s"], ["k32", "k", "s"]]

@@ -75,6 +75,11 @@ PREDICATES = [
     (col("s") != "r") & col("a").is_not_null(),
     (col("a") % -4 == -1) & (col("x") >= -1.0) & (col("b") != 0) & ~(col("s") == "zzz"),
     0 < col("a"),
+    col("a").is_in([1, -3, None, 2.5, 7.0]),
+    ~col("a").is_in([0, 5]) | col("s").is_in(["q"]),
+    ~col("a").is_in([None]),
+    col("x").is_between(-0.5, 1.0, closed="right") & col("b").is_between(-2, 4),
+    ~col("b").is_between(None, 3),
 ]
 
 

@@ -233,6 +233,58 @@ def test_filter_kleene_semantics():
     assert keep((col("a") % col("b")) == 0) == [False, False, False, False, True]
 
 
+def test_membership_and_range_semantics():
+    # docs/expressions.md:445-450: a null candidate never matches, a null
+    # input stays null, an empty list gives false; is_between is the Kleene
+    # AND of its two comparisons, `closed` picking the inclusive bounds
+    df = DataFrame([
+        Series("a", Column.int64([1, 2, 3, 4, 5], [True, True, False, True, True])),
+        Series("x", Column.float64([0.5, -0.0, 2.0, float("nan"), 3.0])),
+        Series("s", Column.string(["p", "q", None, "p", "r"])),
+    ])
+    keep = lambda e: oracle.kept_rows(df, [e]).tolist()  # noqa: E731
+    assert keep(col("a").is_in([2, 4, None])) == [False, True, False, True, False]
+    assert keep(~col("a").is_in([2, 4, None])) == [True, False, False, False, True]  # null input stays null
+    assert keep(col("a").is_in([])) == [False] * 5
+    assert keep(~col("a").is_in([])) == [True, True, False, True, True]
+    assert keep(col("a").is_in([2.0, 4.5])) == [False, True, False, False, False]  # by value
+    assert keep(col("x").is_in([0.0, float("nan"), 3])) == [False, True, False, False, True]  # IEEE equality
+    assert keep(col("s").is_in(["p", "zz"])) == [True, False, False, True, False]
+    assert keep(col("a").is_between(2, 4)) == [False, True, False, True, False]
+    assert keep(col("a").is_between(2, 4, closed="none")) == [False, False, False, False, False]
+    assert keep(col("a").is_between(2, 4, closed="left")) == [False, True, False, False, False]
+    assert keep(col("a").is_between(2, 4, closed="right")) == [False, False, False, True, False]
+    # a null bound: null AND c is false where c is false, else null
+    assert keep(col("a").is_between(None, 3)) == [False] * 5
+    assert keep(~col("a").is_between(None, 3)) == [False, False, False, True, True]
+
+
+def test_membership_matches_brute_force():
+    rng = np.random.default_rng(5)
+    n = 400
+    a = rng.integers(-6, 6, n)
+    av = rng.random(n) > 0.2
+    df = DataFrame([Series("a", Column.int64(a, av))])
+    for cands in ([], [None], [0], [1, -3, 5, None], [2.0, 2.5, -6], list(range(-8, 8))):
+        for neg in (False, True):
+            e = col("a").is_in(cands)
+            got = oracle.kept_rows(df, [~e if neg else e]).tolist()
+            want = []
+            for i in range(n):
+                if not av[i]:
+                    want.append(False)  # null stays null: never kept
+                    continue
+                hit = any(c is not None and float(c) == float(a[i]) for c in cands)
+                want.append(hit != neg)
+            assert got == want, (cands, neg)
+        for lo, hi in ((-2, 3), (3, -2), (0, 0)):
+            for closed in ("both", "left", "right", "none"):
+                got = oracle.kept_rows(df, [col("a").is_between(lo, hi, closed)]).tolist()
+                lo_ok = (lambda v: v >= lo) if closed in ("both", "left") else (lambda v: v > lo)
+                hi_ok = (lambda v: v <= hi) if closed in ("both", "right") else (lambda v: v < hi)
+                assert got == [bool(av[i] and lo_ok(a[i]) and hi_ok(a[i])) for i in range(n)]
+
+
 def test_golden_groupby_1m_fixture():
     with open(os.path.join(HERE, "golden", "groupby_1m.json")) as f:
         gold = json.load(f)
