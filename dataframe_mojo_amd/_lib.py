@@ -29,6 +29,7 @@ DFR_DICT = 4
 EQ, NE, LT, LE, GT, GE, IS_NULL, IS_NOT_NULL = range(8)
 LEAF_MOD = 1
 LEAF_NOT = 2
+LEAF_COLUMN = 4
 MAX_LEAVES = 8
 
 AGG_LEN, AGG_COUNT, AGG_SUM, AGG_MIN, AGG_MAX, AGG_MEAN, AGG_VAR, AGG_STD = range(8)

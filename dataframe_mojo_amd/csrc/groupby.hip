@@ -222,6 +222,15 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
       int pc = colmap[L.column];
       if ((L.flags & DFR_LEAF_MOD) && P.cols[pc].dtype == DFR_FLOAT64)
         return dfr_fail(error, DFR_UNSUPPORTED, "leaf %d: Float64 modulo is not on the device path", i);
+      if (L.flags & DFR_LEAF_COLUMN) {
+        if (L.literal < 0 || L.literal >= q->n_columns)
+          return dfr_fail(error, DFR_INVALID, "leaf %d compares with column %lld", i, (long long)L.literal);
+        const int32_t a = P.cols[pc].dtype, b = P.cols[colmap[L.literal]].dtype;
+        const bool ints = (a == DFR_INT32 || a == DFR_INT64) && (b == DFR_INT32 || b == DFR_INT64);
+        if ((L.flags & DFR_LEAF_MOD) || L.op > DFR_GE || !(ints || (a == DFR_FLOAT64 && b == DFR_FLOAT64)))
+          return dfr_fail(error, DFR_UNSUPPORTED,
+                          "leaf %d: column comparisons take two integer or two Float64 columns", i);
+      }
       order.push_back({pc, i});
     }
     std::stable_sort(order.begin(), order.end(),
@@ -237,6 +246,7 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
         D.bit = order[at].second;
         D.lit = L.literal;
         D.mod = L.modulus;
+        D.col2 = (L.flags & DFR_LEAF_COLUMN) ? colmap[L.literal] : -1;
         P.leaf_cols |= 1 << c;
         ++at;
       }

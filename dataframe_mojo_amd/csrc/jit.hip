@@ -56,7 +56,21 @@ std::string header_dir() {
 // shape.
 std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF, int which) {
   std::ostringstream o;
-  o << "#define DFR_JIT 1\n#include \"kernels_part.h\"\nusing namespace dfr;\n"
+  bool col_leaves = false;
+  for (int l = 0; l < P.n_leaves; ++l) col_leaves |= P.leaves[l].col2 >= 0;
+  o << "#define DFR_JIT 1\n#define DFR_COL_LEAVES " << (col_leaves ? 1 : 0) << "\n#include \"kernels_part.h\"\nusing namespace dfr;\n";
+  if (col_leaves) {
+    // the right-hand column of each column-comparison leaf, by constant index
+    o << "template <int NC>\n__device__ __forceinline__ void dfr::jit_leaf_rhs(int l, const uint64_t (&vals)[NC], "
+         "uint32_t vbits, uint64_t &v, bool &ok) {\n  switch (l) {\n";
+    for (int l = 0; l < P.n_leaves; ++l)
+      if (P.leaves[l].col2 >= 0)
+        o << "    case " << l << ":\n      if constexpr (" << P.leaves[l].col2 << " < NC) {\n        v = vals["
+          << P.leaves[l].col2 << "];\n        ok = ((vbits >> " << P.leaves[l].col2 << ") & 1u) != 0;\n      }\n"
+          << "      break;\n";
+    o << "    default: break;\n  }\n}\n";
+  }
+  o
     << "__device__ __forceinline__ DevPlan make_plan(const DevPlan *plan) {\n"
     << "  PlanRef R = plan_ref(plan);\n"
     << "  DevPlan L{};\n";
@@ -124,7 +138,8 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF, int 
       << P.keys[k].is_float << ";\n";
   for (int l = 0; l < P.n_leaves; ++l)
     o << "  L.leaves[" << l << "].op = " << P.leaves[l].op << "; L.leaves[" << l << "].flags = " << P.leaves[l].flags
-      << "; L.leaves[" << l << "].bit = " << P.leaves[l].bit << ";\n";
+      << "; L.leaves[" << l << "].bit = " << P.leaves[l].bit << "; L.leaves[" << l << "].col2 = " << P.leaves[l].col2
+      << ";\n";
   for (int w = 0; w < P.words; ++w)
     o << "  L.ident[" << w << "] = " << P.ident[w] << "ull; L.wop[" << w << "] = " << (int)P.wop[w] << ";\n";
   o << "  return L;\n}\n";

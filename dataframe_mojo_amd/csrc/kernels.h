@@ -22,6 +22,13 @@
 #ifndef DFR_JIT
 #define DFR_JIT 0
 #endif
+// DFR_COL_LEAVES 0: no predicate leaf compares two columns. A kernel
+// compiled at run time for a plan with such leaves gets jit_leaf_rhs from
+// its generated source: leaf l's right-hand column by constant index (a
+// select over every column per unrolled leaf multiplies compile time).
+#ifndef DFR_COL_LEAVES
+#define DFR_COL_LEAVES 1
+#endif
 #ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #endif
@@ -99,14 +106,17 @@ __device__ __forceinline__ uint64_t atomic_load_u64(const uint64_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ bool leaf_true(const DevLeaf &L, bool is_float,
-                                          uint64_t v, bool valid) {
+// `v2`, `valid2`: the right-hand column's value under DFR_LEAF_COLUMN.
+__device__ __forceinline__ bool leaf_true(const DevLeaf &L, bool is_float, uint64_t v, bool valid,
+                                          uint64_t v2, bool valid2) {
   if (L.op == 6) return !valid;  // DFR_IS_NULL
   if (L.op == 7) return valid;   // DFR_IS_NOT_NULL
-  if (!valid) return false;
+  const bool by_column = (L.flags & 4) != 0;
+  if (!valid || (by_column && !valid2)) return false;
+  const uint64_t rhs = by_column ? v2 : (uint64_t)L.lit;
   bool t;
   if (is_float) {
-    double x = bits_f64(v), y = bits_f64((uint64_t)L.lit);
+    double x = bits_f64(v), y = bits_f64(rhs);
     switch (L.op) {
       case 0: t = x == y; break;
       case 1: t = x != y; break;
@@ -124,7 +134,7 @@ __device__ __forceinline__ bool leaf_true(const DevLeaf &L, bool is_float,
       if (q != 0 && ((q < 0) != (m < 0))) q += m;
       x = q;
     }
-    int64_t y = L.lit;
+    int64_t y = (int64_t)rhs;
     switch (L.op) {
       case 0: t = x == y; break;
       case 1: t = x != y; break;
@@ -135,6 +145,23 @@ __device__ __forceinline__ bool leaf_true(const DevLeaf &L, bool is_float,
     }
   }
   return (L.flags & 2) ? !t : t;
+}
+
+#if DFR_COL_LEAVES && DFR_JIT
+template <int NC>
+__device__ __forceinline__ void jit_leaf_rhs(int l, const uint64_t (&vals)[NC], uint32_t vbits, uint64_t &v, bool &ok);
+#endif
+
+// Column k's value and validity from a row's values (k uniform: a select
+// chain, no scratch).
+template <int NC>
+__device__ __forceinline__ void pick_col(const uint64_t (&vals)[NC], uint32_t vbits, int k, uint64_t &v, bool &ok) {
+#pragma unroll
+  for (int j = 0; j < NC; ++j)
+    if (j == k) {
+      v = vals[j];
+      ok = ((vbits >> j) & 1u) != 0;
+    }
 }
 
 __device__ __forceinline__ uint32_t truth_bit(PlanRef P, uint32_t m) {
@@ -501,12 +528,22 @@ __device__ __forceinline__ void fold_tile(const PV &P, const ScanTile<NC, U> &t,
           for (int l = 0; l < kMaxLeaves; ++l) {
             if (l < P.leaf_begin[c] || l >= P.leaf_begin[c + 1]) continue;
             const DevLeaf L = P.leaves[l];
-            m |= (uint32_t)leaf_true(L, fl, vals[c], (vbits >> c) & 1u) << L.bit;
+            uint64_t v2 = 0;
+            bool ok2 = false;
+#if DFR_COL_LEAVES && DFR_JIT
+            jit_leaf_rhs<NC>(l, vals, vbits, v2, ok2);
+#elif DFR_COL_LEAVES
+            if (L.col2 >= 0) pick_col<NC>(vals, vbits, L.col2, v2, ok2);
+#endif
+            m |= (uint32_t)leaf_true(L, fl, vals[c], (vbits >> c) & 1u, v2, ok2) << L.bit;
           }
         } else {
           for (int l = P.leaf_begin[c]; l < P.leaf_begin[c + 1]; ++l) {
             const DevLeaf L = cget(P.leaves[l]);
-            m |= (uint32_t)leaf_true(L, fl, vals[c], (vbits >> c) & 1u) << L.bit;
+            uint64_t v2 = 0;
+            bool ok2 = false;
+            if (L.col2 >= 0) pick_col<NC>(vals, vbits, L.col2, v2, ok2);
+            m |= (uint32_t)leaf_true(L, fl, vals[c], (vbits >> c) & 1u, v2, ok2) << L.bit;
           }
         }
       }

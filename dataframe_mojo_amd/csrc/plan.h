@@ -105,8 +105,8 @@ constexpr uint64_t kNullFloatKey = 0x7FF0000000000001ull;
 struct DevLeaf {
   int32_t op;
   int32_t flags;
-  int32_t bit;  // position of this leaf's is-true bit in the truth index
-  int32_t pad;
+  int32_t bit;   // position of this leaf's is-true bit in the truth index
+  int32_t col2;  // the right-hand plan column under DFR_LEAF_COLUMN (flag 4), else -1
   int64_t lit;
   int64_t mod;
 };

@@ -49,6 +49,7 @@ class Leaf:
     flags: int = 0
     literal: int = 0  # int64 value or double bits
     modulus: int = 0
+    column2: str | None = None  # the right-hand column of a column comparison (LEAF_COLUMN)
 
 
 @dataclass
@@ -170,8 +171,18 @@ class _Compiler:
         op, a, b = e.args
         if a.kind == "lit" and b.kind != "lit":
             a, b, op = b, a, _FLIP[op]
+        if a.kind == "col" and b.kind == "col":
+            # column <op> column: two integer columns (compared by value) or
+            # two Float64 columns; null on either side is null
+            na, nb = a.args[0], b.args[0]
+            ta, tb = self.dtype(na), self.dtype(nb)
+            ints = ta in (INT32, INT64) and tb in (INT32, INT64)
+            if not (ints or (ta == FLOAT64 and tb == FLOAT64)):
+                raise Unsupported(f"comparison of {ta} with {tb} columns is not on the device path")
+            flags = _lib.LEAF_COLUMN | (_lib.LEAF_NOT if neg else 0)
+            return self.leaf(Leaf(na, _CMP[op], flags, 0, 0, nb))
         if b.kind != "lit":
-            raise Unsupported("comparisons between two columns are not on the device path")
+            raise Unsupported("comparisons of expressions other than columns and literals are not on the device path")
         value = b.args[0]
         flags = _lib.LEAF_NOT if neg else 0
         modulus = 0
@@ -260,6 +271,8 @@ def compile_plan(plan: LazyFrame) -> DevicePlan:
 
     for leaf in program.leaves:
         colindex(leaf.column)
+        if leaf.column2 is not None:
+            colindex(leaf.column2)
     aggs, names = [], []
     for e in plan.aggs:
         if not isinstance(e, Expr) or e.kind != "agg":

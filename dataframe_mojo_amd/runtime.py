@@ -160,7 +160,7 @@ def build_query(plan: DevicePlan, keys: list, columns: list):
         L.column = plan.columns.index(leaf.column)
         L.op = leaf.op
         L.flags = leaf.flags
-        L.literal = leaf.literal
+        L.literal = plan.columns.index(leaf.column2) if leaf.column2 is not None else leaf.literal
         L.modulus = leaf.modulus
     aggs = (_lib.Agg * max(1, len(plan.aggs)))()
     for i, (op, c, ddof, min_count) in enumerate(plan.aggs):

@@ -189,7 +189,10 @@ int64_t dfr_column_null_count(const dfr_column *column);
 
 /* ---- the query ---------------------------------------------------------- */
 /* Predicate leaves: `column <op> literal`, or `floor_mod(column, modulus)
- * <op> literal` when DFR_LEAF_MOD is set, or a null test. A comparison with
+ * <op> literal` when DFR_LEAF_MOD is set, or `column <op> other column`
+ * when DFR_LEAF_COLUMN is set (`literal` is then the other column's index
+ * into dfr_query.columns: both Int32/Int64, compared by value, or both
+ * Float64; no modulus), or a null test. A comparison with
  * a null value is not true (a null is neither true nor false, so it stays
  * not true under DFR_LEAF_NOT as well). Float comparisons are IEEE (NaN
  * compares false, NaN != NaN is true; docs/expressions.md:93-94). The
@@ -206,14 +209,15 @@ int64_t dfr_column_null_count(const dfr_column *column);
 #define DFR_IS_NOT_NULL 7
 #define DFR_LEAF_MOD 1 /* leaf flag: compare floor_mod(column, modulus) */
 #define DFR_LEAF_NOT 2 /* leaf flag: Kleene NOT of the comparison */
+#define DFR_LEAF_COLUMN 4 /* leaf flag: the right-hand side is column `literal` */
 #define DFR_MAX_LEAVES 8
 
 typedef struct dfr_leaf {
     int32_t column;  /* index into dfr_query.columns */
     int32_t op;      /* DFR_EQ .. DFR_IS_NOT_NULL */
-    int32_t flags;   /* DFR_LEAF_MOD */
+    int32_t flags;   /* DFR_LEAF_MOD, DFR_LEAF_NOT, DFR_LEAF_COLUMN */
     int32_t pad;
-    int64_t literal; /* int64 value or double bits */
+    int64_t literal; /* int64 value or double bits; a column index under DFR_LEAF_COLUMN */
     int64_t modulus; /* for DFR_LEAF_MOD; modulo by zero is null */
 } dfr_leaf;
 

@@ -157,6 +157,7 @@ FILTERS = [
     [~((col("x") > 0.25) & (col("s") == "lbl3"))],
     [col("a").is_not_null(), col("x") <= 1.0],
     [col("k").is_in([3, 17, None, 40.0]) | col("s").is_in(["lbl3", "lbl5"]), ~col("x").is_between(-1.0, 0.0)],
+    [(col("a") > col("b")) | ~(col("k32") <= col("k")), col("x") == col("x")],
 ]
 
 KEYS = [["k"], ["k32"], ["s"], ["k", "s"], ["k32", "k", "s"]]
@@ -374,8 +375,7 @@ def test_dict_workload_size_independent_properties(gpu_ctx):
     assert int(cnts.sum()) == int((kept & valid).sum())
 
 
-@pytest.mark.parametrize("ki", range(len(KEYS)))
-@pytest.mark.parametrize("fi", [1, 2, 3])
+@pytest.mark.parametrize("ki,fi", [(ki, fi) for ki in range(len(KEYS)) for fi in (1, 2, 3)] + [(3, 6)])
 def test_jit_scan_matches_precompiled(ki, fi, gpu_ctx):
     """Pass A compiled at run time for the query's shape gives the
     precompiled kernel's results bit for bit, and the oracle's."""

@@ -80,6 +80,9 @@ PREDICATES = [
     ~col("a").is_in([None]),
     col("x").is_between(-0.5, 1.0, closed="right") & col("b").is_between(-2, 4),
     ~col("b").is_between(None, 3),
+    col("a") > col("b"),
+    ~(col("b") <= col("a")) | (col("x") == col("x")),
+    (col("a") % 3 == 1) & (col("b") != col("a")),
 ]
 
 
@@ -98,8 +101,13 @@ def test_truth_tables_match_kleene_evaluation(i):
             lit = leaf.literal
             if c.dtype == "float64":
                 lit = np.int64(lit).view(np.float64)
+            if leaf.flags & _lib.LEAF_COLUMN:
+                other = df.column(leaf.column2).column
+                lit = other.values if c.dtype == "float64" else other.values.astype(np.int64)
             x = v.astype(np.int64) if c.dtype != "float64" else v
             ok = m.copy()
+            if leaf.flags & _lib.LEAF_COLUMN:
+                ok &= df.column(leaf.column2).column.is_valid()
             if leaf.flags & _lib.LEAF_MOD:
                 ok &= leaf.modulus != 0
                 x = np.remainder(x, leaf.modulus if leaf.modulus else 1)
@@ -118,7 +126,10 @@ def test_unsupported_plans_raise_before_submission():
     with pytest.raises(Unsupported):
         compile_plan(df.lazy().group_by("x", "a").agg(col("a").sum()))  # Float64 key beside another
     with pytest.raises(Unsupported):
-        compile_plan(df.filter(col("a") > col("b")).group_by("a").agg(col("b").sum()))
+        compile_plan(df.filter(col("x") > col("a")).group_by("a").agg(col("b").sum()))  # Float64 with Int64
+    with pytest.raises(Unsupported):
+        compile_plan(df.filter(col("s") == col("s")).group_by("a").agg(col("b").sum()))
+    compile_plan(df.filter(col("a") > col("b")).group_by("a").agg(col("b").sum()))  # two integer columns
     with pytest.raises(Unsupported):
         compile_plan(df.lazy().group_by("a").agg(col("s").sum()))
     compile_plan(df.lazy().group_by("x").agg(col("a").sum()))  # a Float64 key alone is on the path
