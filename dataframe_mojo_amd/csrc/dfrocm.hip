@@ -321,6 +321,8 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.part_block = (int)value;
   else if (k == "lds_odd_stride" && (value == 0 || value == 1))
     t.lds_odd_stride = (int)value;
+  else if (k == "part_odd_stride" && (value == 0 || value == 1))
+    t.part_odd_stride = (int)value;
   else if (k == "scan_prefetch" && (value == 0 || value == 1))
     t.scan_prefetch = (int)value;
   else if (k == "jit" && value >= 0 && value <= 2)

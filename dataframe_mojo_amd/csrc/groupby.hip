@@ -65,6 +65,7 @@ Tuning default_tuning(dfr_context *ctx) {
   t.jit = env_int("DFR_JIT", 1);
   t.scan_prefetch = env_int("DFR_SCAN_PREFETCH", 1) ? 1 : 0;
   t.lds_odd_stride = env_int("DFR_LDS_ODD_STRIDE", 0) ? 1 : 0;
+  t.part_odd_stride = env_int("DFR_PART_ODD_STRIDE", 1) ? 1 : 0;
   t.part_block = env_int("DFR_PART_BLOCK", 1024);
   t.two_level_parts = env_int("DFR_TWO_LEVEL_PARTS", 512);
   t.scatter_slices = env_int("DFR_SCATTER_SLICES", 1);
@@ -440,6 +441,7 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
   const uint64_t D = P.domain;
   const size_t slot_bytes = (size_t)P.words * 8;
   P.lds_stride = T.lds_odd_stride ? (P.words | 1) : P.words;
+  P.c_stride = T.part_odd_stride ? (P.words | 1) : P.words;
   const bool direct_fits = D <= (uint64_t)T.direct_max_slots &&
                            (int64_t)(D * slot_bytes) <= std::max<int64_t>(ctx->pool.budget / 4, 1 << 20);
   if (direct_fits) {
@@ -486,7 +488,7 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
     // partition tables: E entries of (key + W words, at an odd stride)
     // within part_lds_bytes
     const int W = P.words;
-    const int S = P.lds_stride;
+    const int S = P.c_stride;
     uint64_t ce = 64;
     // (2 KiB of LDS stay free for the second level's static arrays)
     const uint64_t c_budget = (uint64_t)std::min(T.part_lds_bytes, ctx->lds_per_block - 2048);
@@ -578,7 +580,7 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
     C.lds_bytes = (P.lds_mode == LDS_NONE ? 0
                                           : (size_t)P.lds_entries * 8 * (S + (P.lds_mode == LDS_HASHED ? 1 : 0))) +
                   (size_t)(P.seg_slices * P.n_parts + 2) * 4;
-    C.c_lds_bytes = (size_t)P.c_entries * 8 * (S + 1);
+    C.c_lds_bytes = (size_t)P.c_entries * 8 * (P.c_stride + 1);
   }
 }
 

@@ -117,7 +117,7 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF, int 
     << ";\n";
   o << "  L.rw = " << P.rw << "; L.n_vals = " << P.n_vals << "; L.rec_row = " << P.rec_row
     << "; L.rec_meta = " << P.rec_meta << "; L.rec_val0 = " << P.rec_val0 << "; L.vmask_in_key = " << P.vmask_in_key
-    << "; L.lds_stride = " << P.lds_stride << "; L.n_aggs = " << P.n_aggs << "; L.fused = " << P.fused
+    << "; L.lds_stride = " << P.lds_stride << "; L.c_stride = " << P.c_stride << "; L.n_aggs = " << P.n_aggs << "; L.fused = " << P.fused
     << "; L.seg_slices = " << P.seg_slices << ";\n";
   for (int i = 0; i < 8; ++i) o << "  L.truth[" << i << "] = " << P.truth[i] << "u;\n";
   for (int c = 0; c < P.n_cols; ++c) {

@@ -786,7 +786,7 @@ __device__ __forceinline__ void fold_recs(const PV &P, uint64_t *lkeys, uint64_t
     auto ok = [&](int c) { return ((vmask >> P.val_of[c]) & 1ull) != 0; };
     const int e = part_find(P, lkeys, key);
     if (e >= 0) {
-      fold_full(P, lst + (int64_t)e * P.lds_stride, row, val, ok);
+      fold_full(P, lst + (int64_t)e * P.c_stride, row, val, ok);
     } else if (DENSE) {
       atomicOr(&P.fault[3], 1u);
     } else {
@@ -815,7 +815,7 @@ template <bool DENSE, class SrcPtr, class PV>
 __device__ __forceinline__ void merge_one(const PV &P, uint64_t *lkeys, uint64_t *lst, uint64_t key, SrcPtr s) {
   const int e = part_find(P, lkeys, key);
   if (e >= 0) {
-    merge_full(P, lst + (int64_t)e * P.lds_stride, s);
+    merge_full(P, lst + (int64_t)e * P.c_stride, s);
   } else if (DENSE) {
     atomicOr(&P.fault[3], 1u);
   } else {
@@ -849,7 +849,7 @@ __device__ __forceinline__ void merge_partials(const PV &P, uint64_t *lkeys, uin
 template <class PV>
 __device__ __forceinline__ void part_aggregate_body(const PV &P) {
   extern __shared__ uint64_t smem[];
-  const int p = blockIdx.x, E = P.c_entries, W = P.words, S = P.lds_stride;
+  const int p = blockIdx.x, E = P.c_entries, W = P.words, S = P.c_stride;
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + E;
   for (int i = threadIdx.x; i < E; i += blockDim.x) {
@@ -1084,7 +1084,7 @@ __device__ __forceinline__ void emit_group(const PV &P, uint64_t i, const uint64
 template <int NA, int RW, int U, class PV>
 __device__ __forceinline__ void sub_aggregate_loop(const PV &P, uint32_t *wsum, uint64_t &gbase) {
   extern __shared__ uint64_t smem[];
-  const int E = P.c_entries, W = P.words, S = P.lds_stride, CH = P.sub_chunks;
+  const int E = P.c_entries, W = P.words, S = P.c_stride, CH = P.sub_chunks;
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + E;
   const uint64_t keymask = P.vmask_in_key ? ((1ull << P.key_bits) - 1) : ~0ull;

@@ -211,6 +211,8 @@ struct DevPlan {
   // words rounded up to odd, so 64-bit accesses of entries spread over
   // every LDS bank pair
   int32_t lds_stride;
+  // the same for the pass C tables (partition and sub-partition)
+  int32_t c_stride;
   // pass A's hashed LDS table: 4-entry buckets, any count (a key's bucket
   // is the high half of its 32-bit hash times the count)
   int32_t lds_buckets;

@@ -52,6 +52,7 @@ struct Tuning {
   int64_t jit_rows = 1 << 24;  // rows from which jit = 1 compiles
   int jit_wait = 1;            // 0: a runtime-compiled kernel is used only once compiled (no compile waits)
   int scan_prefetch = 1;       // runtime-compiled pass A: 1 overlaps the next tile's loads with this tile's fold
+  int part_odd_stride = 1;     // pass C tables: entries at an odd number of words
   int lds_odd_stride = 0;      // partitioned path: LDS table entries at an odd number of words
   int part_block = 1024;       // threads of a first-level pass C workgroup
   int two_level_parts = 512;   // first-level partitions of a two-level query (a power of two)

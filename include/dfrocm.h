@@ -140,8 +140,10 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * batch query is routed to the partitioned passes only then; a server's
  * lanes run with 0), "scan_prefetch" (1: the runtime-compiled pass A issues the
  * next tile's loads before it folds the current tile; 0: load, then fold),
- * "lds_odd_stride" (1: the partitioned path's LDS table entries are an odd
- * number of 8-byte words apart; 0: exactly the state's words),
+ * "lds_odd_stride" (1: pass A's LDS table entries are an odd number of
+ * 8-byte words apart; 0: exactly the state's words), "part_odd_stride" (the
+ * same for the pass C partition tables; 1, the default, spreads an
+ * instruction's 64-bit accesses of random entries over every LDS bank pair),
  * "part_block" (threads of a first-level partition table's workgroup),
  * "two_level_parts" (first-level partitions of a two-level query, a power
  * of two up to "max_parts"), "scatter_slices" (the partitioned pass A's

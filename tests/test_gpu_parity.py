@@ -179,7 +179,7 @@ def test_random_queries(seed, fi, ki):
 TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26, "hash_slots": 0, "block": 512,
                  "wg_per_cu": 2, "lds_probe": 4, "scan_unroll": 1, "agg_mode": 1, "two_level": -1,
                  "max_parts": 4096, "part_lds_bytes": 131072, "sub_lds_bytes": 49152, "sub_block": 512, "jit": 1,
-                 "scan_prefetch": 1, "lds_odd_stride": 0, "part_block": 1024, "scan_block": 1024,
+                 "scan_prefetch": 1, "lds_odd_stride": 0, "part_odd_stride": 1, "part_block": 1024, "scan_block": 1024,
                  "scan_wg_per_cu": 1, "scan_lds_bytes": 159744, "two_level_parts": 512,
                  "scatter_slices": 1, "scatter_chunk": 16384, "scatter_stage": 1, "jit_wait": 1,
                  "sub_wg_per_cu": -1, "sub_fill": 50}
@@ -205,6 +205,7 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
     {"jit": 2, "scan_prefetch": 0}, {"jit": 2, "scan_unroll": 2, "scan_block": 256},
     {"jit": 2, "scan_block": 64, "scan_wg_per_cu": 1},
     {"lds_odd_stride": 1}, {"lds_odd_stride": 1, "jit": 2, "two_level": 1},
+    {"part_odd_stride": 0}, {"part_odd_stride": 0, "jit": 2, "two_level": 1},
     {"part_block": 512}, {"part_block": 64, "jit": 2},
     # pass A's launch shape and LDS budget (bucket counts that are not powers of two)
     {"scan_block": 256, "scan_wg_per_cu": 2, "scan_lds_bytes": 40000},
