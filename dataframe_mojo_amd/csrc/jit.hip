@@ -178,9 +178,13 @@ JitEntry compile(const std::string &src, int device, int which) {
     arch = "--offload-arch=" + name.substr(0, colon);
   }
   const std::string inc = "-I" + header_dir();
+  // (no atomic optimizer: it would turn the one-lane reservation atomic of
+  // the second-level pass C into a wave-wide sequence that waits for the
+  // returned value at once, where the kernel means to wait for it later;
+  // the kernels' other uniform-address atomics are aggregated by hand)
   const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", inc.c_str(), "-Xclang", "-target-feature", "-Xclang",
-                        "-packed-fp32-ops"};
-  const hiprtcResult r = hiprtcCompileProgram(prog, 8, opts);
+                        "-packed-fp32-ops", "-mllvm", "-amdgpu-atomic-optimizer-strategy=None"};
+  const hiprtcResult r = hiprtcCompileProgram(prog, 10, opts);
   size_t code_size = 0;
   if (r == HIPRTC_SUCCESS && hiprtcGetCodeSize(prog, &code_size) == HIPRTC_SUCCESS && code_size > 0) {
     std::vector<char> code(code_size);

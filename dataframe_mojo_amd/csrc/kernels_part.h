@@ -718,21 +718,36 @@ __global__ void __launch_bounds__(1024) k_part_scatter_staged(const DevPlan *__r
 
 // LDS probe of a partition table (first-come, linear probing over the whole
 // table); -1 only when every entry holds another key.
+// `placed` (when given) counts the keys this call's wave places in empty
+// entries: one LDS atomic per wave.
 template <class PV>
-__device__ __forceinline__ int part_find(const PV &P, uint64_t *lkeys, uint64_t key) {
+__device__ __forceinline__ int part_find(const PV &P, uint64_t *lkeys, uint64_t key, uint32_t *placed = nullptr) {
   const int E = P.c_entries;
   int h = part_slot(P, key);
+  int e = -1;
+  bool won = false;
   for (int p = 0; p < E; ++p) {
     const uint64_t k = lkeys[h];
-    if (k == key) return h;
+    if (k == key) {
+      e = h;
+      break;
+    }
     if (k == kEmpty) {
       const unsigned long long old = atomicCAS((unsigned long long *)&lkeys[h], (unsigned long long)kEmpty,
                                                (unsigned long long)key);
-      if (old == kEmpty || old == key) return h;
+      if (old == kEmpty || old == key) {
+        won = old == kEmpty;
+        e = h;
+        break;
+      }
     }
     h = (h + 1) & (E - 1);
   }
-  return -1;
+  if (placed) {
+    const uint64_t wm = __ballot(won);  // the lanes that reached this call
+    if (wm && __lane_id() == __ffsll((unsigned long long)wm) - 1) atomicAdd(placed, (uint32_t)__popcll(wm));
+  }
+  return e;
 }
 
 // U records per thread of a pass C tile: their words (those past rw
@@ -765,7 +780,7 @@ __device__ __forceinline__ void load_recs(const PV &P, RecTile<RW, U> &t, const 
 // sub-partition capacity fault.
 template <int RW, int U, bool DENSE, class PV>
 __device__ __forceinline__ void fold_recs(const PV &P, uint64_t *lkeys, uint64_t *lst, uint64_t keymask,
-                                          const RecTile<RW, U> &t) {
+                                          const RecTile<RW, U> &t, uint32_t *placed = nullptr) {
   const int W = P.words;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -784,7 +799,7 @@ __device__ __forceinline__ void fold_recs(const PV &P, uint64_t *lkeys, uint64_t
     const int64_t row = P.rec_row >= 0 ? (int64_t)pick(P.rec_row) : 0;
     auto val = [&](int c) { return pick(P.rec_val0 + P.val_of[c]); };
     auto ok = [&](int c) { return ((vmask >> P.val_of[c]) & 1ull) != 0; };
-    const int e = part_find(P, lkeys, key);
+    const int e = part_find(P, lkeys, key, placed);
     if (e >= 0) {
       fold_full(P, lst + (int64_t)e * P.c_stride, row, val, ok);
     } else if (DENSE) {
@@ -800,11 +815,12 @@ __device__ __forceinline__ void fold_recs(const PV &P, uint64_t *lkeys, uint64_t
 // (loads first).
 template <int RW, int U, bool DENSE, class PV>
 __device__ __forceinline__ void part_fold_records(const PV &P, uint64_t *lkeys, uint64_t *lst, uint64_t keymask,
-                                                  const DFR_GLOBAL uint64_t *recs, uint64_t b0, uint64_t b1) {
+                                                  const DFR_GLOBAL uint64_t *recs, uint64_t b0, uint64_t b1,
+                                                  uint32_t *placed = nullptr) {
   for (uint64_t i0 = b0; i0 < b1; i0 += (uint64_t)U * blockDim.x) {
     RecTile<RW, U> t;
     load_recs<RW, U>(P, t, recs, i0, b1);
-    fold_recs<RW, U, DENSE>(P, lkeys, lst, keymask, t);
+    fold_recs<RW, U, DENSE>(P, lkeys, lst, keymask, t, placed);
   }
 }
 
@@ -812,8 +828,9 @@ __device__ __forceinline__ void part_fold_records(const PV &P, uint64_t *lkeys, 
 // table; a key the table cannot place goes to the HBM table, or, with DENSE
 // (no HBM table), raises the sub-partition capacity fault.
 template <bool DENSE, class SrcPtr, class PV>
-__device__ __forceinline__ void merge_one(const PV &P, uint64_t *lkeys, uint64_t *lst, uint64_t key, SrcPtr s) {
-  const int e = part_find(P, lkeys, key);
+__device__ __forceinline__ void merge_one(const PV &P, uint64_t *lkeys, uint64_t *lst, uint64_t key, SrcPtr s,
+                                          uint32_t *placed) {
+  const int e = part_find(P, lkeys, key, placed);
   if (e >= 0) {
     merge_full(P, lst + (int64_t)e * P.c_stride, s);
   } else if (DENSE) {
@@ -829,7 +846,8 @@ __device__ __forceinline__ void merge_one(const PV &P, uint64_t *lkeys, uint64_t
 // all at once first: one memory latency per record, not one per word.
 template <bool DENSE, class PV>
 __device__ __forceinline__ void merge_partials(const PV &P, uint64_t *lkeys, uint64_t *lst,
-                                               const DFR_GLOBAL uint64_t *par, uint64_t q0, uint64_t q1) {
+                                               const DFR_GLOBAL uint64_t *par, uint64_t q0, uint64_t q1,
+                                               uint32_t *placed = nullptr) {
   const int W = P.words;
   for (uint64_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
     const DFR_GLOBAL uint64_t *rec = par + i * (1 + W);
@@ -838,9 +856,9 @@ __device__ __forceinline__ void merge_partials(const PV &P, uint64_t *lkeys, uin
 #pragma unroll
       for (int w = 0; w <= kMaxWords; ++w)
         if (w <= W) r[w] = rec[w];
-      merge_one<DENSE>(P, lkeys, lst, r[0], (const uint64_t *)(r + 1));
+      merge_one<DENSE>(P, lkeys, lst, r[0], (const uint64_t *)(r + 1), placed);
     } else {
-      merge_one<DENSE>(P, lkeys, lst, rec[0], rec + 1);
+      merge_one<DENSE>(P, lkeys, lst, rec[0], rec + 1, placed);
     }
   }
 }
@@ -1082,7 +1100,7 @@ __device__ __forceinline__ void emit_group(const PV &P, uint64_t i, const uint64
 // its key and state there, or, fused, its output row. The first tile of the
 // next sub-partition is loaded while this one's groups are placed.
 template <int NA, int RW, int U, class PV>
-__device__ __forceinline__ void sub_aggregate_loop(const PV &P, uint32_t *wsum, uint64_t &gbase) {
+__device__ __forceinline__ void sub_aggregate_loop(const PV &P, uint32_t *wsum, uint64_t &gbase, uint32_t &placed) {
   extern __shared__ uint64_t smem[];
   const int E = P.c_entries, W = P.words, S = P.c_stride, CH = P.sub_chunks;
   uint64_t *lkeys = smem;
@@ -1118,6 +1136,7 @@ __device__ __forceinline__ void sub_aggregate_loop(const PV &P, uint32_t *wsum, 
       nq0 = P.base2_par[nx * CH];
       nq1 = P.base2_par[(nx + 1) * CH];
     }
+    if (threadIdx.x == 0) placed = 0;
     for (int i = threadIdx.x; i < E; i += blockDim.x) {
       lkeys[i] = kEmpty;
       if constexpr (RegPlan<PV>::value) {
@@ -1130,10 +1149,10 @@ __device__ __forceinline__ void sub_aggregate_loop(const PV &P, uint32_t *wsum, 
     }
     __syncthreads();
     if (b0 < b1) {
-      fold_recs<RW, U, true>(P, lkeys, lst, keymask, t);
-      if (b0 + tile < b1) part_fold_records<RW, U, true>(P, lkeys, lst, keymask, recs, b0 + tile, b1);
+      fold_recs<RW, U, true>(P, lkeys, lst, keymask, t, &placed);
+      if (b0 + tile < b1) part_fold_records<RW, U, true>(P, lkeys, lst, keymask, recs, b0 + tile, b1, &placed);
     }
-    merge_partials<true>(P, lkeys, lst, gp(P.part2_par), q0, q1);
+    merge_partials<true>(P, lkeys, lst, gp(P.part2_par), q0, q1, &placed);
     // and its first tile, in flight while this one's groups are placed
     if (nb0 < nb1) load_recs<RW, U>(P, t, recs, nb0, nb1);
     b0 = nb0;
@@ -1141,8 +1160,12 @@ __device__ __forceinline__ void sub_aggregate_loop(const PV &P, uint32_t *wsum, 
     q0 = nq0;
     q1 = nq1;
     __syncthreads();
-    // dense slots: per 64-entry chunk its occupied count, a prefix over the
-    // chunks (wave 0), one reservation for the sub-partition
+    // dense slots: one reservation for the sub-partition's groups (counted
+    // as their keys were placed), issued first so that it is in flight
+    // while each 64-entry chunk's occupied count and a prefix over the
+    // chunks (wave 0) give each group its place in the reservation
+    uint64_t reserved = 0;
+    if (threadIdx.x == 0 && placed) reserved = atomicAdd((unsigned long long *)P.gcount, (unsigned long long)placed);
     for (int c = wave; c < nchunks; c += nwaves) {
       const uint64_t ball = __ballot(lkeys[c * 64 + lane] != kEmpty);
       if (lane == 0) wsum[c] = (uint32_t)__popcll(ball);
@@ -1170,8 +1193,7 @@ __device__ __forceinline__ void sub_aggregate_loop(const PV &P, uint32_t *wsum, 
           run += x;
         }
       }
-      const uint32_t total = __shfl(inc, 63, 64);
-      if (lane == 0) gbase = total ? atomicAdd((unsigned long long *)P.gcount, (unsigned long long)total) : 0ull;
+      if (lane == 0) gbase = reserved;
     }
     __syncthreads();
     // the next tile has arrived by now: waiting for it here, before this
@@ -1208,12 +1230,13 @@ template <int NA = 0, class PV>
 __device__ __forceinline__ void sub_aggregate_body(const PV &P) {
   __shared__ uint32_t wsum[kMaxPartEntries / 64];
   __shared__ uint64_t gbase;
+  __shared__ uint32_t placed;
   if (P.rw <= 4)
-    sub_aggregate_loop<NA, 4, 2>(P, wsum, gbase);
+    sub_aggregate_loop<NA, 4, 2>(P, wsum, gbase, placed);
   else if (P.rw <= 8)
-    sub_aggregate_loop<NA, 8, 1>(P, wsum, gbase);
+    sub_aggregate_loop<NA, 8, 1>(P, wsum, gbase, placed);
   else
-    sub_aggregate_loop<NA, kMaxCols + 3, 1>(P, wsum, gbase);
+    sub_aggregate_loop<NA, kMaxCols + 3, 1>(P, wsum, gbase, placed);
 }
 
 #if !DFR_JIT
