@@ -26,6 +26,12 @@ DFR_INT64 = 2
 DFR_FLOAT64 = 3
 DFR_DICT = 4
 
+DFR_BUFFER_KEY = 0
+DFR_BUFFER_AGG = 1
+DFR_BUFFER_FIRST_ROWS = 2
+DFR_BUFFER_VALUES = 0
+DFR_BUFFER_VALIDITY = 1
+
 EQ, NE, LT, LE, GT, GE, IS_NULL, IS_NOT_NULL = range(8)
 LEAF_MOD = 1
 LEAF_NOT = 2
@@ -67,6 +73,9 @@ EXPORTS = [
     "dfr_result_device_ms",
     "dfr_result_phase_ms",
     "dfr_result_release",
+    "dfr_result_buffer",
+    "dfr_buffer_copy_to_host",
+    "dfr_buffer_release",
 ]
 
 
@@ -202,6 +211,10 @@ def load(required: bool = True):
     lib.dfr_result_phase_ms.argtypes = [vp, i32]
     lib.dfr_result_phase_ms.restype = ctypes.c_double
     lib.dfr_result_release.argtypes = [vp]
+    lib.dfr_result_buffer.argtypes = [vp, i32, i32, i32, ctypes.POINTER(vp), ctypes.POINTER(i64), errp]
+    lib.dfr_buffer_copy_to_host.argtypes = [vp, vp, i64]
+    lib.dfr_buffer_copy_to_host.restype = i64
+    lib.dfr_buffer_release.argtypes = [vp]
     _lib = lib
     return lib
 

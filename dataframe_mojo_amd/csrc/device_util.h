@@ -162,6 +162,86 @@ DFR_HD double i128_to_f64(uint64_t lo, uint64_t hi, int scale) {
   return neg ? -d : d;
 }
 
+// ---- fixed point wider than 128 bits (2 to 4 words; used when a column's
+// exponent range needs them, so kept simple rather than fast)
+
+// 64 bits of the little-endian word array src[0..ns) starting at bit `o`
+// (o may be negative or past the end: missing bits read as zero).
+DFR_HD uint64_t bits_at(const uint64_t *src, int ns, int o) {
+  if (o <= -64 || o >= 64 * ns) return 0;
+  const int q = o >> 6, r = o & 63;  // floor division
+  const uint64_t lo = (q >= 0 && q < ns) ? src[q] : 0ull;
+  const uint64_t hi = (q + 1 >= 0 && q + 1 < ns) ? src[q + 1] : 0ull;
+  return r ? ((lo >> r) | (hi << (64 - r))) : lo;
+}
+
+// Two's-complement negation of an N-word value.
+template <int N>
+DFR_HD void neg_words(uint64_t (&w)[N]) {
+  uint64_t c = 1;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint64_t t = ~w[i] + c;
+    c = (c && t == 0) ? 1ull : 0ull;
+    w[i] = t;
+  }
+}
+
+// Fixed-point image of a finite double in N words: trunc(|x| * 2^S) with
+// x's sign, two's complement. The caller's S keeps the magnitude below
+// 2^(64N - 1).
+template <int N>
+DFR_HD void fix_words(uint64_t bits, int S, uint64_t (&w)[N]) {
+  const int eb = (int)((bits >> 52) & 0x7ff);
+  uint64_t m[1] = {(bits & ((1ull << 52) - 1)) | (eb ? (1ull << 52) : 0ull)};
+  const int sh = (eb ? eb : 1) - 1075 + S;  // value = m * 2^(e - 1075)
+#pragma unroll
+  for (int i = 0; i < N; ++i) w[i] = bits_at(m, 1, 64 * i - sh);
+  if (bits >> 63) neg_words<N>(w);
+}
+
+// Fixed-point image of x*x in N words: trunc(x^2 * 2^S2), non-negative.
+template <int N>
+DFR_HD void fixsq_words(uint64_t bits, int S2, uint64_t (&w)[N]) {
+  const int eb = (int)((bits >> 52) & 0x7ff);
+  const uint64_t m = (bits & ((1ull << 52) - 1)) | (eb ? (1ull << 52) : 0ull);
+  uint64_t p[2];
+#if defined(__HIP_DEVICE_COMPILE__)
+  p[1] = __umul64hi(m, m);
+#else
+  p[1] = (uint64_t)(((unsigned __int128)m * m) >> 64);
+#endif
+  p[0] = m * m;
+  const int sh = 2 * ((eb ? eb : 1) - 1075) + S2;
+#pragma unroll
+  for (int i = 0; i < N; ++i) w[i] = bits_at(p, 2, 64 * i - sh);
+}
+
+// Correctly rounded value of the non-negative n-word x times 2^-scale
+// (round to odd into 64 bits, then one rounding to double).
+DFR_HD double words_to_f64(const uint64_t *x, int n, int scale) {
+  int top = n - 1;
+  while (top > 0 && x[top] == 0) --top;
+  if (top == 0) return ldexp(u64_to_f64(x[0]), -scale);
+  const int lz = __builtin_clzll(x[top]);
+  const uint64_t hi = x[top], lo = x[top - 1];
+  uint64_t t = lz ? ((hi << lz) | (lo >> (64 - lz))) : hi;
+  uint64_t sticky = lz ? (lo << lz) : lo;
+  for (int i = top - 2; i >= 0; --i) sticky |= x[i];
+  t |= sticky ? 1 : 0;
+  return ldexp(u64_to_f64(t), 64 * top - lz - scale);
+}
+
+// Correctly rounded value of the two's-complement n-word x times 2^-scale.
+DFR_HD double sword_to_f64(const uint64_t *x, int n, int scale) {
+  uint64_t a[4] = {0, 0, 0, 0};
+  for (int i = 0; i < n && i < 4; ++i) a[i] = x[i];
+  const bool neg = (int64_t)a[n - 1] < 0;
+  if (neg) neg_words<4>(a);
+  const double d = words_to_f64(a, n, scale);
+  return neg ? -d : d;
+}
+
 // 256-bit unsigned arithmetic for the variance finish.
 // q = n / d, r = n % d for d >= 1, given inv = 1 / d rounded to double.
 // Below 2^53 the double estimate of n / d is within one of the quotient (n
@@ -269,6 +349,88 @@ DFR_HD double u256_to_f64(U256 x, int scale) {
   // 2^(64*top - lz)
   int exp = 64 * top - lz;
   return ldexp(u64_to_f64(t), exp - scale);
+}
+
+// ---- 512-bit unsigned arithmetic for the variance finish of accumulators
+// wider than 128 bits (little-endian words).
+struct U512 {
+  uint64_t w[8];
+};
+
+// a (na words) * b (nb words), na + nb <= 8.
+DFR_HD U512 mul_words(const uint64_t *a, int na, const uint64_t *b, int nb) {
+  U512 r;
+  for (int i = 0; i < 8; ++i) r.w[i] = 0;
+  for (int i = 0; i < na; ++i) {
+    uint64_t carry = 0;
+    for (int j = 0; j < nb; ++j) {
+      uint64_t plo, phi;
+      mul64(a[i], b[j], plo, phi);
+      const uint64_t t = r.w[i + j] + plo;
+      const uint64_t c1 = t < plo ? 1 : 0;
+      const uint64_t t2 = t + carry;
+      const uint64_t c2 = t2 < t ? 1 : 0;
+      r.w[i + j] = t2;
+      carry = phi + c1 + c2;
+    }
+    for (int k = i + nb; k < 8 && carry; ++k) {
+      const uint64_t t = r.w[k] + carry;
+      carry = t < carry ? 1 : 0;
+      r.w[k] = t;
+    }
+  }
+  return r;
+}
+
+// x << s for s >= 0 (bits past 512 are dropped).
+DFR_HD U512 shl512(const U512 &x, int s) {
+  U512 r;
+  for (int i = 0; i < 8; ++i) r.w[i] = bits_at(x.w, 8, 64 * i - s);
+  return r;
+}
+
+// a - b; returns false when the difference is negative (then r = 0).
+DFR_HD bool sub512(const U512 &a, const U512 &b, U512 &r) {
+  uint64_t borrow = 0;
+  for (int i = 0; i < 8; ++i) {
+    const uint64_t d = a.w[i] - b.w[i];
+    const uint64_t b1 = a.w[i] < b.w[i] ? 1 : 0;
+    const uint64_t d2 = d - borrow;
+    const uint64_t b2 = d < borrow ? 1 : 0;
+    r.w[i] = d2;
+    borrow = b1 | b2;
+  }
+  if (borrow) {
+    for (int i = 0; i < 8; ++i) r.w[i] = 0;
+    return false;
+  }
+  return true;
+}
+
+// n * sum(x^2) and sum(x)^2 of a fixed-point column block (sum: nf words
+// at scale S, two's complement; sum of squares: nq words at scale S2),
+// brought to one scale, subtracted exactly, and rounded once:
+// (n * Q - T^2) / (n * (n - ddof)) as a double. Both products stay below
+// 2^508 for the widths and scales the host chooses (at most 4 words each).
+DFR_HD double var_words(const uint64_t *sum, int nf, int S, const uint64_t *sq, int nq, int S2, uint64_t n,
+                        uint64_t ddof) {
+  uint64_t t[4] = {0, 0, 0, 0};
+  for (int i = 0; i < nf; ++i) t[i] = sum[i];
+  if ((int64_t)t[nf - 1] < 0) {
+    // |T| (sign-extended to 4 words first)
+    for (int i = nf; i < 4; ++i) t[i] = ~0ull;
+    neg_words<4>(t);
+  }
+  const U512 nq_ = mul_words(sq, nq, &n, 1);
+  const U512 t2 = mul_words(t, 4, t, 4);
+  const int U = (2 * S > S2) ? 2 * S : S2;  // the common scale 2^-U
+  const U512 a = shl512(nq_, U - S2);
+  const U512 b = shl512(t2, U - 2 * S);
+  U512 r;
+  sub512(a, b, r);
+  const double num = words_to_f64(r.w, 8, U);
+  const double den = (double)n * (double)(n - ddof);
+  return num / den;
 }
 
 }  // namespace dfr

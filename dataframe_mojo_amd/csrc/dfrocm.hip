@@ -235,6 +235,23 @@ extern "C" int dfr_context_create(int32_t device, int64_t budget_bytes, dfr_cont
   return DFR_OK;
 }
 
+void context_hold(dfr_result *r) {
+  if (!r || r->holds_ctx) return;
+  std::lock_guard<std::mutex> g(r->ctx->hold_mu);
+  r->ctx->holds++;
+  r->holds_ctx = true;
+}
+
+void context_retire(dfr_context *ctx) {
+  bool now;
+  {
+    std::lock_guard<std::mutex> g(ctx->hold_mu);
+    ctx->retired = true;
+    now = ctx->holds == 0;
+  }
+  if (now) dfr_context_destroy(ctx);
+}
+
 extern "C" void dfr_context_destroy(dfr_context *ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
@@ -447,6 +464,7 @@ extern "C" int dfr_column_slice(const dfr_column *parent, int64_t offset, int64_
   // the parent's statistics bound the slice's (range, exponents); its null
   // count is an upper bound, so a slice of a nullable column stays nullable
   col->stats = parent->stats;
+  col->stats_window = 0;
   col->nulls = parent->nulls;
   *out = col;
   return DFR_OK;
@@ -528,8 +546,70 @@ extern "C" double dfr_result_phase_ms(const dfr_result *r, int32_t phase) {
   return r->phase_ms[phase];
 }
 
+extern "C" int dfr_result_buffer(dfr_result *r, int32_t kind, int32_t index, int32_t part, dfr_buffer **out,
+                                 int64_t *bytes, char **error) {
+  if (!r || !out) return dfr_fail(error, DFR_INVALID, "null argument");
+  const void *dev = nullptr;
+  int64_t n = 0;
+  const int64_t g = r->groups;
+  if (kind == DFR_BUFFER_KEY || kind == DFR_BUFFER_AGG) {
+    const std::vector<int32_t> &dt = kind == DFR_BUFFER_KEY ? r->key_dtypes : r->agg_dtypes;
+    if (index < 0 || index >= (int32_t)dt.size()) return dfr_fail(error, DFR_INVALID, "buffer index out of range");
+    if (part == DFR_BUFFER_VALUES) {
+      dev = kind == DFR_BUFFER_KEY ? r->key_values[index] : r->agg_values[index];
+      n = g * dtype_width(dt[index]);
+    } else if (part == DFR_BUFFER_VALIDITY) {
+      dev = kind == DFR_BUFFER_KEY ? (const void *)r->key_valid[index] : (const void *)r->agg_valid[index];
+      n = (g + 7) / 8;
+    } else {
+      return dfr_fail(error, DFR_INVALID, "unknown buffer part %d", part);
+    }
+  } else if (kind == DFR_BUFFER_FIRST_ROWS) {
+    if (!r->first_rows) return dfr_fail(error, DFR_UNSUPPORTED, "first rows are kept only under maintain_order");
+    dev = r->first_rows;
+    n = g * 8;
+  } else {
+    return dfr_fail(error, DFR_INVALID, "unknown buffer kind %d", kind);
+  }
+  dfr_buffer *b = new dfr_buffer();
+  b->result = r;
+  b->device = dev;
+  b->bytes = n;
+  r->refs.fetch_add(1);
+  *out = b;
+  if (bytes) *bytes = n;
+  return DFR_OK;
+}
+
+extern "C" int64_t dfr_buffer_copy_to_host(void *buffer, void *host_dst, int64_t bytes) {
+  dfr_buffer *b = (dfr_buffer *)buffer;
+  if (!b || bytes < 0 || bytes > b->bytes || (bytes > 0 && !host_dst)) return DFR_INVALID;
+  if (bytes == 0) return DFR_OK;
+  dfr_context *ctx = b->result->ctx;
+  if (hipSetDevice(ctx->device) != hipSuccess) return DFR_DEVICE;
+  if (hipMemcpy(host_dst, b->device, (size_t)bytes, hipMemcpyDeviceToHost) != hipSuccess) return DFR_DEVICE;
+  return DFR_OK;
+}
+
+extern "C" void dfr_buffer_release(void *buffer) {
+  dfr_buffer *b = (dfr_buffer *)buffer;
+  if (!b) return;
+  dfr_result_release(b->result);
+  delete b;
+}
+
 extern "C" void dfr_result_release(dfr_result *r) {
   if (!r) return;
-  if (r->buffer) r->ctx->pool.release(r->buffer);
-  delete r;
+  if (r->refs.fetch_sub(1) != 1) return;  // a buffer of it is still held
+  dfr_context *ctx = r->ctx;
+  const bool held = r->holds_ctx;
+  if (r->buffer) ctx->pool.release(r->buffer);
+  delete r;  // (a batch's shared block goes back to the pool with its last result)
+  if (!held) return;
+  bool last;
+  {
+    std::lock_guard<std::mutex> g(ctx->hold_mu);
+    last = --ctx->holds == 0 && ctx->retired;
+  }
+  if (last) dfr_context_destroy(ctx);
 }

@@ -45,6 +45,90 @@ int env_int(const char *name, int dflt) {
   return v && *v ? atoi(v) : dflt;
 }
 
+// Bounds of a column's values for its fixed-point sums: every value has
+// |x| < 2^B and is a multiple of 2^lb. Integers (whose variance goes
+// through doubles) are multiples of 1; a double of an integer below 2^b can
+// round up to 2^b, hence one more bit.
+void fix_bounds(const ColStats &st, int dtype, int &B, int &lb) {
+  if (dtype == DFR_FLOAT64) {
+    if (!(st.flags & 32u)) {  // zeros and non-finite values only
+      B = 0;
+      lb = 0;
+      return;
+    }
+    B = (int)st.emax - 1022;
+    lb = (int)st.emin - 1075;
+    return;
+  }
+  lb = 0;
+  if (st.imin > st.imax) {  // no valid value
+    B = 0;
+    return;
+  }
+  const int64_t lo = st.imin, hi = st.imax;
+  const uint64_t a = std::max(lo < 0 ? (uint64_t)(-(lo + 1)) + 1 : (uint64_t)lo,
+                              hi < 0 ? (uint64_t)(-(hi + 1)) + 1 : (uint64_t)hi);
+  B = bitlen_u64(a) + 1;
+}
+
+// Widths (2 to 4 words) and scales of a column block's fixed-point sum and
+// sum of squares for `rows` values with bounds (B, lb). With T = 64 * words,
+// the scale S = T - 2 - B - L (L = bit length of the row count) keeps every
+// partial sum below 2^(T - 2), so nothing overflows.
+//   - A sum is exact when S + lb >= 0 (every value is a whole number of
+//     units 2^-S). The narrowest exact width is taken; when none up to 4
+//     words is exact, 4 words are taken if the truncation bound
+//     rows * 2^-S <= 2^-46 holds.
+//   - A variance (n * Q - T^2) / (n (n - ddof)), computed exactly from the
+//     two fixed-point totals, is off by at most
+//     (ddof + 1) * (2^-S2 + 2^(B + 1 - S) + 2^-2S) (each value truncated
+//     once per total; 2^(bitlen(ddof) + 2 + max(-S2, B + 1 - S)) bounds it).
+//     The narrowest widths for which it is exact or that bound is at most
+//     2^-84 are taken: its square root (std) is then within 2^-42.
+// Both bounds are far inside the parity budget |a - e| <= 1e-12 * max(1, |e|).
+// Returns false when no width fits.
+bool choose_fixed(int B, int lb, int64_t rows, bool need_sum, bool need_sq, int ddof_max, int &nf, int &nq,
+                  int &S, int &S2) {
+  const int L = std::max(1, bitlen_u64((uint64_t)rows));
+  const int dbits = bitlen_u64((uint64_t)std::max(ddof_max, 0));
+  int exact_nf = 0;
+  for (int w = 2; w <= 4 && !exact_nf; ++w)
+    if (64 * w - 2 - B - L + lb >= 0) exact_nf = w;
+  int best = 99;
+  for (int f = 2; f <= 4; ++f) {
+    const int s = 64 * f - 2 - B - L;
+    const bool sum_exact = s + lb >= 0;
+    if (need_sum) {
+      const bool ok = exact_nf ? f >= exact_nf : (f == 4 && L - s <= -46);
+      if (!ok) continue;
+    }
+    if (!need_sq) {
+      if (f < best) {
+        best = f;
+        nf = f;
+        nq = 2;
+        S = s;
+        S2 = 0;
+      }
+      continue;
+    }
+    for (int q = 2; q <= 4; ++q) {
+      const int s2 = 64 * q - 2 - 2 * B - L;
+      const bool exact = sum_exact && s2 + 2 * lb >= 0;
+      const int log2e = dbits + 2 + std::max(-s2, B + 1 - s);
+      if (!(exact || log2e <= -84)) continue;
+      if (f + q < best) {
+        best = f + q;
+        nf = f;
+        nq = q;
+        S = s;
+        S2 = s2;
+      }
+    }
+  }
+  return best != 99;
+}
+
 }  // namespace
 
 Tuning default_tuning(dfr_context *ctx) {
@@ -77,6 +161,84 @@ Tuning default_tuning(dfr_context *ctx) {
   t.scan_lds_bytes = std::min(env_int("DFR_SCAN_LDS_BYTES", 159744), ctx->lds_per_block);
   return t;
 }
+
+namespace {
+
+ColStats stats_identity() {
+  ColStats init;
+  init.imin = INT64_MAX;
+  init.imax = INT64_MIN;
+  init.emin = 0x7ff;
+  init.emax = 0;
+  init.flags = 0;
+  init.pad = 0;
+  init.valid = 0;
+  return init;
+}
+
+// Statistics of rows [0, col->length) of the column's window, computed on
+// the context's device into *out (the column is left as it is: a slice
+// keeps its parent's statistics, which other threads may be reading).
+int window_stats(dfr_context *ctx, const dfr_column *col, ColStats *out, char **error) {
+  ColStats *d = nullptr;
+  int st = ctx->pool.alloc(sizeof(ColStats), (void **)&d, error);
+  if (st) return st;
+  ColStats *h = (ColStats *)ctx->staging;
+  *h = stats_identity();
+  hipError_t e = hipMemcpyAsync(d, h, sizeof(ColStats), hipMemcpyHostToDevice, ctx->stream);
+  const int64_t n = col->length;
+  const int blocks = (int)std::min<int64_t>((n + 16 * kBlock - 1) / (16 * kBlock), (int64_t)ctx->cus * 2);
+  if (e == hipSuccess && blocks > 0) {
+    hipLaunchKernelGGL(k_col_stats, dim3(blocks), dim3(kBlock), 0, ctx->stream, col->dev(), n, d);
+    ctx->obs.launches++;
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(ColStats), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->obs.waits++;
+  ctx->pool.release(d);
+  if (e != hipSuccess) return dfr_fail(error, DFR_DEVICE, "window statistics: %s", hipGetErrorString(e));
+  *out = *h;
+  return DFR_OK;
+}
+
+// Statistics of the values in the rows the plan's predicate keeps, for the
+// plan columns of `cols`, into st[c] (the other entries are left alone).
+int kept_stats(dfr_context *ctx, const DevPlan &P, uint32_t cols, ColStats *st, char **error) {
+  const size_t plan_bytes = (sizeof(DevPlan) + 255) / 256 * 256;
+  const size_t stat_bytes = sizeof(ColStats) * kMaxCols;
+  void *d = nullptr;
+  int rc = ctx->pool.alloc(plan_bytes + stat_bytes, &d, error);
+  if (rc) return rc;
+  char *h = ctx->staging;
+  memcpy(h, &P, sizeof(DevPlan));
+  ColStats *hs = (ColStats *)(h + plan_bytes);
+  for (int c = 0; c < kMaxCols; ++c) hs[c] = stats_identity();
+  const DevPlan *dplan = (const DevPlan *)d;
+  ColStats *ds = (ColStats *)((char *)d + plan_bytes);
+  hipError_t e = hipMemcpyAsync(d, h, plan_bytes + stat_bytes, hipMemcpyHostToDevice, ctx->stream);
+  const int64_t tiles = (P.rows + 2 * kBlock - 1) / (2 * kBlock);
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ctx->cus * 4));
+  if (e == hipSuccess && P.rows > 0) {
+    if (P.n_cols <= 4)
+      hipLaunchKernelGGL(k_kept_stats<4>, dim3(blocks), dim3(kBlock), 0, ctx->stream, dplan, cols, ds);
+    else if (P.n_cols <= 8)
+      hipLaunchKernelGGL(k_kept_stats<8>, dim3(blocks), dim3(kBlock), 0, ctx->stream, dplan, cols, ds);
+    else
+      hipLaunchKernelGGL(k_kept_stats<kMaxCols>, dim3(blocks), dim3(kBlock), 0, ctx->stream, dplan, cols, ds);
+    ctx->obs.launches++;
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(hs, ds, stat_bytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->obs.waits++;
+  ctx->pool.release(d);
+  if (e != hipSuccess) return dfr_fail(error, DFR_DEVICE, "kept-row statistics: %s", hipGetErrorString(e));
+  for (int c = 0; c < P.n_cols; ++c)
+    if ((cols >> c) & 1u) st[c] = hs[c];
+  return DFR_OK;
+}
+
+}  // namespace
 
 int column_stats(dfr_context *ctx, dfr_column *col, char **error) {
   ColStats init;
@@ -257,6 +419,8 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
   // aggregates -> per-column blocks
   uint32_t mask[kMaxCols] = {0};
   int int_as_float[kMaxCols] = {0};
+  bool want_fsum[kMaxCols] = {false};  // a Float64 SUM or MEAN reads the column's fixed-point sum
+  int ddof_max[kMaxCols] = {0};
   P.n_aggs = q->n_aggs;
   C.agg_dtypes.clear();
   for (int a = 0; a < q->n_aggs; ++a) {
@@ -301,6 +465,7 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
         break;
       case DFR_AGG_SUM:
         if (fl) {
+          want_fsum[c] = true;
           mask[c] |= M_FSUM | (nonfinite ? M_FLAGS : 0);
           out = T_FLOAT64;
         } else {
@@ -322,14 +487,16 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
         break;
       }
       case DFR_AGG_MEAN:
-        if (fl)
+        if (fl) {
+          want_fsum[c] = true;
           mask[c] |= M_FSUM | (nonfinite ? M_FLAGS : 0);
-        else
+        } else
           mask[c] |= narrow ? M_ISUM64 : M_ISUM128;
         out = T_FLOAT64;
         break;
       default:  // VAR, STD
         if (G.ddof < 0) return dfr_fail(error, DFR_INVALID, "aggregate %d: negative ddof", a);
+        ddof_max[c] = std::max(ddof_max[c], (int)G.ddof);
         mask[c] |= M_FSUM | M_FSUMSQ | (nonfinite ? M_FLAGS : 0);
         if (!fl) int_as_float[c] = 1;
         out = T_FLOAT64;
@@ -337,6 +504,50 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
     }
     D.out_dtype = out;
     C.agg_dtypes.push_back(out);
+  }
+
+  // fixed-point widths and scales of the columns' Float64 sums, from the
+  // column statistics; for a slice (whose statistics are its parent's),
+  // then from its own rows; then from the rows the predicate keeps; and
+  // when still no width fits, the query is not run on the device
+  int fnf[kMaxCols] = {0}, fnq[kMaxCols] = {0}, fS[kMaxCols] = {0}, fS2[kMaxCols] = {0};
+  {
+    ColStats st[kMaxCols];
+    for (int c = 0; c < P.n_cols; ++c) st[c] = C.cols[c]->stats;
+    auto settle = [&](uint32_t &bad) {
+      bad = 0;
+      for (int c = 0; c < P.n_cols; ++c) {
+        if (!(mask[c] & (M_FSUM | M_FSUMSQ))) continue;
+        int B, lb;
+        fix_bounds(st[c], C.cols[c]->dtype, B, lb);
+        if (!choose_fixed(B, lb, rows, want_fsum[c], (mask[c] & M_FSUMSQ) != 0, ddof_max[c], fnf[c], fnq[c], fS[c],
+                          fS2[c]))
+          bad |= 1u << c;
+      }
+    };
+    uint32_t bad;
+    settle(bad);
+    if (bad) {
+      for (int c = 0; c < P.n_cols; ++c) {
+        if (!((bad >> c) & 1u) || C.cols[c]->stats_window) continue;
+        int st2 = window_stats(ctx, C.cols[c], &st[c], error);
+        if (st2) return st2;
+      }
+      settle(bad);
+    }
+    if (bad) {
+      int st2 = kept_stats(ctx, P, bad, st, error);
+      if (st2) return st2;
+      settle(bad);
+    }
+    if (bad) {
+      int c = 0;
+      while (!((bad >> c) & 1u)) ++c;
+      return dfr_fail(error, DFR_UNSUPPORTED,
+                      "the Float64 values of input column %d span too wide a range for an exact device %s over %lld "
+                      "rows",
+                      c, (mask[c] & M_FSUMSQ) ? "variance" : "sum", (long long)rows);
+    }
   }
 
   // slot layout
@@ -362,15 +573,16 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
     A.o_count = A.o_isum = A.o_fsum = A.o_sq = A.o_min = A.o_max = A.o_flags = A.o_zfirst = -1;
     A.mask = mask[c];
     if (!mask[c]) continue;
-    if (w + 14 > kMaxWords) return dfr_fail(error, DFR_UNSUPPORTED, "too many aggregate words");
+    if (w + 16 > kMaxWords) return dfr_fail(error, DFR_UNSUPPORTED, "too many aggregate words");
     A.gbase = (int16_t)w;
     A.lbase = (int16_t)lw;
     int o = 0;
     auto put = [&](int8_t &field, int n, uint64_t id, bool lds, int8_t op) {
       field = (int8_t)o;
+      const int8_t low = n == 2 ? W_ADD128 : (n == 3 ? W_ADD192 : W_ADD256);
       for (int i = 0; i < n; ++i) {
         ident[w] = id;
-        wop[w] = (n == 2) ? (i == 0 ? W_ADD128 : W_HI) : op;
+        wop[w] = (n >= 2) ? (i == 0 ? low : W_HI) : op;
         if (lds) l2g[lw++] = w;
         ++w;
         ++o;
@@ -379,29 +591,19 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
     if (mask[c] & M_VCOUNT) put(A.o_count, 1, 0, true, W_ADD);
     if (mask[c] & M_ISUM64) put(A.o_isum, 1, 0, true, W_ADD);
     if (mask[c] & M_ISUM128) put(A.o_isum, 2, 0, true, W_ADD128);
-    if (mask[c] & M_FSUM) put(A.o_fsum, 2, 0, true, W_ADD128);
-    if (mask[c] & M_FSUMSQ) put(A.o_sq, 2, 0, true, W_ADD128);
+    if (mask[c] & M_FSUM) put(A.o_fsum, fnf[c], 0, true, W_ADD128);
+    if (mask[c] & M_FSUMSQ) put(A.o_sq, fnq[c], 0, true, W_ADD128);
     if (mask[c] & M_IMIN) put(A.o_min, 1, (uint64_t)INT64_MAX, true, W_MIN_S);
     if (mask[c] & M_IMAX) put(A.o_max, 1, (uint64_t)INT64_MIN, true, W_MAX_S);
     if (mask[c] & M_FMIN) put(A.o_min, 1, kEmpty, true, W_MIN_U);
     if (mask[c] & M_FMAX) put(A.o_max, 1, 0, true, W_MAX_U);
     if (mask[c] & M_FLAGS) put(A.o_flags, 1, 0, false, W_OR);
     if (mask[c] & M_ZFIRST) put(A.o_zfirst, 1, kEmpty, false, W_MIN_U);
-    // fixed-point scales: N values below 2^B each sum below 2^126
     if (mask[c] & (M_FSUM | M_FSUMSQ)) {
-      dfr_column *col = C.cols[c];
-      int L = std::max(1, bitlen_u64((uint64_t)rows));
-      int B;
-      if (col->dtype == DFR_FLOAT64) {
-        B = (col->stats.flags & 32u) ? (int)col->stats.emax - 1023 + 1 : 0;
-      } else {
-        int64_t lo = col->stats.imin, hi = col->stats.imax;
-        uint64_t a = std::max(lo < 0 ? (uint64_t)(-(lo + 1)) + 1 : (uint64_t)lo,
-                              hi < 0 ? (uint64_t)(-(hi + 1)) + 1 : (uint64_t)hi);
-        B = col->stats.valid ? bitlen_u64(a) + 1 : 0;
-      }
-      A.fshift = 126 - B - L;
-      A.fshift2 = 126 - 2 * B - L;
+      A.fshift = fS[c];
+      A.fshift2 = fS2[c];
+      A.nfw = (int8_t)fnf[c];
+      A.nqw = (int8_t)((mask[c] & M_FSUMSQ) ? fnq[c] : 0);
       A.int_as_float = int_as_float[c];
     }
   }

@@ -15,6 +15,7 @@
 
 #include <dlfcn.h>
 
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
@@ -36,11 +37,13 @@ struct JitEntry {
   hipModule_t module = nullptr;
   hipFunction_t fn = nullptr;
   bool failed = false;
+  bool compiling = false;  // another thread is compiling it (outside the lock)
 };
 
 const char *const kJitNames[3] = {"dfr_scan_jit", "dfr_part_jit", "dfr_sub_jit"};  // JIT_SCAN, JIT_PART, JIT_SUB
 
 std::mutex g_mu;
+std::condition_variable g_cv;  // a compile finished
 std::unordered_map<std::string, JitEntry> g_cache;
 
 // The directory of the kernel headers: csrc/ beside this library.
@@ -125,7 +128,8 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF, int 
     o << "  L.cols[" << c << "].dtype = " << P.cols[c].dtype << "; L.cols[" << c << "].is_float = " << P.cols[c].is_float
       << "; L.val_of[" << c << "] = " << (int)P.val_of[c] << ";\n"
       << "  L.cagg[" << c << "].mask = " << A.mask << "u; L.cagg[" << c << "].gbase = " << A.gbase << "; L.cagg[" << c
-      << "].lbase = " << A.lbase << "; L.cagg[" << c << "].int_as_float = " << A.int_as_float << ";\n"
+      << "].lbase = " << A.lbase << "; L.cagg[" << c << "].int_as_float = " << A.int_as_float << "; L.cagg[" << c
+      << "].nfw = " << (int)A.nfw << "; L.cagg[" << c << "].nqw = " << (int)A.nqw << ";\n"
       << "  L.cagg[" << c << "].o_count = " << (int)A.o_count << "; L.cagg[" << c << "].o_isum = " << (int)A.o_isum
       << "; L.cagg[" << c << "].o_fsum = " << (int)A.o_fsum << "; L.cagg[" << c << "].o_sq = " << (int)A.o_sq
       << "; L.cagg[" << c << "].o_min = " << (int)A.o_min << "; L.cagg[" << c << "].o_max = " << (int)A.o_max
@@ -206,24 +210,38 @@ JitEntry compile(const std::string &src, int device, int which) {
 hipFunction_t jit_function(dfr_context *ctx, const DevPlan &P, int LDSM, int NC, int U, int which) {
   const std::string src = shape_source(P, LDSM, NC, U, ctx->tune.scan_prefetch, which);
   const std::string key = std::to_string(ctx->device) + "\n" + src;
-  std::lock_guard<std::mutex> g(g_mu);
-  auto it = g_cache.find(key);
-  if (it == g_cache.end()) {
-    if (!ctx->tune.jit_wait) return nullptr;
-    hipSetDevice(ctx->device);
-    if (const char *dir = getenv("DFR_JIT_DUMP")) {
-      // the generated source, for inspection
-      const std::string path =
-          std::string(dir) + "/" + kJitNames[which] + "_" + std::to_string(std::hash<std::string>()(src)) + ".hip";
-      if (FILE *f = fopen(path.c_str(), "w")) {
-        fputs(src.c_str(), f);
-        fclose(f);
-      }
+  {
+    // the lock guards the cache only: a compile (seconds) runs outside it,
+    // so a caller that does not wait for compiles is never held up by one
+    std::unique_lock<std::mutex> g(g_mu);
+    for (;;) {
+      auto it = g_cache.find(key);
+      if (it == g_cache.end()) break;
+      if (!it->second.compiling) return it->second.failed ? nullptr : it->second.fn;
+      if (!ctx->tune.jit_wait) return nullptr;
+      g_cv.wait(g);  // the same shape is being compiled by another thread
     }
-    it = g_cache.emplace(key, compile(src, ctx->device, which)).first;
-    ctx->obs.jit_compiles++;
+    if (!ctx->tune.jit_wait) return nullptr;
+    g_cache[key].compiling = true;
   }
-  return it->second.failed ? nullptr : it->second.fn;
+  hipSetDevice(ctx->device);
+  if (const char *dir = getenv("DFR_JIT_DUMP")) {
+    // the generated source, for inspection
+    const std::string path =
+        std::string(dir) + "/" + kJitNames[which] + "_" + std::to_string(std::hash<std::string>()(src)) + ".hip";
+    if (FILE *f = fopen(path.c_str(), "w")) {
+      fputs(src.c_str(), f);
+      fclose(f);
+    }
+  }
+  const JitEntry e = compile(src, ctx->device, which);
+  ctx->obs.jit_compiles++;
+  {
+    std::lock_guard<std::mutex> g(g_mu);
+    g_cache[key] = e;
+  }
+  g_cv.notify_all();
+  return e.failed ? nullptr : e.fn;
 }
 
 // Whether jit_function has a compiled kernel for this shape already.
@@ -232,7 +250,7 @@ bool jit_compiled(dfr_context *ctx, const DevPlan &P, int LDSM, int NC, int U, i
       std::to_string(ctx->device) + "\n" + shape_source(P, LDSM, NC, U, ctx->tune.scan_prefetch, which);
   std::lock_guard<std::mutex> g(g_mu);
   auto it = g_cache.find(key);
-  return it != g_cache.end() && !it->second.failed;
+  return it != g_cache.end() && !it->second.failed && !it->second.compiling;
 }
 
 // The generated source of a plan's pass A (for tests and inspection).

@@ -202,6 +202,50 @@ __device__ __forceinline__ void add128(uint64_t *p, uint64_t lo, uint64_t hi) {
   if (h) atomicAdd((unsigned long long *)(p + 1), (unsigned long long)h);
 }
 
+// An n-word add (n <= 4) by atomics, word by word from the lowest: each
+// word's returned old value gives its carry into the next. Whatever the
+// interleaving with other adds, the final words are the exact sum modulo
+// 2^(64n): word i + 1 receives one carry for every wrap of word i.
+__device__ __forceinline__ void add_words(uint64_t *p, const uint64_t *w, int n) {
+  uint64_t carry = 0;
+  for (int i = 0; i < n; ++i) {
+    const uint64_t x = w[i] + carry;
+    const uint64_t c1 = x < carry ? 1ull : 0ull;  // w[i] all ones plus a carry
+    if (i == n - 1) {
+      if (x) atomicAdd((unsigned long long *)(p + i), (unsigned long long)x);
+      break;
+    }
+    uint64_t c2 = 0;
+    if (x) {
+      const uint64_t old = atomicAdd((unsigned long long *)(p + i), (unsigned long long)x);
+      c2 = (old + x) < old ? 1ull : 0ull;
+    }
+    carry = c1 | c2;
+  }
+}
+
+// Folds one finite value's fixed-point images into the sum (nfw words at
+// `fsum`, scale S) or the sum of squares (nqw words at `sq`, scale S2) of a
+// column block, for those wider than 128 bits (0: not folded here).
+__device__ __noinline__ void add_wide_at(uint64_t *fsum, int nfw, int S, uint64_t *sq, int nqw, int S2,
+                                         uint64_t bits) {
+  uint64_t w[4];
+  if (nfw > 2) {
+    fix_words<4>(bits, S, w);
+    add_words(fsum, w, nfw);
+  }
+  if (nqw > 2) {
+    fixsq_words<4>(bits, S2, w);
+    add_words(sq, w, nqw);
+  }
+}
+
+__device__ __forceinline__ void add_wide(uint64_t *b, const DevColAgg &A, uint64_t bits) {
+  const uint32_t m = A.mask;
+  add_wide_at(b + A.o_fsum, (m & M_FSUM) ? A.nfw : 0, A.fshift, b + A.o_sq, (m & M_FSUMSQ) ? A.nqw : 0,
+              A.fshift2, bits);
+}
+
 // ------------------------------------------------------------ finalize
 
 struct AggOut {
@@ -213,12 +257,15 @@ __device__ __forceinline__ double finish_fsum(const DevColAgg &A, const uint64_t
   if ((flags & FL_NAN) || ((flags & FL_PINF) && (flags & FL_NINF))) return __builtin_nan("");
   if (flags & FL_PINF) return __builtin_inf();
   if (flags & FL_NINF) return -__builtin_inf();
+  if (A.nfw > 2) return sword_to_f64(b + A.o_fsum, A.nfw, A.fshift);
   return i128_to_f64(b[A.o_fsum], b[A.o_fsum + 1], A.fshift);
 }
 
 __device__ __forceinline__ double finish_var(const DevColAgg &A, const uint64_t *b, uint64_t n,
                                              int ddof, uint64_t flags) {
   if (flags & (FL_NAN | FL_PINF | FL_NINF)) return __builtin_nan("");
+  if (A.nfw > 2 || A.nqw > 2)
+    return var_words(b + A.o_fsum, A.nfw, A.fshift, b + A.o_sq, A.nqw, A.fshift2, n, (uint64_t)ddof);
   // n * sumsq * 2^(2S - S2) - sum^2, exactly, in units of 2^-2S
   uint64_t qlo = b[A.o_sq], qhi = b[A.o_sq + 1];
   uint64_t tlo = b[A.o_fsum], thi = b[A.o_fsum + 1];
@@ -263,6 +310,8 @@ __device__ __forceinline__ void col_agg_of(const PV &P, int col, DevColAgg &A, b
       A.fshift = hit ? P.cagg[c].fshift : A.fshift;
       A.fshift2 = hit ? P.cagg[c].fshift2 : A.fshift2;
       A.int_as_float = hit ? P.cagg[c].int_as_float : A.int_as_float;
+      A.nfw = hit ? P.cagg[c].nfw : A.nfw;
+      A.nqw = hit ? P.cagg[c].nqw : A.nqw;
       is_float = hit ? (P.cols[c].is_float != 0) : is_float;
     }
   } else {
@@ -601,16 +650,17 @@ __device__ __forceinline__ void fold_row(PlanRef P, uint64_t *st, int64_t row,
         uint64_t fl = nan ? FL_NAN : ((bits >> 63) ? FL_NINF : FL_PINF);
         if (m & M_FLAGS) rare_update(P, packed, A.gbase + A.o_flags, fl, false);
       } else {
-        if (m & M_FSUM) {
+        if ((m & M_FSUM) && A.nfw == 2) {
           uint64_t lo, hi;
           fix_of(bits, A.fshift, lo, hi);
           add128(&b[A.o_fsum], lo, hi);
         }
-        if (m & M_FSUMSQ) {
+        if ((m & M_FSUMSQ) && A.nqw == 2) {
           uint64_t lo, hi;
           fixsq_of(bits, A.fshift2, lo, hi);
           add128(&b[A.o_sq], lo, hi);
         }
+        if (A.nfw > 2 || A.nqw > 2) add_wide(b, A, bits);
         if ((m & M_ZFIRST) && (bits << 1) == 0)
           rare_update(P, packed, A.gbase + A.o_zfirst, ((uint64_t)row << 1) | (bits >> 63), true);
       }
@@ -689,8 +739,8 @@ __device__ __forceinline__ void merge_entry(PlanRef P, const uint64_t *s, uint64
     if ((m & M_VCOUNT) && a[A.o_count]) atomicAdd((unsigned long long *)&b[A.o_count], (unsigned long long)a[A.o_count]);
     if (m & M_ISUM64) atomicAdd((unsigned long long *)&b[A.o_isum], (unsigned long long)a[A.o_isum]);
     if (m & M_ISUM128) add128(&b[A.o_isum], a[A.o_isum], a[A.o_isum + 1]);
-    if (m & M_FSUM) add128(&b[A.o_fsum], a[A.o_fsum], a[A.o_fsum + 1]);
-    if (m & M_FSUMSQ) add128(&b[A.o_sq], a[A.o_sq], a[A.o_sq + 1]);
+    if (m & M_FSUM) add_words(&b[A.o_fsum], &a[A.o_fsum], A.nfw);
+    if (m & M_FSUMSQ) add_words(&b[A.o_sq], &a[A.o_sq], A.nqw);
     if (m & M_IMIN) atomicMin((long long *)&b[A.o_min], (long long)a[A.o_min]);
     if (m & M_IMAX) atomicMax((long long *)&b[A.o_max], (long long)a[A.o_max]);
     if ((m & M_FMIN) && a[A.o_min] != kEmpty) atomicMin((unsigned long long *)&b[A.o_min], (unsigned long long)a[A.o_min]);
@@ -997,6 +1047,80 @@ __global__ void k_col_stats(DevCol C, int64_t n, ColStats *out) {
     atomicMax(&out->emax, emax);
     atomicOr(&out->flags, flags);
     atomicAdd((unsigned long long *)&out->valid, (unsigned long long)cnt);
+  }
+}
+
+// Statistics of the values in the rows the query's predicate keeps, for the
+// columns of `cols` (bit c: plan column c): integer min/max, and the
+// exponent range of finite non-zero doubles (flags bit 32 when one was
+// seen). Run only when a column's own statistics leave no fixed-point
+// width that keeps its Float64 sums within the parity budget (a filtered
+// out row may hold a value far larger than every kept one).
+template <int NC>
+__global__ void __launch_bounds__(kBlock) k_kept_stats(const DevPlan *__restrict__ plan, uint32_t cols,
+                                                      ColStats *out) {
+  PlanRef P = plan_ref(plan);
+  int64_t lo[NC], hi[NC];
+  uint32_t emin[NC], emax[NC], fl[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    lo[c] = INT64_MAX;
+    hi[c] = INT64_MIN;
+    emin[c] = 0x7ff;
+    emax[c] = 0;
+    fl[c] = 0;
+  }
+  auto sink = [&](bool kept, int64_t, const uint64_t(&vals)[NC], uint32_t vbits) {
+    if (!kept) return;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (!((cols >> c) & 1u) || !((vbits >> c) & 1u)) continue;
+      const uint64_t v = vals[c];
+      if (P.cols[c].is_float) {
+        const uint32_t e = (uint32_t)(v >> 52) & 0x7ffu;
+        if (e == 0x7ffu || (v << 1) == 0) continue;
+        const uint32_t eb = e ? e : 1u;
+        emin[c] = min(emin[c], eb);
+        emax[c] = max(emax[c], eb);
+        fl[c] |= 32u;
+      } else {
+        lo[c] = min(lo[c], (int64_t)v);
+        hi[c] = max(hi[c], (int64_t)v);
+      }
+    }
+  };
+  const int64_t rows = P.rows;
+  const int64_t even = rows & ~(int64_t)1;
+  const int64_t step = 2 * (int64_t)blockDim.x;
+  if (even >= 2) {
+    for (int64_t base = (int64_t)blockIdx.x * step; base < even; base += (int64_t)gridDim.x * step) {
+      ScanTile<NC, 1> t;
+      load_tile<NC, 1>(P, t, base, even);
+      fold_tile<NC, 1, true>(P, t, base, sink);
+    }
+  }
+  if ((rows & 1) && blockIdx.x == 0) {
+    ScanTile<NC, 1> t;
+    load_last_row<NC>(P, t, even);
+    fold_tile<NC, 1, true>(P, t, even, sink);
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (!((cols >> c) & 1u)) continue;
+    for (int d = 32; d >= 1; d >>= 1) {
+      lo[c] = min(lo[c], (int64_t)__shfl_xor((long long)lo[c], d, 64));
+      hi[c] = max(hi[c], (int64_t)__shfl_xor((long long)hi[c], d, 64));
+      emin[c] = min(emin[c], (uint32_t)__shfl_xor((unsigned int)emin[c], d, 64));
+      emax[c] = max(emax[c], (uint32_t)__shfl_xor((unsigned int)emax[c], d, 64));
+      fl[c] |= (uint32_t)__shfl_xor((unsigned int)fl[c], d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin((long long *)&out[c].imin, (long long)lo[c]);
+      atomicMax((long long *)&out[c].imax, (long long)hi[c]);
+      atomicMin(&out[c].emin, emin[c]);
+      atomicMax(&out[c].emax, emax[c]);
+      atomicOr(&out[c].flags, fl[c]);
+    }
   }
 }
 

@@ -81,14 +81,15 @@ __device__ __forceinline__ void fold_full(const PV &P, uint64_t *st, int64_t row
                    (unsigned long long)(nan ? FL_NAN : ((bits >> 63) ? FL_NINF : FL_PINF)));
       } else {
         finite = true;
-        if (m & M_FSUM) {
+        if ((m & M_FSUM) && A.nfw == 2) {
           fix_of(bits, A.fshift, flo, fhi);
           f_old = add128_issue(&b[A.o_fsum], flo);
         }
-        if (m & M_FSUMSQ) {
+        if ((m & M_FSUMSQ) && A.nqw == 2) {
           fixsq_of(bits, A.fshift2, qlo, qhi);
           q_old = add128_issue(&b[A.o_sq], qlo);
         }
+        if (A.nfw > 2 || A.nqw > 2) add_wide(b, A, bits);
         if ((m & M_ZFIRST) && (bits << 1) == 0)
           atomicMin((unsigned long long *)&b[A.o_zfirst],
                     (unsigned long long)(((uint64_t)row << 1) | (bits >> 63)));
@@ -104,8 +105,8 @@ __device__ __forceinline__ void fold_full(const PV &P, uint64_t *st, int64_t row
     if (m & M_IMIN) atomicMin((long long *)&b[A.o_min], (long long)v);
     if (m & M_IMAX) atomicMax((long long *)&b[A.o_max], (long long)v);
     if (m & M_ISUM128) add128_finish(&b[A.o_isum], i_old, v, vhi);
-    if (finite && (m & M_FSUM)) add128_finish(&b[A.o_fsum], f_old, flo, fhi);
-    if (finite && (m & M_FSUMSQ)) add128_finish(&b[A.o_sq], q_old, qlo, qhi);
+    if (finite && (m & M_FSUM) && A.nfw == 2) add128_finish(&b[A.o_fsum], f_old, flo, fhi);
+    if (finite && (m & M_FSUMSQ) && A.nqw == 2) add128_finish(&b[A.o_sq], q_old, qlo, qhi);
   }
 }
 
@@ -121,6 +122,15 @@ __device__ __forceinline__ void merge_word(const PV &P, uint64_t *d, SrcPtr s, i
       case W_ADD128:
         add128(&d[w], v, s[w + 1]);
         break;
+      case W_ADD192:
+      case W_ADD256: {
+        const int n = P.wop[w] == W_ADD192 ? 3 : 4;
+        uint64_t x[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[i] = i < n ? s[w + i] : 0ull;
+        add_words(&d[w], x, n);
+        break;
+      }
       case W_MIN_S:
         if ((int64_t)v < (int64_t)d[w]) atomicMin((long long *)&d[w], (long long)v);
         break;

@@ -11,16 +11,19 @@
 //   word 1 (optional) FIRST  smallest row index of the group (maintain_order)
 //   then, for each input column with aggregates, a block:
 //     VCOUNT  non-null values                        (1 word)
-//     SUM     int64 total (narrow) or 128-bit total  (1 or 2 words)
-//     SUMSQ   128-bit total of squares (fixed point) (2 words)
+//     SUM     int64 total (narrow) or 128-bit total  (1 or 2 words);
+//             Float64: fixed point of 2, 3 or 4 words
+//     SUMSQ   total of squares (fixed point)         (2, 3 or 4 words)
 //     MIN     int64 or ordered float key             (1 word)
 //     MAX                                            (1 word)
 //     FLAGS   NaN / +inf / -inf seen                 (1 word, HBM only)
 //     ZFIRST  first zero's (row << 1 | sign)         (1 word, HBM only)
 // An LDS entry holds the same words except FLAGS and ZFIRST, which are
-// rare-event words updated in HBM directly. Float sums are exact 128-bit
-// fixed-point integers (scale 2^shift chosen per column from its exponent
-// range), so every state is order independent and results are bitwise
+// rare-event words updated in HBM directly. Float sums are fixed-point
+// integers of 128 to 256 bits (width and scale 2^shift chosen per column
+// from its exponent range and the row count, so that the sum is exact or,
+// when no width up to 256 bits is, its error is bounded below the parity
+// budget), so every state is order independent and results are bitwise
 // reproducible whatever the launch geometry or batching.
 #pragma once
 #ifndef __HIPCC_RTC__
@@ -66,6 +69,8 @@ enum : int8_t {
   W_MIN_U = 5,   // unsigned min
   W_MAX_U = 6,   // unsigned max
   W_OR = 7,
+  W_ADD192 = 8,  // low word of a 192-bit add (the next two words are W_HI)
+  W_ADD256 = 9,  // low word of a 256-bit add (the next three words are W_HI)
 };
 
 // How kept rows reach the group table.
@@ -118,10 +123,12 @@ struct DevColAgg {
   int16_t lbase;   // first LDS word of the block (LDS entry without its key)
   // word offsets inside the block (-1 when absent)
   int8_t o_count, o_isum, o_fsum, o_sq, o_min, o_max, o_flags, o_zfirst;
-  int32_t fshift;   // scale of the 128-bit fixed-point sum
-  int32_t fshift2;  // scale of the 128-bit fixed-point sum of squares
+  int32_t fshift;   // scale of the fixed-point sum
+  int32_t fshift2;  // scale of the fixed-point sum of squares
   int32_t int_as_float;  // an integer column whose VAR goes through doubles
-  int32_t pad2;
+  int8_t nfw;       // words of the fixed-point sum (2 to 4)
+  int8_t nqw;       // words of the fixed-point sum of squares (2 to 4)
+  int16_t pad2;
 };
 
 // Output aggregate (finalize).

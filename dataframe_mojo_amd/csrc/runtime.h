@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -92,7 +93,18 @@ struct dfr_context {
   hipEvent_t evp[4] = {nullptr, nullptr, nullptr, nullptr};  // phase boundaries
   dfr_observation obs = {};
   dfr::Tuning tune;
+  // A server's lane context outlives its server while results it produced
+  // are still held: each such result counts here, and the context is
+  // destroyed by whichever comes last, the server or the last release.
+  std::mutex hold_mu;
+  int64_t holds = 0;
+  bool retired = false;
 };
+
+// Counts a result of a lane context (see dfr_context::holds).
+void context_hold(dfr_result *r);
+// Marks a lane context for destruction once no result holds it.
+void context_retire(dfr_context *ctx);
 
 struct dfr_column {
   dfr_context *ctx = nullptr;
@@ -106,6 +118,7 @@ struct dfr_column {
   void *owned_values = nullptr;
   void *owned_valid = nullptr;
   dfr::ColStats stats = {};
+  int32_t stats_window = 1;  // 0: the statistics are a parent's (a slice), bounds of this window's
   dfr::DevCol dev() const {
     dfr::DevCol c;
     c.values = values;
@@ -127,8 +140,19 @@ struct dfr_result {
   void *buffer = nullptr;  // one pool block holding every output
   std::shared_ptr<void> shared;  // or a block shared by a batch's results
   size_t bytes = 0;
+  bool holds_ctx = false;  // counted in ctx->holds (a server lane's result)
+  std::atomic<int32_t> refs{1};  // the caller's handle plus each dfr_buffer of it
   double device_ms = 0.0;
   double phase_ms[4] = {0, 0, 0, 0};
+};
+
+// One output buffer of a result (a key's or an aggregate's values or
+// validity, or the first rows), as a device residence: it keeps the result
+// alive until released.
+struct dfr_buffer {
+  dfr_result *result = nullptr;
+  const void *device = nullptr;
+  int64_t bytes = 0;
 };
 
 int dfr_fail(char **error, int status, const char *fmt, ...);

@@ -156,6 +156,8 @@ void dfr_server::worker(int lane) {
         errs[at[k]] = bst == DFR_OK ? e2[k] : (berr ? strdup(berr) : nullptr);
       }
     }
+    // results keep their lane's context alive past dfr_server_destroy
+    for (int i = 0; i < n; ++i) context_hold(res[i]);
     const int64_t t_done = now_ns();
     {
       std::lock_guard<std::mutex> lk(mu);
@@ -280,11 +282,12 @@ extern "C" void dfr_server_destroy(dfr_server *s) {
   s->work_cv.notify_all();
   s->done_cv.notify_all();
   for (std::thread &t : s->workers) t.join();
-  // results never waited for are released with the lanes
+  // results never waited for are released; a lane whose results are
+  // still held by callers is destroyed when the last of them is released
   for (auto &kv : s->done) {
     dfr_result_release(kv.second.result);
     free(kv.second.error);
   }
-  for (dfr_context *c : s->lanes) dfr_context_destroy(c);
+  for (dfr_context *c : s->lanes) context_retire(c);
   delete s;
 }

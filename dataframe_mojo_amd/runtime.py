@@ -129,10 +129,63 @@ class Result:
         _lib.check(self.ctx.lib.dfr_result_copy_first_rows(self.handle, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(e)), e)
         return out
 
+    def residence(self, kind: int, index: int, part: int) -> "DeviceResidence":
+        """One output buffer as a device residence (storage/residence.mojo:
+        context, bytes, and the _CopyToHost/_ReleaseDevice callbacks by
+        address); it keeps this result's device memory alive."""
+        h = ctypes.c_void_p()
+        n = ctypes.c_int64()
+        e = _err()
+        _lib.check(self.ctx.lib.dfr_result_buffer(self.handle, kind, index, part, ctypes.byref(h), ctypes.byref(n),
+                                                  ctypes.byref(e)), e)
+        lib = self.ctx.lib
+        return DeviceResidence(
+            backend="amd", device=self.ctx.device, context=h.value, bytes=n.value,
+            to_host=ctypes.cast(lib.dfr_buffer_copy_to_host, ctypes.c_void_p).value,
+            release=ctypes.cast(lib.dfr_buffer_release, ctypes.c_void_p).value)
+
     def release(self):
         if self.handle:
             self.ctx.lib.dfr_result_release(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+# (context, host destination address, bytes) -> 0 on success; (context) -> None
+# (storage/residence.mojo:20-22)
+_COPY_TO_HOST = ctypes.CFUNCTYPE(ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64)
+_RELEASE_DEVICE = ctypes.CFUNCTYPE(None, ctypes.c_int64)
+
+
+class DeviceResidence:
+    """A buffer left on the device, as the reference's _DeviceResidence
+    holds it (storage/residence.mojo:33-80): the first host read copies the
+    bytes once through the `to_host` callback; `release` (or the object's
+    end) calls the `release` callback. Both are called by address."""
+
+    def __init__(self, backend: str, device: int, context: int, bytes: int, to_host: int, release: int):
+        self.backend, self.device, self.context, self.bytes = backend, device, context, bytes
+        self.to_host, self._release = to_host, release
+        self._host = None
+
+    def host(self) -> np.ndarray:
+        if self._host is None:
+            buf = np.empty(max(self.bytes, 1), dtype=np.uint8)
+            status = _COPY_TO_HOST(self.to_host)(self.context, buf.ctypes.data, self.bytes)
+            if status != 0:
+                raise RuntimeError(f"copying a {self.backend} device buffer to the host failed (status {status})")
+            self._host = buf[: self.bytes]
+        return self._host
+
+    def release(self):
+        if self.context:
+            _RELEASE_DEVICE(self._release)(self.context)
+            self.context = 0
 
     def __del__(self):
         try:

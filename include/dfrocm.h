@@ -304,7 +304,8 @@ int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int32_t n, dfr
  * work overlaps on the device. A result equals what dfr_groupby gives for
  * the query alone. The query's columns may belong to any context on the
  * same device and must outlive the wait; results come from the lanes'
- * contexts and must be released before dfr_server_destroy. */
+ * contexts, and each keeps its lane's context alive: it may be released
+ * before or after dfr_server_destroy. */
 typedef struct dfr_server dfr_server;
 
 typedef struct dfr_server_stats {
@@ -347,10 +348,36 @@ int dfr_result_copy_agg(const dfr_result *result, int32_t a, void *values,
 /* Each group's first kept row (its ordering key under maintain_order). */
 int dfr_result_copy_first_rows(const dfr_result *result, int64_t *rows,
                                char **error);
-/* Device bytes the result holds, and release of them
- * (the _CopyToHost/_ReleaseDevice pair of storage/residence.mojo:21-22). */
+/* Device bytes the result holds. */
 int64_t dfr_result_device_bytes(const dfr_result *result);
+/* Releases the caller's handle; the device memory goes once no dfr_buffer
+ * of the result is held either. */
 void dfr_result_release(dfr_result *result);
+
+/* ---- result buffers as device residences --------------------------------
+ * The reference keeps a buffer that lives in accelerator memory as a
+ * _DeviceResidence: an opaque context and two C callbacks,
+ * _CopyToHost(context, host address, bytes) -> 0 on success and
+ * _ReleaseDevice(context) (dataframe/storage/residence.mojo:20-22, 33-80).
+ * A dfr_buffer is that context for one output buffer of a result; it keeps
+ * its result alive until released, so the result handle may be released
+ * first. dfr_buffer_copy_to_host and dfr_buffer_release have exactly the
+ * callbacks' shapes (Mojo Int is 64 bits), so their addresses are the
+ * residence's `to_host` and `release`. */
+#define DFR_BUFFER_KEY 0        /* kind: key column `index` */
+#define DFR_BUFFER_AGG 1        /* kind: aggregate `index` */
+#define DFR_BUFFER_FIRST_ROWS 2 /* kind: each group's first kept row (int64; maintain_order) */
+#define DFR_BUFFER_VALUES 0     /* part: groups values of the column's width */
+#define DFR_BUFFER_VALIDITY 1   /* part: ceil(groups / 8) bytes of LSB-first bits */
+typedef struct dfr_buffer dfr_buffer;
+/* A residence handle for one buffer of `result`; *bytes receives its size. */
+int dfr_result_buffer(dfr_result *result, int32_t kind, int32_t index, int32_t part, dfr_buffer **out,
+                      int64_t *bytes, char **error);
+/* _CopyToHost: copies the first `bytes` bytes (at most the buffer's size) of
+ * the buffer to host_dst; returns 0 (DFR_OK) on success, else a status. */
+int64_t dfr_buffer_copy_to_host(void *buffer, void *host_dst, int64_t bytes);
+/* _ReleaseDevice. */
+void dfr_buffer_release(void *buffer);
 
 /* Milliseconds the device spent on the query's kernels (HIP events on the
  * context's stream). */

@@ -477,3 +477,36 @@ def test_float_keys(mo, setting, gpu_ctx):
     m = ek.is_valid()
     assert np.array_equal(gk.values[m].view(np.uint64), ek.values[m].view(np.uint64))
     assert_same(got, exp, True, ("x_min",), 1)
+
+
+def test_result_buffers_as_device_residences(gpu_ctx):
+    """Each output buffer through the residence callbacks, called by address
+    as storage/residence.mojo calls _CopyToHost/_ReleaseDevice, after the
+    result handle itself is released: the same bytes as the direct copies."""
+    from dataframe_mojo_amd.plan import compile_plan
+
+    df = random_frame(5000, 3)
+    lf = df.filter(col("a") > 0).group_by("k", "s", maintain_order=True).agg(
+        col("x").sum().alias("xs"), col("b").sum().alias("bs"), col("a").min().alias("amin"), len_())
+    plan = compile_plan(lf)
+    dev = {name: gpu_ctx.upload(df.column(name).column) for name in {*plan.keys, *plan.columns}}
+    res = gpu_ctx.groupby(plan, [dev[k] for k in plan.keys], [dev[c] for c in plan.columns])
+    G = res.groups
+    want = []
+    for kind, count, get in ((_lib.DFR_BUFFER_KEY, len(plan.keys), res.key), (_lib.DFR_BUFFER_AGG, len(plan.aggs), res.agg)):
+        for i in range(count):
+            _, vals, valid = get(i)
+            want.append((kind, i, vals.tobytes(), np.packbits(valid, bitorder="little").tobytes()))
+    first = res.first_rows().tobytes()
+    held = [(w, res.residence(w[0], w[1], _lib.DFR_BUFFER_VALUES), res.residence(w[0], w[1], _lib.DFR_BUFFER_VALIDITY))
+            for w in want]
+    fr = res.residence(_lib.DFR_BUFFER_FIRST_ROWS, 0, _lib.DFR_BUFFER_VALUES)
+    res.release()  # the residences keep the device memory
+    for (kind, i, vals, bits), rv, rb in held:
+        assert rv.backend == "amd" and rv.bytes == len(vals)
+        assert rv.host().tobytes() == vals, (kind, i)
+        assert rb.bytes == (G + 7) // 8 and rb.host().tobytes() == bits, (kind, i)
+        rv.release()
+        rb.release()
+    assert fr.host().tobytes() == first
+    fr.release()

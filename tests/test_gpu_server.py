@@ -102,3 +102,19 @@ def test_server_errors_and_tickets(gpu_ctx):
             srv.wait(t)
         with pytest.raises(_lib.DfrError):
             srv.wait(t)  # answered once
+
+
+def test_result_released_after_server_close(gpu_ctx):
+    # a served result keeps its lane's context alive: it can be read and
+    # released after the server is gone (the lane goes with the last result)
+    rng = np.random.default_rng(12)
+    tabs = tables(gpu_ctx, 2)
+    work = [window_query(*tabs[i % 2], rng, i) for i in range(5)]
+    srv = Server(gpu_ctx, lanes=2, max_batch=4)
+    ts = [srv.submit(job) for _, job in work]
+    got = [srv.wait(t) for t in ts[:4]]  # the last ticket is never waited for
+    srv.close()
+    for (lf, (plan, keys, _)), r in zip(work, got):
+        exp, _ = oracle.collect(lf)
+        assert_same(result_frame(r, plan, keys), exp, False, ("x_min",), len(plan.keys))
+        r.release()

@@ -367,3 +367,138 @@ def test_parallel_restatement_raises_the_same_overflow():
     lf = df.lazy().group_by("k").agg(col("v").sum())
     with pytest.raises(oracle.OracleError, match="Int64 expression sum overflow"):
         oracle.collect(lf, threads=4)
+
+
+# --- Float64 min/max between distinct values, by brute force
+
+
+def _brute_extrema(keys, x, valid):
+    """Per key in first-row order: (min, max) by the reference's rules
+    (dataframe/aggregation/function.mojo:1143-1235 _scatter, reducer.mojo
+    MIN/MAX; tests/kernel_equivalence/test_state_merge.mojo:605-639): NaN
+    values are skipped unless every valid value is NaN (then NaN), no valid
+    value gives null, and a tie keeps the first value in row order (so the
+    sign of a zero is the first zero's). Python's min/max keep the first of
+    equal elements, which is that rule."""
+    groups: dict = {}
+    for i, k in enumerate(keys):
+        groups.setdefault(int(k), [])
+        if valid[i]:
+            groups[int(k)].append(float(x[i]))
+    out = {}
+    for k, vals in groups.items():
+        if not vals:
+            out[k] = (None, None)
+            continue
+        real = [v for v in vals if not math.isnan(v)]
+        if not real:
+            out[k] = (math.nan, math.nan)
+            continue
+        out[k] = (min(real), max(real))
+    return out
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_float_min_max_brute_force(seed):
+    rng = np.random.default_rng(100 + seed)
+    n = 4000
+    keys = rng.integers(0, 60, n)
+    x = rng.normal(size=n) * 10.0 ** rng.integers(-3, 4, n)
+    # signed zeros, NaNs, infinities and repeated values, in every group
+    pick = rng.random(n)
+    x[pick < 0.05] = 0.0
+    x[(pick >= 0.05) & (pick < 0.10)] = -0.0
+    x[(pick >= 0.10) & (pick < 0.13)] = math.nan
+    x[(pick >= 0.13) & (pick < 0.14)] = math.inf
+    x[(pick >= 0.14) & (pick < 0.15)] = -math.inf
+    x[(pick >= 0.15) & (pick < 0.25)] = 1.25
+    x[keys == 7] = math.nan  # a group of NaNs only
+    valid = rng.random(n) > 0.1
+    valid[keys == 9] = False  # a group with no valid value
+    df = DataFrame([Series("k", Column.int64(keys)), Series("x", Column.float64(x, valid))])
+    out = run(df.lazy().group_by("k", maintain_order=True).agg(col("x").min().alias("mn"), col("x").max().alias("mx")))
+    exp = _brute_extrema(keys, x, valid)
+    assert out["k"] == list(exp)  # first-row order
+    for i, k in enumerate(out["k"]):
+        for got, want in ((out["mn"][i], exp[k][0]), (out["mx"][i], exp[k][1])):
+            if want is None:
+                assert got is None, k
+            elif math.isnan(want):
+                assert math.isnan(got), k
+            else:
+                assert np.float64(got).tobytes() == np.float64(want).tobytes(), (k, got, want)
+
+
+def test_float_min_max_distinct_values_not_swapped():
+    df = DataFrame([Series("k", Column.int64([1, 1, 1, 2, 2])),
+                    Series("x", Column.float64([3.5, -2.25, 7.0, -0.0, 1e-300]))])
+    out = run(df.lazy().group_by("k", maintain_order=True).agg(col("x").min().alias("mn"), col("x").max().alias("mx")))
+    assert out["mn"] == [-2.25, -0.0] and out["mx"] == [7.0, 1e-300]
+    assert math.copysign(1, out["mn"][1]) == -1
+
+
+# --- var/std at ddof 0 and 1 against exact two-pass references
+
+
+def _variance_frame(n, nulls):
+    """The columns of tests/public_contracts/test_numeric_variance.mojo:14-88
+    (step = row * 7919 % 2001; Float64 1e6 + step / 8, Int32 100000 + step,
+    Int64 step; null every row % 7 == 3 when asked), and the reference's
+    expected variance of `step` from its kept values."""
+    step = (np.arange(n, dtype=np.int64) * 7919) % 2001
+    valid = np.ones(n, dtype=bool) if not nulls else (np.arange(n) % 7 != 3)
+    df = DataFrame([
+        Series("g", Column.int64(np.zeros(n, dtype=np.int64))),
+        Series("d", Column.float64(1e6 + step / 8, valid)),
+        Series("i", Column.int32((100_000 + step).astype(np.int32), valid)),
+        Series("u", Column.int64(step, valid)),
+    ])
+    return df, [Fraction(int(s)) for s, ok in zip(step, valid) if ok]
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 9, 127, 128, 129, 257, 1000, 70_001])
+@pytest.mark.parametrize("nulls", [False, True])
+@pytest.mark.parametrize("ddof", [0, 1])
+def test_variance_matches_two_pass_reference(n, nulls, ddof):
+    df, kept = _variance_frame(n, nulls)
+    if n == 0:
+        return  # no group at all
+    lf = df.lazy().group_by("g").agg(
+        [col(c).var(ddof=ddof).alias("v" + c) for c in "diu"] + [col(c).std(ddof=ddof).alias("s" + c) for c in "diu"])
+    out = run(lf)
+    defined = len(kept) > ddof
+    if defined:
+        m = sum(kept) / len(kept)
+        unit = sum((v - m) ** 2 for v in kept) / (len(kept) - ddof)
+    for c, scale in (("d", Fraction(1, 64)), ("i", Fraction(1)), ("u", Fraction(1))):
+        v, s = out["v" + c][0], out["s" + c][0]
+        if not defined:
+            assert v is None and s is None
+            continue
+        want = unit * scale
+        # the reference's own bound (1e-9, its _close) and the exact value's
+        assert abs(Fraction(v) - want) <= Fraction(1, 10**9) * max(1, want)
+        assert abs(Fraction(v) - want) <= Fraction(1, 10**12) * max(1, want), (c, v, float(want))
+        assert abs(s - math.sqrt(float(want))) <= 1e-12 * max(1.0, math.sqrt(float(want))), (c, s)
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("ddof", [0, 1, 3])
+def test_random_variance_against_exact(seed, ddof):
+    rng = np.random.default_rng(200 + seed)
+    n = 3000
+    keys = rng.integers(0, 25, n)
+    x = rng.normal(size=n) * 10.0 ** rng.integers(-2, 3, n) + rng.choice([0.0, 50.0], n)
+    valid = rng.random(n) > 0.05
+    df = DataFrame([Series("k", Column.int64(keys)), Series("x", Column.float64(x, valid))])
+    out = run(df.lazy().group_by("k", maintain_order=True).agg(
+        col("x").var(ddof=ddof).alias("v"), col("x").std(ddof=ddof).alias("s")))
+    for i, k in enumerate(out["k"]):
+        vals = [Fraction(float(v)) for v, ok, kk in zip(x, valid, keys) if ok and kk == k]
+        if len(vals) - ddof <= 0:
+            assert out["v"][i] is None and out["s"][i] is None
+            continue
+        m = sum(vals) / len(vals)
+        want = sum((v - m) ** 2 for v in vals) / (len(vals) - ddof)
+        assert abs(Fraction(out["v"][i]) - want) <= Fraction(1, 10**12) * max(1, want), (k, out["v"][i], float(want))
+        assert abs(out["s"][i] - math.sqrt(float(want))) <= 1e-12 * max(1.0, math.sqrt(float(want)))
