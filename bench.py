@@ -15,15 +15,20 @@ Workloads (columns generated on the device, resident in HBM):
     sum/mean(x)); one dfr_groupby_batch call per step, against one
     dfr_groupby call per query.
   serve: Poisson arrivals at 2000 queries/s of that query distribution over
-    row windows of 64 device-resident tables (1e6 to 1e8 rows; Int64 and
-    dictionary String keys) through the serving loop (dfr_server_*); p50 and
+    row windows of 64 device-resident tables (1e6 to 1e8 rows; Int64,
+    dictionary String and Float64 keys) through the serving loop (dfr_server_*); p50 and
     p99 latency and queries/s, against serial one-call-at-a-time execution
     of the same arrivals.
 A step is one whole query through the C ABI (dfr_groupby: table set-up,
 fused scan, partition passes, compaction, finish) with its inputs already on
 the device and its results left on the device. Prints one JSON line.
 
-  python bench.py [--workload W] [--steps K] [--warmup W] [--rows N] [--cpu-sample S]
+The default run (zipf_200m) also measures, after its own line's figures,
+dict_500m, batch_4096 (graph captured) and serve at shorter settings, and
+reports them as the extra keys "dict_500m", "batch_4096" and "serve"
+(--extras 0 leaves them out).
+
+  python bench.py [--workload W] [--steps K] [--warmup W] [--rows N] [--cpu-sample S] [--extras 0|1]
 """
 from __future__ import annotations
 
@@ -153,9 +158,43 @@ def cpu_baseline(spec, cols, sample: int, threads: int):
     times = []
     for _ in range(2):
         t0 = time.perf_counter()
-        oracle.collect(lf, threads=threads)
+        out, _ = oracle.collect(lf, threads=threads)
         times.append(time.perf_counter() - t0)
-    return n / min(times), min(times)
+    return n / min(times), min(times), frame, lf, out
+
+
+def dd_error(ctx, spec, cols, frame, lf, cpu_out):
+    """Per-group error of the Float64 sum and variance against a
+    double-double reference (double-double sums, two-pass variances), for
+    the device result and the CPU (oracle) result on the same sample:
+    max over groups of |r - dd| / max(1, |dd|)."""
+    from dataframe_mojo_amd.plan import compile_plan
+    from dataframe_mojo_amd.runtime import result_frame
+    from oracle import oracle
+
+    n = frame.height()
+    dev = spec["wrap"](ctx, cols, n)
+    plan = compile_plan(lf)
+    res = ctx.groupby(plan, [dev[k] for k in plan.keys], [dev[c] for c in plan.columns])
+    gpu_out = result_frame(res, plan, [dev[k] for k in plan.keys])
+    res.release()
+    keep = oracle.kept_rows(frame, lf.filters)
+    key = frame.column(spec["dd"]["key"]).column
+    x = frame.column(spec["dd"]["x"]).column
+    uniq, gid = np.unique(key.values[keep], return_inverse=True)
+    hi, lo, cnt, var = oracle.dd_moments(gid, x.values[keep], x.is_valid()[keep], len(uniq), ddof=1)
+    dd_sum = hi + lo
+    out = {"sample_rows": n, "groups": int(len(uniq))}
+    for who, fr in (("gpu", gpu_out), ("cpu", cpu_out)):
+        kc = fr.column(spec["dd"]["key"]).column
+        order = np.argsort(kc.values, kind="stable")
+        assert np.array_equal(kc.values[order], uniq), who
+        for agg, ref, mask in (("sum", dd_sum, np.ones(len(uniq), bool)), ("var", var, cnt > 1)):
+            c = fr.column(spec["dd"][agg]).column
+            vals = np.where(c.is_valid(), c.values, np.nan)[order]
+            err = np.abs(vals[mask] - ref[mask]) / np.maximum(1.0, np.abs(ref[mask]))
+            out[f"{who}_{agg}_max"] = float(err.max()) if err.size else 0.0
+    return out
 
 
 def batch_query(threshold: int):
@@ -171,7 +210,7 @@ def batch_query(threshold: int):
         col("x").sum().alias("x_sum"), col("x").mean().alias("x_mean"))
 
 
-def bench_batch(args) -> int:
+def bench_batch(args) -> dict:
     """batch_4096: 4096 independent filtered group-by queries (rows
     log-uniform in [1e3, 1e7], key cardinality log-uniform in [1, 1e5],
     threshold uniform) as one dfr_groupby_batch call per step, against the
@@ -287,11 +326,10 @@ def bench_batch(args) -> int:
                               [statistics.mean(p[i] for p in phases) for i in range(4)])),
         "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
-def bench_serve(args) -> int:
+def bench_serve(args) -> dict:
     """serve: Poisson arrivals (args.rate queries/s for args.duration s) of
     the batched workload's query distribution over 64 device-resident
     tables, through the serving loop, against the same arrivals served one
@@ -312,9 +350,13 @@ def bench_serve(args) -> int:
     tabs = []
     for t in range(n_tables):
         n, base = int(rows[t]), 5000 + 8 * t
-        dict_key = t % 2 == 1
+        kind = t % 3  # Int64, dictionary String, Float64 keys
+        dict_key = kind == 1
         hi = min(int(card[t]), 1000) if dict_key else int(card[t])
-        key = datagen.device_uniform(n, args.seed, base, 0, hi - 1, dtype=np.uint32 if dict_key else np.int64)
+        if kind == 2:
+            key = datagen.device_float_keys(n, args.seed, base, hi)
+        else:
+            key = datagen.device_uniform(n, args.seed, base, 0, hi - 1, dtype=np.uint32 if dict_key else np.int64)
         value = datagen.device_uniform(n, args.seed, base + 1, -1_000_000, 1_000_000)
         x = datagen.device_normal(n, args.seed, base + 2, 0.0, 1.0)
         xv = datagen.device_validity(n, args.seed, base + 3, 0.05)
@@ -442,7 +484,7 @@ def bench_serve(args) -> int:
         "dtype": "int64+f64",
         "data": "synthetic (seeded counter-based generator, device resident)",
         "config": {"workload": "serve", "tables": n_tables, "table_rows": "log-uniform [1e6, %.0e]" % args.table_rows_max,
-                   "keys": "Int64 (cardinality log-uniform [1, 1e5]) and dictionary String, alternating",
+                   "keys": "Int64, dictionary String and Float64 (cardinality log-uniform [1, 1e5]), in turn",
                    "arrivals": f"Poisson {args.rate:.0f}/s for {args.duration} s", "queries": n_q,
                    "query": "window rows log-uniform [1e3, 1e7], value > t, len/sum/max(value), sum/mean(x)",
                    "lanes": args.lanes, "max_batch": args.max_batch, "budget_bytes": args.serve_budget},
@@ -453,8 +495,7 @@ def bench_serve(args) -> int:
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
 WORKLOADS = {
@@ -464,6 +505,7 @@ WORKLOADS = {
         "out_bytes_per_group": 4 + 8 * 8 + 7 / 8,  # key; 8 aggregates; 7 nullable
         "config": {"key": "Int32 Zipf(1.1) over 1e6", "filter": "a > 0",
                    "aggs": "sum/count/min/max(y), sum/mean/var/min(x)"},
+        "dd": {"key": "key", "x": "x", "sum": "x_sum", "var": "x_var"},
     },
     "dict_500m": {
         "rows": 500_000_000, "query": dict_query, "wrap": wrap_dict, "host": host_dict,
@@ -495,6 +537,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=50_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extras", type=int, default=1,
+                    help="zipf_200m: also measure dict_500m, batch_4096 (graph) and serve, as extra keys")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first run one streaming copy of this many bytes (a known-size kernel for byte counters)")
     args = ap.parse_args()
@@ -504,20 +548,66 @@ def main() -> int:
         return 2
 
     if args.workload == "batch_4096":
-        return bench_batch(args)
+        print(json.dumps(bench_batch(args)), flush=True)
+        return 0
     if args.workload == "serve":
-        return bench_serve(args)
+        print(json.dumps(bench_serve(args)), flush=True)
+        return 0
 
-    from dataframe_mojo_amd import datagen, runtime
-    from dataframe_mojo_amd.plan import compile_plan
+    from dataframe_mojo_amd import datagen
 
     if args.copy_calibration:
         rate = datagen.stream_rate(args.copy_calibration, copy=True, reps=1)
         log(f"calibration copy of {args.copy_calibration} bytes: {rate / 1e9:.0f} GB/s (read + write)")
-    spec = WORKLOADS[args.workload]
-    n = args.rows or spec["rows"]
-    log(f"generating {args.workload} columns on the device: {n} rows")
-    cols = getattr(datagen, f"{args.workload}_device")(n, args.seed)
+    line, ctx = bench_query(args, args.workload, args.steps, args.warmup, args.rows, not args.no_cpu)
+    if args.extras and args.workload == "zipf_200m":
+        line.update(extras(args, ctx))
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def extras(args, ctx) -> dict:
+    """The other configurations at shorter settings, each summarised: the
+    500M-row dictionary query, the 4096-query batch with its launch sequence
+    captured as a graph, and the serving loop."""
+    import gc
+
+    out = {}
+    d, _ = bench_query(args, "dict_500m", 5, 2, 0, False)
+    out["dict_500m"] = {"ms_per_query": d["ms_per_step"], "rows_per_s": d["value"], "groups": d["config"]["groups"],
+                        "query_hbm_frac": d["query_hbm"]["frac"], "scan_frac": d["roofline"]["frac"],
+                        "phases_ms": d["phases_ms"], "ordered_ms_per_query": d["ordered_ms_per_query"]}
+    gc.collect()
+    ctx.trim()
+    b_args = argparse.Namespace(**{**vars(args), "steps": 3, "warmup": 1, "batch_graph": 1})
+    b = bench_batch(b_args)
+    out["batch_4096"] = {"batched_ms": b["ms_per_step"], "plain_ms": b["plain_ms"],
+                         "batched_vs_plain": b["batched_vs_plain"], "graph": True, "rows": b["config"]["rows"],
+                         "identical_results_checked": b["identical_results_checked"],
+                         "rows_per_s": b["value"], "query_hbm_GBps": b["query_hbm"]["GB/s"]}
+    gc.collect()
+    ctx.trim()
+    s_args = argparse.Namespace(**{**vars(args), "duration": 2.0, "overload": 4.0})
+    sv = bench_serve(s_args)
+    out["serve"] = {"rate": s_args.rate, "queries": sv["config"]["queries"], "served": sv["served"],
+                    "serial_plain": sv["serial_plain"], "overload": sv["overload"],
+                    "identical_results_checked": sv["identical_results_checked"]}
+    gc.collect()
+    ctx.trim()
+    return out
+
+
+def bench_query(args, workload: str, steps: int, warmup: int, rows: int, with_cpu: bool):
+    """One-query workloads (zipf_200m, dict_500m): the JSON line's fields."""
+    import gc
+
+    from dataframe_mojo_amd import datagen, runtime
+    from dataframe_mojo_amd.plan import compile_plan
+
+    spec = WORKLOADS[workload]
+    n = rows or spec["rows"]
+    log(f"generating {workload} columns on the device: {n} rows")
+    cols = getattr(datagen, f"{workload}_device")(n, args.seed)
     ctx = runtime.context(0)
     dev = spec["wrap"](ctx, cols, n)
     dplan = compile_plan(spec["query"]())
@@ -525,20 +615,20 @@ def main() -> int:
     inputs = [dev[c] for c in dplan.columns]
     log("warm-up")
     groups = 0
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         r = ctx.groupby(dplan, keys, inputs)
         groups = r.groups
         r.release()
-    log(f"timing {args.steps} steps ({groups} groups)")
+    log(f"timing {steps} steps ({groups} groups)")
     phases = []
     jit_before = ctx.observe()["jit_launches"]
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         r = ctx.groupby(dplan, keys, inputs)
         phases.append(r.phase_ms)
         r.release()
     wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
+    ms_per_step = wall * 1e3 / steps
     # pass A ran as the kernel compiled for the query's shape, or precompiled
     kernel = "dfr_scan_jit" if ctx.observe()["jit_launches"] > jit_before else "k_scan_spill"
     scan_ms = statistics.mean(p[1] for p in phases)
@@ -552,39 +642,45 @@ def main() -> int:
     dplan_o = compile_plan(spec["query"](True))
     ctx.groupby(dplan_o, keys, inputs).release()  # warm-up: its shape's kernels are compiled here
     ordered = []
-    for _ in range(max(3, args.steps // 4)):
+    for _ in range(max(3, steps // 4)):
         r = ctx.groupby(dplan_o, keys, inputs)
         ordered.append(r.device_ms)
         r.release()
 
     cpu = None
-    if not args.no_cpu:
+    dd = None
+    if with_cpu:
         sample = min(args.cpu_sample, n)
         threads = cpu_threads()
         log(f"CPU baseline on {sample} rows, {threads} threads")
-        rate, secs = cpu_baseline(spec, cols, sample, threads)
+        rate, secs, frame, lf, cpu_out = cpu_baseline(spec, cols, sample, threads)
         cpu = {"value": rate, "unit": "rows/s", "cores": threads, "kind": "port",
                "sample": f"first {sample} rows of the same workload; restated reference CPU algorithm in C "
                          f"(oracle dfo_groupby_parallel: {threads} workers over contiguous rows, per-worker hash "
                          f"partitions, partition-parallel aggregation and finish), host filter included, "
                          f"best of 2 ({secs:.2f} s)"}
+        if spec.get("dd"):
+            log("per-group error against double-double on the same sample")
+            dd = dd_error(ctx, spec, cols, frame, lf, cpu_out)
+            log(f"double-double error: {dd}")
+        del frame, lf, cpu_out
 
     line = {
         "metric": "filtered group-by rows/s",
         "value": n / (ms_per_step / 1e3),
         "unit": "rows/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int64+f64",
         "data": "synthetic (seeded counter-based generator, device resident)",
-        "config": {"workload": args.workload, "rows": n, "groups": groups, **spec["config"], "maintain_order": False},
+        "config": {"workload": workload, "rows": n, "groups": groups, **spec["config"], "maintain_order": False},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic_of(args.workload, kernel, n),
+                     "frac": achieved / HBM_PEAK, "traffic": traffic_of(workload, kernel, n),
                      "kernel": kernel, "kernel_ms": scan_ms, "bytes_per_row": bytes_per_row,
                      "alg_bytes": scan_bytes},
         "query_hbm": {"alg_bytes": alg_bytes, "GB/s": alg_bytes / (ms_per_step / 1e3) / 1e9,
@@ -594,8 +690,11 @@ def main() -> int:
         "ordered_ms_per_query": statistics.mean(ordered),
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    if dd is not None:
+        line["dd_error"] = dd
+    del dev, keys, inputs, cols
+    gc.collect()
+    return line, ctx
 
 
 if __name__ == "__main__":

@@ -5,6 +5,8 @@ libdfrocm.so  the C-ABI provider library (csrc/dfrocm.hip, csrc/groupby.hip,
               for a query's shape at run time from the headers in csrc/)
 libdfrgen.so  seeded synthetic column generators (csrc/datagen.hip), used by
               the benchmark and the tests to create device-resident inputs
+dfr_jitc      the runtime compiler libdfrocm runs as a child process for its
+              shape-specialised kernels (csrc/jitc.cpp, host code over hiprtc)
 
 Run as ``python -m dataframe_mojo_amd.build`` or through
 ``__graft_entry__.build()``.
@@ -75,7 +77,14 @@ def build(verbose: bool = False) -> list[str]:
     with concurrent.futures.ThreadPoolExecutor(workers) as pool:
         for f in [pool.submit(_run, cmd, verbose, what) for cmd, what in jobs]:
             f.result()
-    out = []
+    # the runtime compiler program (host code only)
+    jitc = os.path.join(HERE, "dfr_jitc")
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", f"-I{rocm}/include",
+          "-o", jitc + ".tmp", os.path.join(CSRC, "jitc.cpp"), f"-L{rocm}/lib", "-lhiprtc", f"-Wl,-rpath,{rocm}/lib"],
+         verbose, "building dfr_jitc")
+    os.replace(jitc + ".tmp", jitc)
+    out = [jitc]
     for name, spec in LIBS.items():
         target = os.path.join(HERE, name)
         objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in spec["sources"]]

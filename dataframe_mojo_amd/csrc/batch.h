@@ -204,8 +204,9 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
   if (m == 0) return DFR_OK;
   // Large queries whose groups outgrow the batch kernel's LDS table (their
   // rows would fold into HBM with device atomics) take the partitioned
-  // passes instead, launched after the batch kernel into the same tables;
-  // not under graph capture (a new shape may compile at run time). On
+  // passes instead, launched after the batch kernel into the same tables.
+  // Under graph capture their runtime-compiled kernels are built first,
+  // before the capture begins (a compile cannot be captured). On
   // batch_4096 (rows >= 4M, 4096 queries) this takes 172 ms to 140 ms.
   const bool graph = ctx->tune.batch_graph != 0;
   std::vector<char> parted(m, 0);
@@ -225,18 +226,24 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
     return 2 * r(w[0] * 8) + 2 * r(w[1] * 8) + r(w[2] * 4) + r(w[3] * 4) + r(w[4] * 4) + r(w[5] * 4) +
            2 * r(w[6] * 8);
   };
-  if (!graph && ctx->tune.batch_part_rows > 0) {
+  if (ctx->tune.batch_part_rows > 0) {
     for (int j = 0; j < m; ++j) {
       Compiled &c = C[live[j]];
       if (c.plan.lds_mode == LDS_DIRECT || c.rows < ctx->tune.batch_part_rows) continue;
       Compiled t = c;
       choose_layout(ctx, t, ctx->tune.hash_slots, 0);  // the context's (partitioned) tuning
       if (t.plan.agg_mode != AGG_PARTITION || t.plan.two_level) continue;
-      if (!ctx->tune.jit_wait) {
-        // no compile waits (a serving lane): routed only once the shape's
-        // kernels exist, since the precompiled ones lose to the batch kernel
+      if (!ctx->tune.jit_wait || graph) {
+        // routed only once the shape's kernels exist, since the precompiled
+        // ones lose to the batch kernel: a lane that never waits for a
+        // compile (a server's) takes them only when compiled already; a
+        // graph capture compiles them now, outside the capture
         int ldsm, nc, u;
         scan_shape(ctx, t, &ldsm, &nc, &u);
+        if (graph && ctx->tune.jit_wait && ctx->tune.jit) {
+          jit_function(ctx, t.plan, ldsm, nc, u, JIT_SCAN);
+          jit_function(ctx, t.plan, ldsm, nc, u, JIT_PART);
+        }
         if (!jit_compiled(ctx, t.plan, ldsm, nc, u, JIT_SCAN) || !jit_compiled(ctx, t.plan, ldsm, nc, u, JIT_PART))
           continue;
       }
@@ -429,7 +436,7 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
     }
     for (int j = 0; j < m; ++j) {
       if (!parted[j] || C[live[j]].rows == 0) continue;
-      if ((st = launch_partitioned(ctx, C[live[j]], dplans + j, error))) {
+      if ((st = launch_partitioned(ctx, C[live[j]], dplans + j, error, !graph))) {
         hipStreamSynchronize(ctx->stream);
         drop_all();
         return st;

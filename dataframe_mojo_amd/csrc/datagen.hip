@@ -40,6 +40,16 @@ __global__ void k_uniform(void *out, int width, uint64_t n, uint64_t base, int64
   }
 }
 
+// Float64 keys from integer draws: k * scale, with k = 0 as -0.0 (a signed
+// zero, which groups with 0.0)
+__global__ void k_f64_keys(double *out, const int64_t *in, uint64_t n, double scale) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const int64_t k = in[r];
+    out[r] = k == 0 ? -0.0 : (double)k * scale;
+  }
+}
+
 // value = 1 + first i with (u >> 11) < cdf[i]   (cdf scaled to 2^53)
 __global__ void k_zipf(int32_t *out, uint64_t n, uint64_t base, const uint64_t *cdf, uint64_t m,
                        int64_t offset) {
@@ -175,6 +185,12 @@ int dfrg_uniform(void *out, int width, uint64_t n, uint64_t seed, uint64_t strea
   uint64_t span = (uint64_t)hi - (uint64_t)lo + 1;  // 0 means the full 2^64 range
   hipLaunchKernelGGL(k_uniform, dim3(launch_blocks(n)), dim3(256), 0, 0, out, width, n,
                      stream_base(seed, stream), lo, span);
+  return finish();
+}
+
+// out[r] = in[r] * scale as Float64 (0 as -0.0); out may alias in.
+int dfrg_f64_keys(double *out, const int64_t *in, uint64_t n, double scale) {
+  hipLaunchKernelGGL(k_f64_keys, dim3(launch_blocks(n)), dim3(256), 0, 0, out, in, n, scale);
   return finish();
 }
 

@@ -268,6 +268,13 @@ extern "C" void dfr_context_destroy(dfr_context *ctx) {
   delete ctx;
 }
 
+extern "C" void dfr_context_trim(dfr_context *ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  std::lock_guard<std::mutex> g(ctx->pool.mu);
+  ctx->pool.trim();
+}
+
 extern "C" int dfr_context_observe(const dfr_context *ctx, dfr_observation *out) {
   if (!ctx || !out) return DFR_INVALID;
   *out = ctx->obs;

@@ -922,8 +922,9 @@ int launch_scan_cols(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, 
   return launch_scan<LDSM, kMaxCols, 1>(ctx, C, dplan, error);
 }
 
-// Passes A, B and C of the partitioned aggregation.
-int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
+// Passes A, B and C of the partitioned aggregation. `events`: record the
+// phase boundary (off under graph capture).
+int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error, bool events = true) {
   const DevPlan &P = C.plan;
   int st;
   switch (P.lds_mode) {
@@ -933,7 +934,7 @@ int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan
   }
   if (st) return st;
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
+  if (events) HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
   hipLaunchKernelGGL(k_part_offsets, dim3(P.n_parts), dim3(kBlock), 0, ctx->stream, dplan);
   hipLaunchKernelGGL(k_part_base, dim3(1), dim3(kBlock), 0, ctx->stream, dplan);
   // k_part_scatter: 64-bit write cursors per partition and kind, then its

@@ -9,11 +9,18 @@
 // that vary with the data or the query's constants (column pointers, key
 // ranges, literals, fixed-point scales, table sizes) stay run-time values,
 // so one compiled kernel serves every query of the same shape. Kernels are
-// cached per device and shape for the life of the process.
+// cached per device and shape for the life of the process. Each compile
+// runs in a child process (dfr_jitc, csrc/jitc.cpp), so a compiler crash
+// costs only the specialised kernel.
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
 #include <dlfcn.h>
+#include <spawn.h>
+#include <sys/wait.h>
+#include <unistd.h>
+
+#include <cerrno>
 
 #include <condition_variable>
 #include <cstdio>
@@ -30,6 +37,8 @@
 #include "runtime.h"
 
 using namespace dfr;
+
+extern char **environ;
 
 namespace {
 
@@ -169,11 +178,8 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF, int 
   return o.str();
 }
 
-JitEntry compile(const std::string &src, int device, int which) {
-  JitEntry e;
-  e.failed = true;
-  hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "dfr_scan_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) return e;
+// The compiler options of a generated kernel for the device's architecture.
+std::vector<std::string> compile_options(int device) {
   hipDeviceProp_t prop;
   std::string arch = "--offload-arch=gfx950";
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
@@ -181,23 +187,93 @@ JitEntry compile(const std::string &src, int device, int which) {
     const size_t colon = name.find(':');
     arch = "--offload-arch=" + name.substr(0, colon);
   }
-  const std::string inc = "-I" + header_dir();
   // (no atomic optimizer: it would turn the one-lane reservation atomic of
   // the second-level pass C into a wave-wide sequence that waits for the
   // returned value at once, where the kernel means to wait for it later;
   // the kernels' other uniform-address atomics are aggregated by hand)
-  const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", inc.c_str(), "-Xclang", "-target-feature", "-Xclang",
-                        "-packed-fp32-ops", "-mllvm", "-amdgpu-atomic-optimizer-strategy=None"};
-  const hiprtcResult r = hiprtcCompileProgram(prog, 10, opts);
-  size_t code_size = 0;
-  if (r == HIPRTC_SUCCESS && hiprtcGetCodeSize(prog, &code_size) == HIPRTC_SUCCESS && code_size > 0) {
-    std::vector<char> code(code_size);
-    if (hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS &&
-        hipModuleLoadData(&e.module, code.data()) == hipSuccess &&
-        hipModuleGetFunction(&e.fn, e.module, kJitNames[which]) == hipSuccess)
-      e.failed = false;
+  return {arch, "-O3", "-std=c++17", "-I" + header_dir(), "-Xclang", "-target-feature", "-Xclang",
+          "-packed-fp32-ops", "-mllvm", "-amdgpu-atomic-optimizer-strategy=None"};
+}
+
+// hiprtc in this process (when the compiler program cannot be started).
+bool compile_here(const std::string &src, const std::vector<std::string> &opts, std::vector<char> &code) {
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "dfr_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) return false;
+  std::vector<const char *> o;
+  for (const std::string &x : opts) o.push_back(x.c_str());
+  const hiprtcResult r = hiprtcCompileProgram(prog, (int)o.size(), o.data());
+  size_t n = 0;
+  bool ok = r == HIPRTC_SUCCESS && hiprtcGetCodeSize(prog, &n) == HIPRTC_SUCCESS && n > 0;
+  if (ok) {
+    code.resize(n);
+    ok = hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS;
   }
   hiprtcDestroyProgram(&prog);
+  return ok;
+}
+
+// The compile in a child process (dfr_jitc beside this library), so that
+// a compiler crash cannot end the caller's process. Returns 1 with the
+// code object, 0 when the compile failed or crashed, -1 when the compiler
+// program could not be started (then the caller compiles here).
+int compile_child(const std::string &src, const std::vector<std::string> &opts, std::vector<char> &code) {
+  if (const char *v = getenv("DFR_JIT_INPROCESS"))
+    if (*v && *v != '0') return -1;
+  const std::string dir = header_dir();
+  if (dir.size() < 5) return -1;
+  const std::string exe = dir.substr(0, dir.size() - 5) + "/dfr_jitc";  // (header_dir is <lib dir>/csrc)
+  if (access(exe.c_str(), X_OK) != 0) return -1;
+  const char *tmp = getenv("TMPDIR");
+  std::string base = std::string(tmp && *tmp ? tmp : "/tmp") + "/dfr_jit_XXXXXX";
+  std::vector<char> path(base.begin(), base.end());
+  path.push_back('\0');
+  const int fd = mkstemp(path.data());
+  if (fd < 0) return -1;
+  const std::string src_path = path.data(), out_path = src_path + ".co";
+  bool wrote = write(fd, src.data(), src.size()) == (ssize_t)src.size();
+  close(fd);
+  int result = -1;
+  if (wrote) {
+    std::vector<std::string> args = {exe, src_path, out_path};
+    args.insert(args.end(), opts.begin(), opts.end());
+    std::vector<char *> argv;
+    for (std::string &a : args) argv.push_back(&a[0]);
+    argv.push_back(nullptr);
+    pid_t pid;
+    if (posix_spawn(&pid, exe.c_str(), nullptr, nullptr, argv.data(), environ) == 0) {
+      int status = 0;
+      while (waitpid(pid, &status, 0) < 0 && errno == EINTR) {
+      }
+      result = 0;
+      if (WIFEXITED(status) && WEXITSTATUS(status) == 0) {
+        if (FILE *f = fopen(out_path.c_str(), "rb")) {
+          fseek(f, 0, SEEK_END);
+          const long n = ftell(f);
+          fseek(f, 0, SEEK_SET);
+          if (n > 0) {
+            code.resize((size_t)n);
+            if (fread(code.data(), 1, (size_t)n, f) == (size_t)n) result = 1;
+          }
+          fclose(f);
+        }
+      }
+    }
+  }
+  unlink(src_path.c_str());
+  unlink(out_path.c_str());
+  return result;
+}
+
+JitEntry compile(const std::string &src, int device, int which) {
+  JitEntry e;
+  e.failed = true;
+  const std::vector<std::string> opts = compile_options(device);
+  std::vector<char> code;
+  int got = compile_child(src, opts, code);
+  if (got < 0) got = compile_here(src, opts, code) ? 1 : 0;
+  if (got == 1 && hipModuleLoadData(&e.module, code.data()) == hipSuccess &&
+      hipModuleGetFunction(&e.fn, e.module, kJitNames[which]) == hipSuccess)
+    e.failed = false;
   return e;
 }
 

@@ -110,6 +110,18 @@ __device__ __forceinline__ void fold_full(const PV &P, uint64_t *st, int64_t row
   }
 }
 
+// An n-word (3 or 4) fixed-point total of a partial state added into `d`.
+// Not inlined: where a plan's word layout is a run-time value, its inlined
+// form (a carry chain of returning atomics under a run-time word count)
+// crashed the AMDGPU backend (RenameIndependentSubregs) in some shapes.
+template <class SrcPtr>
+__device__ __noinline__ void merge_wide(uint64_t *d, SrcPtr s, int n) {
+  uint64_t x[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) x[i] = i < n ? (uint64_t)s[i] : 0ull;
+  add_words(d, x, n);
+}
+
 // Merges a partial state `s` (W words) into `d`, word by word.
 template <class SrcPtr, class PV>
 __device__ __forceinline__ void merge_word(const PV &P, uint64_t *d, SrcPtr s, int w) {
@@ -123,14 +135,9 @@ __device__ __forceinline__ void merge_word(const PV &P, uint64_t *d, SrcPtr s, i
         add128(&d[w], v, s[w + 1]);
         break;
       case W_ADD192:
-      case W_ADD256: {
-        const int n = P.wop[w] == W_ADD192 ? 3 : 4;
-        uint64_t x[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) x[i] = i < n ? s[w + i] : 0ull;
-        add_words(&d[w], x, n);
+      case W_ADD256:
+        merge_wide(&d[w], &s[w], P.wop[w] == W_ADD192 ? 3 : 4);
         break;
-      }
       case W_MIN_S:
         if ((int64_t)v < (int64_t)d[w]) atomicMin((long long *)&d[w], (long long)v);
         break;

@@ -144,6 +144,7 @@ def _libgen():
         lib.dfrg_copy_to_device.argtypes = [vp, vp, u64]
         lib.dfrg_fill_zero.argtypes = [vp, u64]
         lib.dfrg_uniform.argtypes = [vp, ctypes.c_int, u64, u64, u64, i64, i64]
+        lib.dfrg_f64_keys.argtypes = [vp, vp, u64, ctypes.c_double]
         lib.dfrg_zipf_i32.argtypes = [vp, u64, u64, u64, vp, u64, i64]
         lib.dfrg_normal_f64.argtypes = [vp, u64, u64, u64, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         lib.dfrg_validity.argtypes = [vp, u64, u64, u64, u64]
@@ -205,6 +206,22 @@ def device_uniform(n, seed, stream, lo, hi, dtype=np.int64) -> DeviceArray:
     t = DeviceArray(n, dtype)
     _check(_libgen().dfrg_uniform(t.ptr, t.dtype.itemsize, n, seed, stream, lo, hi), "dfrg_uniform")
     return t
+
+
+def device_float_keys(n, seed, stream, card, scale=0.125) -> DeviceArray:
+    """Float64 keys k * scale for k uniform in [0, card) (k = 0 as -0.0);
+    the host twin is float_keys."""
+    t = device_uniform(n, seed, stream, 0, max(card, 1) - 1)
+    _check(_libgen().dfrg_f64_keys(t.ptr, t.ptr, n, scale), "dfrg_f64_keys")
+    t.dtype = np.dtype(np.float64)
+    return t
+
+
+def float_keys(seed, stream, n, card, scale=0.125) -> np.ndarray:
+    k = uniform(seed, stream, n, 0, max(card, 1) - 1)
+    out = k.astype(np.float64) * scale
+    out[k == 0] = -0.0
+    return out
 
 
 def device_zipf(n, seed, stream, s, m) -> DeviceArray:

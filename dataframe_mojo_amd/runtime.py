@@ -242,6 +242,10 @@ class Context:
             self.lib.dfr_context_destroy(self.handle)
             self.handle = None
 
+    def trim(self):
+        """Returns the device memory the context caches for reuse."""
+        self.lib.dfr_context_trim(self.handle)
+
     def tune(self, key: str, value: int):
         e = _err()
         _lib.check(self.lib.dfr_context_tune(self.handle, key.encode(), int(value), ctypes.byref(e)), e)

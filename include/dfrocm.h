@@ -108,6 +108,10 @@ typedef struct dfr_observation {
 } dfr_observation;
 int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
 
+/* Frees the device blocks the context keeps for reuse (released results'
+ * and queries' scratch); blocks in use are not touched. */
+void dfr_context_trim(dfr_context *ctx);
+
 /* Sets one launch or table parameter of the context (no counterpart in the
  * reference: the autotuner's knob). Keys: "block", "wg_per_cu",
  * "lds_bytes" (threads per workgroup, persistent workgroups per CU and LDS
@@ -132,7 +136,8 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * is captured and run as one HIP graph), "batch_part_rows" (a batch query
  * with at least this many rows whose key domain outgrows the batch kernel's
  * LDS table runs the partitioned passes instead, after the batch kernel;
- * 0: never; not under graph capture), "jit" (pass A compiled at run time
+ * 0: never; under graph capture their runtime-compiled kernels are built
+ * before the capture begins), "jit" (pass A compiled at run time
  * for the query's shape: 0 never, 1 when the input has at least "jit_rows"
  * rows, 2 always; the precompiled kernel runs when it cannot be built),
  * "jit_rows", "jit_wait" (0: a runtime-compiled kernel is used only when

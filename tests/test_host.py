@@ -155,3 +155,32 @@ def test_memory_limit_env_rejects_bad_values(monkeypatch):
         runtime.memory_limit()
     monkeypatch.setenv("DATAFRAME_ACCEL_MEMORY_LIMIT", "1048576")
     assert runtime.memory_limit() == 1048576
+
+
+def _jitc():
+    exe = os.path.join(ROOT, "dataframe_mojo_amd", "dfr_jitc")
+    if not os.path.exists(exe):
+        pytest.skip("dfr_jitc is not built")
+    return exe
+
+
+JITC_OPTS = ["--offload-arch=gfx950", "-O3", "-std=c++17"]
+
+
+def test_runtime_compiler_program_writes_a_code_object(tmp_path):
+    """dfr_jitc, which the library runs as a child process for each runtime
+    compile, turns a kernel source into a code object (exit 0), and reports
+    a source that does not compile with exit 2 and the compiler's log."""
+    import subprocess
+
+    exe = _jitc()
+    src = tmp_path / "k.hip"
+    src.write_text('extern "C" __global__ void k(int *p) { p[threadIdx.x] = threadIdx.x; }\n')
+    out = tmp_path / "k.co"
+    r = subprocess.run([exe, str(src), str(out), *JITC_OPTS], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert out.stat().st_size > 0 and out.read_bytes()[:4] == b"\x7fELF"
+    bad = tmp_path / "bad.hip"
+    bad.write_text('extern "C" __global__ void k(int *p) { p[threadIdx.x] = undefined_name; }\n')
+    r = subprocess.run([exe, str(bad), str(tmp_path / "bad.co"), *JITC_OPTS], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2 and "undefined_name" in r.stderr
