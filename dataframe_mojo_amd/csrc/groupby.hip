@@ -173,7 +173,26 @@ ColStats stats_identity() {
   init.flags = 0;
   init.pad = 0;
   init.valid = 0;
+  init.distinct = 0;
   return init;
+}
+
+// An upper estimate of the distinct values a HyperLogLog sketch of kHllRegs
+// registers saw (Flajolet et al.'s estimator with its small-range
+// correction; standard error 1.04 / sqrt(m), so three of them are added),
+// at most `valid`.
+uint64_t hll_upper(const uint32_t *regs, uint64_t valid) {
+  const double m = kHllRegs;
+  double inv = 0.0;
+  int zeros = 0;
+  for (int j = 0; j < kHllRegs; ++j) {
+    inv += ldexp(1.0, -(int)regs[j]);
+    zeros += regs[j] == 0;
+  }
+  double e = (0.7213 / (1.0 + 1.079 / m)) * m * m / inv;
+  if (e <= 2.5 * m && zeros > 0) e = m * log(m / zeros);
+  const double hi = e * (1.0 + 3.0 * 1.04 / sqrt(m)) + 16.0;
+  return std::min<uint64_t>(valid, (uint64_t)std::ceil(hi));
 }
 
 // Statistics of rows [0, col->length) of the column's window, computed on
@@ -189,7 +208,7 @@ int window_stats(dfr_context *ctx, const dfr_column *col, ColStats *out, char **
   const int64_t n = col->length;
   const int blocks = (int)std::min<int64_t>((n + 16 * kBlock - 1) / (16 * kBlock), (int64_t)ctx->cus * 2);
   if (e == hipSuccess && blocks > 0) {
-    hipLaunchKernelGGL(k_col_stats, dim3(blocks), dim3(kBlock), 0, ctx->stream, col->dev(), n, d);
+    hipLaunchKernelGGL(k_col_stats, dim3(blocks), dim3(kBlock), 0, ctx->stream, col->dev(), n, d, (uint32_t *)nullptr);
     ctx->obs.launches++;
   }
   if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(ColStats), hipMemcpyDeviceToHost, ctx->stream);
@@ -241,32 +260,33 @@ int kept_stats(dfr_context *ctx, const DevPlan &P, uint32_t cols, ColStats *st, 
 }  // namespace
 
 int column_stats(dfr_context *ctx, dfr_column *col, char **error) {
-  ColStats init;
-  init.imin = INT64_MAX;
-  init.imax = INT64_MIN;
-  init.emin = 0x7ff;
-  init.emax = 0;
-  init.flags = 0;
-  init.pad = 0;
-  init.valid = 0;
-  ColStats *d = nullptr;
-  int st = ctx->pool.alloc(sizeof(ColStats), (void **)&d, error);
+  // the statistics, then the distinct sketch's registers
+  const size_t stat_bytes = (sizeof(ColStats) + 255) / 256 * 256;
+  void *d = nullptr;
+  int st = ctx->pool.alloc(stat_bytes + kHllRegs * 4, &d, error);
   if (st) return st;
+  ColStats *ds = (ColStats *)d;
+  uint32_t *dh = (uint32_t *)((char *)d + stat_bytes);
   ColStats *h = (ColStats *)ctx->staging;
-  *h = init;
-  HIP_TRY(hipMemcpyAsync(d, h, sizeof init, hipMemcpyHostToDevice, ctx->stream));
+  *h = stats_identity();
+  const uint32_t *hregs = (const uint32_t *)(ctx->staging + stat_bytes);
+  HIP_TRY(hipMemcpyAsync(ds, h, sizeof(ColStats), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_fill_u64, dim3(kHllRegs / 2 / kBlock), dim3(kBlock), 0, ctx->stream, (uint64_t *)dh,
+                     (uint64_t)(kHllRegs / 2), 0ull);
   DevCol C = col->dev();
   int64_t n = col->length;
   // at least 16 rows a thread, at most two blocks a CU
   int blocks = (int)std::min<int64_t>((n + 16 * kBlock - 1) / (16 * kBlock), (int64_t)ctx->cus * 2);
   if (blocks > 0) {
-    hipLaunchKernelGGL(k_col_stats, dim3(blocks), dim3(kBlock), 0, ctx->stream, C, n, d);
+    hipLaunchKernelGGL(k_col_stats, dim3(blocks), dim3(kBlock), 0, ctx->stream, C, n, ds, dh);
     ctx->obs.launches++;
   }
-  HIP_TRY(hipMemcpyAsync(h, d, sizeof(ColStats), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(h, ds, sizeof(ColStats), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync((void *)hregs, dh, kHllRegs * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   ctx->obs.waits++;
   ctx->obs.completed = ctx->obs.launches;
+  h->distinct = hll_upper(hregs, h->valid);
   col->stats = *h;
   col->nulls = n - (int64_t)h->valid;
   ctx->pool.release(d);
@@ -635,12 +655,30 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
   return DFR_OK;
 }
 
-// Table mode and launch shape from the tuning and the plan.
+// An upper bound of the query's groups from its key columns' distinct
+// estimates (made at hand-off; a null key is one more group): their
+// product, or UINT64_MAX when a key has none.
+static uint64_t distinct_bound(const Compiled &C) {
+  unsigned __int128 b = 1;
+  for (int k = 0; k < C.plan.n_keys; ++k) {
+    const dfr_column *c = C.cols[k];
+    if (c->stats.distinct == 0 && c->stats.valid > 0) return UINT64_MAX;
+    b *= (unsigned __int128)c->stats.distinct + (c->nulls > 0 ? 1 : 0);
+    if (b > ((unsigned __int128)1 << 62)) return UINT64_MAX;
+  }
+  return (uint64_t)std::max<unsigned __int128>(b, 1);
+}
+
+// Table mode and launch shape from the tuning and the plan. The groups a
+// hash table and the partition fan-out are sized for are at most the key
+// domain, the rows and the distinct bound; a table that fills up anyway
+// raises the capacity fault, and the query runs again with a larger one.
 void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_boost, const Tuning *tune) {
   DevPlan &P = C.plan;
   const Tuning &T = tune ? *tune : ctx->tune;
   const int64_t rows = C.rows;
   const uint64_t D = P.domain;
+  const uint64_t G_hi = std::min<uint64_t>(std::min<uint64_t>(D, (uint64_t)std::max<int64_t>(rows, 1)), distinct_bound(C));
   const size_t slot_bytes = (size_t)P.words * 8;
   P.lds_stride = T.lds_odd_stride ? (P.words | 1) : P.words;
   P.c_stride = T.part_odd_stride ? (P.words | 1) : P.words;
@@ -652,8 +690,7 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
     P.cap_log2 = 1;
   } else {
     P.mode = MODE_HASH;
-    uint64_t want = hash_slots > 0 ? (uint64_t)hash_slots
-                                   : next_pow2(std::max<uint64_t>(64, 2 * std::min<uint64_t>(D, (uint64_t)rows)));
+    uint64_t want = hash_slots > 0 ? (uint64_t)hash_slots : next_pow2(std::max<uint64_t>(64, 2 * G_hi));
     P.cap = next_pow2(want);
     P.cap_log2 = log2_exact(P.cap);
   }
@@ -697,7 +734,7 @@ void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_bo
     while (ce * 2 * 8ull * (S + 1) <= c_budget && ce < (uint64_t)kMaxPartEntries) ce *= 2;
     P.c_entries = (int32_t)ce;
     P.c_log2 = log2_exact(ce);
-    const uint64_t groups_est = std::max<uint64_t>(1, std::min<uint64_t>(D, (uint64_t)std::max<int64_t>(rows, 1)));
+    const uint64_t groups_est = std::max<uint64_t>(1, G_hi);
     const uint64_t max_parts = (uint64_t)T.max_parts;
     uint64_t np = next_pow2((2 * groups_est + ce - 1) / ce);
     np = std::min<uint64_t>(std::max<uint64_t>(np, 1), max_parts);

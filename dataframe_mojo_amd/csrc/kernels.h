@@ -975,7 +975,30 @@ __global__ void k_finalize(const DevPlan *__restrict__ plan) {
 
 // ------------------------------------------------------- column stats
 
-__global__ void k_col_stats(DevCol C, int64_t n, ColStats *out) {
+// A HyperLogLog sketch of a column's distinct values (its group-key words:
+// Float64 values canonical, as grouping sees them): kHllRegs registers, each
+// the largest rank (leading zeros + 1) of the hashed words it received.
+constexpr int kHllBits = 12;
+constexpr int kHllRegs = 1 << kHllBits;
+
+__device__ __forceinline__ void hll_add(uint32_t *regs, uint64_t word) {
+  uint64_t h = word * 0x9E3779B97F4A7C15ull;  // SplitMix64's finalizer
+  h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+  h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+  h ^= h >> 31;
+  const uint32_t j = (uint32_t)(h >> (64 - kHllBits));
+  const uint32_t rank = (uint32_t)__builtin_clzll((h << kHllBits) | (1ull << (kHllBits - 1))) + 1u;
+  if (rank > regs[j]) atomicMax(&regs[j], rank);
+}
+
+// One pass over a column: min/max, exponent range, flags, valid count, and
+// (when `hll` is given: kHllRegs zeroed registers) the distinct sketch.
+__global__ void k_col_stats(DevCol C, int64_t n, ColStats *out, uint32_t *hll) {
+  __shared__ uint32_t regs[kHllRegs];
+  if (hll) {
+    for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x) regs[i] = 0;
+    __syncthreads();
+  }
   int64_t lo = INT64_MAX, hi = INT64_MIN;
   uint32_t emin = 0x7ff, emax = 0, flags = 0;
   uint64_t cnt = 0;
@@ -988,6 +1011,7 @@ __global__ void k_col_stats(DevCol C, int64_t n, ColStats *out) {
     ++cnt;
     if (C.dtype == T_FLOAT64) {
       uint64_t b = ((const uint64_t *)C.values)[r];
+      if (hll) hll_add(regs, canon_key_bits(b));
       uint32_t e = (uint32_t)(b >> 52) & 0x7ff;
       if (e == 0x7ff) {
         flags |= (b << 12) ? 1u : ((b >> 63) ? 4u : 2u);
@@ -1007,9 +1031,15 @@ __global__ void k_col_stats(DevCol C, int64_t n, ColStats *out) {
         v = ((const int32_t *)C.values)[r];
       else
         v = ((const uint32_t *)C.values)[r];
+      if (hll) hll_add(regs, (uint64_t)v);
       lo = min(lo, v);
       hi = max(hi, v);
     }
+  }
+  if (hll) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x)
+      if (regs[i]) atomicMax(&hll[i], regs[i]);
   }
   for (int d = 32; d >= 1; d >>= 1) {
     lo = min(lo, (int64_t)__shfl_xor((long long)lo, d, 64));

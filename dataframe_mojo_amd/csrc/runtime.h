@@ -20,6 +20,7 @@ struct ColStats {
   uint32_t flags;       // FLOAT64: 1 NaN, 2 +inf, 4 -inf, 8 -0.0, 16 +0.0, 32 finite non-zero
   uint32_t pad;
   uint64_t valid;       // non-null values
+  uint64_t distinct;    // an upper estimate of distinct non-null values (0: not estimated)
 };
 
 // Launch and table parameters. Defaults come from the device; the autotuner
