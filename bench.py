@@ -224,8 +224,7 @@ def bench_batch(args) -> dict:
     log(f"generating {len(params)} queries' columns on the device: {total_rows} rows")
     cols = datagen.batch_4096_device(params, args.seed)
     ctx = runtime.context(0)
-    if args.batch_graph:
-        ctx.tune("batch_graph", 1)
+    ctx.tune("batch_graph", 1 if args.batch_graph else 0)
     jobs, prepared = [], []
     for p, c in zip(params, cols):
         n = p["rows"]
@@ -290,6 +289,7 @@ def bench_batch(args) -> dict:
     release(out)
     release(plain)
     log(f"batched == plain on {checked} queries")
+    ctx.tune("batch_graph", 0)
 
     bytes_per_row = 8 + 8 + 8 + 1 / 8  # key, value, x, x validity
     out_bytes = groups * (8 + 5 * 8 + 3 / 8)
@@ -346,11 +346,13 @@ def bench_serve(args) -> dict:
     card = np.exp(rng.uniform(0.0, np.log(1e5), n_tables)).astype(np.int64).clip(1)
     log(f"generating {n_tables} tables on the device: {int(rows.sum())} rows")
     ctx = runtime.context(0)
+    ctx.tune("batch_graph", args.serve_graph)  # the lanes take the context's tuning
     labels = datagen.dict_labels()
+    kinds = [{"int": 0, "dict": 1, "float": 2}[k] for k in args.serve_keys.split(",")]
     tabs = []
     for t in range(n_tables):
         n, base = int(rows[t]), 5000 + 8 * t
-        kind = t % 3  # Int64, dictionary String, Float64 keys
+        kind = kinds[t % len(kinds)]  # 0 Int64, 1 dictionary String, 2 Float64 keys
         dict_key = kind == 1
         hi = min(int(card[t]), 1000) if dict_key else int(card[t])
         if kind == 2:
@@ -487,7 +489,8 @@ def bench_serve(args) -> dict:
                    "keys": "Int64, dictionary String and Float64 (cardinality log-uniform [1, 1e5]), in turn",
                    "arrivals": f"Poisson {args.rate:.0f}/s for {args.duration} s", "queries": n_q,
                    "query": "window rows log-uniform [1e3, 1e7], value > t, len/sum/max(value), sum/mean(x)",
-                   "lanes": args.lanes, "max_batch": args.max_batch, "budget_bytes": args.serve_budget},
+                   "lanes": args.lanes, "max_batch": args.max_batch, "budget_bytes": args.serve_budget,
+                   "graph": bool(args.serve_graph)},
         "served": {**srv_res, "rows_per_s": total_rows / served_s},
         "serial_plain": plain_res,
         "overload": over,
@@ -531,6 +534,8 @@ def main() -> int:
     ap.add_argument("--max-batch", type=int, default=256, help="serve: queries per batch at most")
     ap.add_argument("--serve-budget", type=int, default=0, help="serve: admission budget in bytes (0: none)")
     ap.add_argument("--overload", type=float, default=4.0, help="serve: also run the arrivals this many times faster")
+    ap.add_argument("--serve-keys", default="int,dict,float", help="serve: key types of the tables, in turn")
+    ap.add_argument("--serve-graph", type=int, default=0, help="serve: lanes capture each batch as a HIP graph")
     ap.add_argument("--queries", type=int, default=4096, help="batch_4096: queries per batch")
     ap.add_argument("--batch-graph", type=int, default=0, help="batch_4096: run the batch as a captured HIP graph")
     ap.add_argument("--rows", type=int, default=0, help="rows (default: the workload's own)")

@@ -56,6 +56,7 @@ EXPORTS = [
     "dfr_column_release",
     "dfr_column_length",
     "dfr_column_null_count",
+    "dfr_column_distinct",
     "dfr_groupby",
     "dfr_groupby_batch",
     "dfr_column_slice",
@@ -190,6 +191,8 @@ def load(required: bool = True):
     lib.dfr_column_length.restype = i64
     lib.dfr_column_null_count.argtypes = [vp]
     lib.dfr_column_null_count.restype = i64
+    lib.dfr_column_distinct.argtypes = [vp]
+    lib.dfr_column_distinct.restype = i64
     lib.dfr_groupby.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(vp), errp]
     lib.dfr_column_slice.argtypes = [vp, i64, i64, ctypes.POINTER(vp), errp]
     lib.dfr_server_create.argtypes = [vp, i32, i32, i64, ctypes.POINTER(vp), errp]

@@ -486,6 +486,7 @@ extern "C" void dfr_column_release(dfr_column *col) {
 
 extern "C" int64_t dfr_column_length(const dfr_column *col) { return col ? col->length : -1; }
 extern "C" int64_t dfr_column_null_count(const dfr_column *col) { return col ? col->nulls : -1; }
+extern "C" int64_t dfr_column_distinct(const dfr_column *col) { return col ? (int64_t)col->stats.distinct : -1; }
 
 // ------------------------------------------------------------ results
 

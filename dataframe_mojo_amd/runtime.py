@@ -72,6 +72,10 @@ class DeviceColumn:
     def null_count(self) -> int:
         return int(self.ctx.lib.dfr_column_null_count(self.handle))
 
+    def distinct(self) -> int:
+        """Upper estimate of the distinct non-null values (made at hand-off)."""
+        return int(self.ctx.lib.dfr_column_distinct(self.handle))
+
     def slice(self, offset: int, length: int) -> "DeviceColumn":
         """Rows [offset, offset + length) as a column of their own, borrowing
         this one's buffers and statistics (no device work)."""

@@ -193,6 +193,10 @@ int dfr_column_slice(const dfr_column *parent, int64_t offset, int64_t length, d
 void dfr_column_release(dfr_column *column);
 int64_t dfr_column_length(const dfr_column *column);
 int64_t dfr_column_null_count(const dfr_column *column);
+/* An upper estimate of the column's distinct non-null values, from a
+ * HyperLogLog sketch made at hand-off (a slice reports its parent's); it
+ * bounds the groups a key column can form, which sizes hash tables. */
+int64_t dfr_column_distinct(const dfr_column *column);
 
 /* ---- the query ---------------------------------------------------------- */
 /* Predicate leaves: `column <op> literal`, or `floor_mod(column, modulus)
