@@ -59,6 +59,7 @@ EXPORTS = [
     "dfr_column_distinct",
     "dfr_groupby",
     "dfr_groupby_batch",
+    "dfr_groupby_into",
     "dfr_column_slice",
     "dfr_server_create",
     "dfr_server_submit",
@@ -152,6 +153,19 @@ class Query(ctypes.Structure):
     ]
 
 
+class OutputColumn(ctypes.Structure):
+    _fields_ = [("values", ctypes.c_void_p), ("validity", ctypes.c_void_p)]
+
+
+class Output(ctypes.Structure):
+    _fields_ = [
+        ("capacity", ctypes.c_int64),
+        ("keys", ctypes.POINTER(OutputColumn)),
+        ("aggs", ctypes.POINTER(OutputColumn)),
+        ("first_rows", ctypes.c_void_p),
+    ]
+
+
 def library_path() -> str:
     return os.environ.get(LIBRARY_ENV) or os.path.join(HERE, "libdfrocm.so")
 
@@ -194,6 +208,7 @@ def load(required: bool = True):
     lib.dfr_column_distinct.argtypes = [vp]
     lib.dfr_column_distinct.restype = i64
     lib.dfr_groupby.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(vp), errp]
+    lib.dfr_groupby_into.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(Output), ctypes.POINTER(i64), errp]
     lib.dfr_column_slice.argtypes = [vp, i64, i64, ctypes.POINTER(vp), errp]
     lib.dfr_server_create.argtypes = [vp, i32, i32, i64, ctypes.POINTER(vp), errp]
     lib.dfr_server_submit.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(i64), errp]

@@ -231,6 +231,11 @@ extern "C" int dfr_context_create(int32_t device, int64_t budget_bytes, dfr_cont
     return dfr_fail(error, DFR_DEVICE, "context set-up: %s", hipGetErrorString(e));
   }
   ctx->tune = default_tuning(ctx);
+  int st = noalloc_init(device, error);
+  if (st) {
+    dfr_context_destroy(ctx);
+    return st;
+  }
   *out = ctx;
   return DFR_OK;
 }

@@ -22,7 +22,10 @@ struct Compiled {
   uint64_t *scan2_par = nullptr;
 };
 
-int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **error);
+// may_allocate false: the query's columns' own statistics must settle its
+// fixed-point widths (no window or kept-row statistics pass, which needs
+// device scratch); otherwise DFR_UNSUPPORTED.
+int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **error, bool may_allocate = true);
 // `tune` overrides the context's tuning (null: the context's).
 void choose_layout(dfr_context *ctx, Compiled &C, int64_t hash_slots, int sub_boost,
                    const dfr::Tuning *tune = nullptr);

@@ -295,7 +295,7 @@ int column_stats(dfr_context *ctx, dfr_column *col, char **error) {
 
 // ------------------------------------------------------------ compiler
 
-int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **error) {
+int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **error, bool may_allocate) {
   if (!q) return dfr_fail(error, DFR_INVALID, "query is null");
   if (q->n_keys < 1 || q->n_keys > kMaxKeys || !q->keys)
     return dfr_fail(error, DFR_UNSUPPORTED, "the device path groups by 1 to %d keys", kMaxKeys);
@@ -547,7 +547,7 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
     };
     uint32_t bad;
     settle(bad);
-    if (bad) {
+    if (bad && may_allocate) {
       for (int c = 0; c < P.n_cols; ++c) {
         if (!((bad >> c) & 1u) || C.cols[c]->stats_window) continue;
         int st2 = window_stats(ctx, C.cols[c], &st[c], error);
@@ -555,7 +555,7 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
       }
       settle(bad);
     }
-    if (bad) {
+    if (bad && may_allocate) {
       int st2 = kept_stats(ctx, P, bad, st, error);
       if (st2) return st2;
       settle(bad);
@@ -1368,3 +1368,4 @@ extern "C" int dfr_groupby(dfr_context *ctx, const dfr_query *query, dfr_result 
 }
 
 #include "batch.h"
+#include "noalloc.h"

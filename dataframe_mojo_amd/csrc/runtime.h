@@ -157,5 +157,7 @@ struct dfr_buffer {
 };
 
 int dfr_fail(char **error, int status, const char *fmt, ...);
+// Looks up a device's address of the no-alloc scratch array (once per device).
+int noalloc_init(int device, char **error);
 dfr::Tuning default_tuning(dfr_context *ctx);
 int column_stats(dfr_context *ctx, dfr_column *col, char **error);

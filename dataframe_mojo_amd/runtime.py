@@ -319,6 +319,25 @@ class Context:
         return Result(self, h, len(keys), len(plan.aggs))
 
 
+    def groupby_into(self, plan: DevicePlan, keys: list, columns: list, out) -> int:
+        """dfr_groupby_into: the query into caller-owned device buffers,
+        allocating nothing. `out` is (capacity, [(values_ptr, validity_ptr)]
+        per key, [(values_ptr, validity_ptr)] per aggregate, first_rows_ptr
+        or 0). Returns the group count."""
+        capacity, kbufs, abufs, first = out
+        q, keep = build_query(plan, keys, columns)
+        ko = (_lib.OutputColumn * max(1, len(kbufs)))(*[_lib.OutputColumn(v, m) for v, m in kbufs])
+        ao = (_lib.OutputColumn * max(1, len(abufs)))(*[_lib.OutputColumn(v, m) for v, m in abufs])
+        o = _lib.Output(capacity, ctypes.cast(ko, ctypes.POINTER(_lib.OutputColumn)),
+                        ctypes.cast(ao, ctypes.POINTER(_lib.OutputColumn)), first or None)
+        g = ctypes.c_int64()
+        e = _err()
+        with self.lock:
+            _lib.check(self.lib.dfr_groupby_into(self.handle, ctypes.byref(q), ctypes.byref(o), ctypes.byref(g),
+                                                 ctypes.byref(e)), e)
+        del keep
+        return g.value
+
     def groupby_batch(self, jobs) -> list:
         """Runs [(plan, keys, columns), ...] (or a PreparedBatch) as one
         dfr_groupby_batch call. Returns one entry per job: a Result, or the

@@ -388,6 +388,36 @@ int64_t dfr_buffer_copy_to_host(void *buffer, void *host_dst, int64_t bytes);
 /* _ReleaseDevice. */
 void dfr_buffer_release(void *buffer);
 
+/* ---- a group-by that allocates nothing ---------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid it).
+ * dfr_groupby_into runs a query like dfr_groupby, but writes the result
+ * into the caller's device buffers and calls no hipMalloc or hipFree (nor
+ * any allocating library routine). Its other device memory (plan, group
+ * table, compaction arrays, under maintain_order the row bitmap) comes from
+ * a fixed per-GPU scratch array of 64 MiB held until the process ends, so a
+ * query whose table and arrays exceed it fails with DFR_CAPACITY before any
+ * work, as does one with more groups than `capacity`. Every call on one GPU
+ * shares that GPU's scratch: calls that overlap on one GPU race, so callers
+ * serialise them. It runs the one-pass aggregation (no partition buffers),
+ * and its columns' own statistics must settle the Float64 accumulators
+ * (DFR_UNSUPPORTED otherwise). On DFR_OK, *groups holds the group count:
+ * rows [0, groups) of every output, in the order dfr_groupby would give
+ * under maintain_order, unspecified otherwise. */
+typedef struct dfr_output_column {
+    void *values;       /* device: capacity values of the column's width (4 bytes for
+                           DFR_INT32/DFR_DICT, else 8) */
+    uint64_t *validity; /* device: capacity / 64 + 1 words of LSB-first bits */
+} dfr_output_column;
+typedef struct dfr_output {
+    int64_t capacity;          /* groups the buffers hold */
+    dfr_output_column *keys;   /* query->n_keys of them (host array) */
+    dfr_output_column *aggs;   /* query->n_aggs of them (host array) */
+    int64_t *first_rows;       /* device, capacity entries; required under maintain_order */
+} dfr_output;
+int dfr_groupby_into(dfr_context *ctx, const dfr_query *query, const dfr_output *out, int64_t *groups,
+                     char **error);
+
 /* Milliseconds the device spent on the query's kernels (HIP events on the
  * context's stream). */
 double dfr_result_device_ms(const dfr_result *result);
