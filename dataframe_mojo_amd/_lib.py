@@ -106,6 +106,7 @@ class Observation(ctypes.Structure):
             "queries",
             "jit_compiles",
             "jit_launches",
+            "hot_keys",
         )
     ]
 

@@ -342,6 +342,16 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.scatter_slices = (int)value;
   else if (k == "scatter_stage" && (value == 0 || value == 1))
     t.scatter_stage = (int)value;
+  else if (k == "hot_keys" && value >= -1 && value <= 1)
+    t.hot_keys = (int)value;
+  else if (k == "hot_rows" && value >= 0)
+    t.hot_rows = value;
+  else if (k == "hot_domain" && value >= 0 && value <= ((int64_t)1 << 28))
+    t.hot_domain = value;
+  else if (k == "hot_sample" && value >= 4096 && value <= (1 << 28))
+    t.hot_sample = (int)value;
+  else if (k == "hot_share" && value >= 0 && value <= 100)
+    t.hot_share = (int)value;
   else if (k == "scatter_chunk" && (value == 8192 || value == 16384))
     t.scatter_chunk = (int)value;
   else if (k == "two_level_parts" && value >= 1 && value <= 4096 && (value & (value - 1)) == 0)
@@ -452,7 +462,7 @@ extern "C" int dfr_column_wrap(dfr_context *ctx, int32_t dtype, const void *devi
   col->vbit = bit_offset;
   int st = column_stats(ctx, col, error);
   if (st) {
-    delete col;
+    dfr_column_release(col);  // (owns nothing of the caller's)
     return st;
   }
   *out = col;
@@ -486,6 +496,7 @@ extern "C" void dfr_column_release(dfr_column *col) {
   if (!col) return;
   if (col->owned_values) col->ctx->pool.release(col->owned_values);
   if (col->owned_valid) col->ctx->pool.release(col->owned_valid);
+  if (col->hot) col->ctx->pool.release(col->hot);
   delete col;
 }
 

@@ -20,7 +20,20 @@ struct Compiled {
   size_t c_lds_bytes = 0;  // partition tables (pass C)
   uint64_t *scan2_raw = nullptr;  // block sums of the second level's scans
   uint64_t *scan2_par = nullptr;
+  // hot-key seeding of pass A (plan.hot_image set): sampled counts per
+  // packed key, their histogram, the sorted candidates, keys placed
+  uint32_t *hot_counts = nullptr;
+  uint32_t *hot_hist = nullptr;
+  uint64_t *hot_list = nullptr;
+  uint32_t *hot_out = nullptr;
+  bool hot_sampled = false;  // the query samples its own rows (else: its key column's list)
+  int64_t hot_delta = 0;     // listed key + delta = packed key, placed when below hot_limit
+  uint64_t hot_limit = 0;
 };
+
+// Words of a column's heavy-hitter buffer: candidates and their counter,
+// then the sample's histogram (32-bit bins) and row total.
+size_t column_hot_words();
 
 // may_allocate false: the query's columns' own statistics must settle its
 // fixed-point widths (no window or kept-row statistics pass, which needs

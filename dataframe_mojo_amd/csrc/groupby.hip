@@ -155,6 +155,7 @@ Tuning default_tuning(dfr_context *ctx) {
   t.scatter_slices = env_int("DFR_SCATTER_SLICES", 1);
   t.scatter_chunk = env_int("DFR_SCATTER_CHUNK", 16384) == 8192 ? 8192 : 16384;
   t.scatter_stage = env_int("DFR_SCATTER_STAGE", 1) ? 1 : 0;
+  t.hot_keys = std::max(-1, std::min(1, env_int("DFR_HOT_KEYS", -1)));
   t.batch_part_rows = env_int("DFR_BATCH_PART_ROWS", 1 << 22);
   t.scan_block = env_int("DFR_SCAN_BLOCK", 1024);
   t.scan_wg_per_cu = env_int("DFR_SCAN_WG_PER_CU", 1);
@@ -259,6 +260,65 @@ int kept_stats(dfr_context *ctx, const DevPlan &P, uint32_t cols, ColStats *st, 
 
 }  // namespace
 
+size_t column_hot_words() { return (size_t)kHotList + 1 + (kHotBins + 2) / 2; }
+
+namespace {
+
+int fill(dfr_context *ctx, uint64_t *p, uint64_t n, uint64_t value);
+
+// The heavy hitters of a sample of an integer or dictionary column's valid
+// values at hand-off (columns of hot_rows rows or more whose value range is
+// at most hot_domain): counted like a query's sample, without a filter, and
+// kept as the column's candidate list for queries grouped by it alone.
+int column_hot(dfr_context *ctx, dfr_column *col, char **error) {
+  const Tuning &T = ctx->tune;
+  const ColStats &st = col->stats;
+  if (T.hot_keys == 0 || col->length < T.hot_rows || st.valid == 0) return DFR_OK;
+  if (col->dtype != DFR_INT32 && col->dtype != DFR_INT64 && col->dtype != DFR_DICT) return DFR_OK;
+  const uint64_t range = (uint64_t)st.imax - (uint64_t)st.imin + 1;
+  if (range == 0 || range > (uint64_t)T.hot_domain) return DFR_OK;
+  void *hot = nullptr, *counts = nullptr;
+  int rc = ctx->pool.alloc(column_hot_words() * 8, &hot, error);
+  if (rc) return rc;
+  if ((rc = ctx->pool.alloc((range + 2) * 4, &counts, error))) {
+    ctx->pool.release(hot);
+    return rc;
+  }
+  uint64_t *list = (uint64_t *)hot;
+  uint32_t *hist = (uint32_t *)(list + kHotList + 1);
+  fill(ctx, (uint64_t *)counts, (range + 1) / 2, 0);
+  fill(ctx, (uint64_t *)hist, (kHotBins + 2) / 2, 0);
+  fill(ctx, list + kHotList, 1, 0);
+  const int64_t span = 1024, n = col->length;
+  const int64_t nblk = std::max<int64_t>(1, std::min<int64_t>((n + span - 1) / span, (int64_t)T.hot_sample / span));
+  const int64_t stride = (nblk == 1 || nblk * span >= n) ? span : (n - span) / (nblk - 1);
+  hipLaunchKernelGGL(k_col_hot_count, dim3((unsigned)nblk), dim3(kHotSampleThreads), 0, ctx->stream, col->dev(), n,
+                     st.imin, range, (uint32_t *)counts, stride, span);
+  const unsigned hgrid =
+      (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((range + 255) / 256, (uint64_t)ctx->cus * 4));
+  hipLaunchKernelGGL(k_hot_hist, dim3(hgrid), dim3(256), 0, ctx->stream, (const uint32_t *)counts, range, hist);
+  const unsigned cgrid = (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>((range + kHotSelectThreads - 1) / kHotSelectThreads, (uint64_t)ctx->cus));
+  hipLaunchKernelGGL(k_hot_collect, dim3(cgrid), dim3(kHotSelectThreads), 0, ctx->stream, (const uint32_t *)counts,
+                     range, (const uint32_t *)hist, (uint32_t)kHotList, list);
+  ctx->obs.launches += 6;
+  // complete before the column is handed out: other contexts' streams read it
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->obs.waits++;
+  ctx->obs.completed = ctx->obs.launches;
+  ctx->pool.release(counts);
+  if (e != hipSuccess) {
+    ctx->pool.release(hot);
+    return dfr_fail(error, DFR_DEVICE, "column heavy hitters: %s", hipGetErrorString(e));
+  }
+  col->hot = list;
+  col->hot_min = st.imin;
+  return DFR_OK;
+}
+
+}  // namespace
+
 int column_stats(dfr_context *ctx, dfr_column *col, char **error) {
   // the statistics, then the distinct sketch's registers
   const size_t stat_bytes = (sizeof(ColStats) + 255) / 256 * 256;
@@ -290,7 +350,7 @@ int column_stats(dfr_context *ctx, dfr_column *col, char **error) {
   col->stats = *h;
   col->nulls = n - (int64_t)h->valid;
   ctx->pool.release(d);
-  return DFR_OK;
+  return column_hot(ctx, col, error);
 }
 
 // ------------------------------------------------------------ compiler
@@ -959,11 +1019,92 @@ int launch_scan_cols(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, 
   return launch_scan<LDSM, kMaxCols, 1>(ctx, C, dplan, error);
 }
 
+// Whether a query's pass A seeds its table with sampled heavy-hitter keys:
+// a one-level partitioned query with a hashed LDS table over a packed key
+// domain small enough to count every key of a sample.
+bool hot_wanted(const dfr_context *ctx, const Compiled &C, bool dense, bool col_list) {
+  const DevPlan &P = C.plan;
+  const Tuning &T = ctx->tune;
+  if (T.hot_keys == 0 || dense || P.agg_mode != AGG_PARTITION || P.lds_mode != LDS_HASHED) return false;
+  if (P.lds_entries < 4 || C.rows <= 0) return false;
+  if (!col_list && (P.domain == 0 || P.domain > (uint64_t)T.hot_domain)) return false;
+  return T.hot_keys == 1 || C.rows >= T.hot_rows;
+}
+
+template <int NC>
+void launch_hot_count(dfr_context *ctx, const DevPlan *dplan, uint32_t *counts, uint64_t domain, unsigned blocks,
+                      int64_t stride, int64_t span) {
+  hipLaunchKernelGGL(k_hot_count<NC>, dim3(blocks), dim3(kHotSampleThreads), 0, ctx->stream, dplan, counts, domain,
+                     stride, span);
+}
+
+// The seeded keys: sampled counts, their histogram, the candidates above
+// the count threshold, and the image of pass A's table they are placed in.
+int launch_hot_select(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
+  const DevPlan &P = C.plan;
+  const Tuning &T = ctx->tune;
+  const uint64_t D = P.domain;
+  const unsigned long long *total = (const unsigned long long *)(C.hot_hist + kHotBins);
+  if (!C.hot_sampled) {
+    // the key column's list
+    hipLaunchKernelGGL(k_hot_select, dim3(1), dim3(kHotSelectThreads), 0, ctx->stream, C.hot_list, total,
+                       C.hot_delta, C.hot_limit, P.hot_image, P.lds_entries, (float)T.hot_share / 100.0f,
+                       C.hot_out);
+    ctx->obs.launches += 1;
+    HIP_TRY(hipGetLastError());
+    return DFR_OK;
+  }
+  fill(ctx, (uint64_t *)C.hot_counts, (D + 1) / 2, 0);
+  fill(ctx, (uint64_t *)C.hot_hist, (kHotBins + 2) / 2, 0);
+  fill(ctx, C.hot_list + kHotList, 1, 0);
+  // blocks of `span` rows spread evenly over the rows (even row offsets, as
+  // the paired loads need)
+  const int64_t span = 1024, rows = C.rows;
+  const int64_t nblk =
+      std::max<int64_t>(1, std::min<int64_t>((rows + span - 1) / span, (int64_t)T.hot_sample / span));
+  const int64_t stride = (nblk == 1 || nblk * span >= rows) ? span : ((rows - span) / (nblk - 1)) & ~(int64_t)1;
+  int ldsm, nc, u;
+  scan_shape(ctx, C, &ldsm, &nc, &u);
+  if (nc == 4)
+    launch_hot_count<4>(ctx, dplan, C.hot_counts, D, (unsigned)nblk, stride, span);
+  else if (nc == 8)
+    launch_hot_count<8>(ctx, dplan, C.hot_counts, D, (unsigned)nblk, stride, span);
+  else
+    launch_hot_count<kMaxCols>(ctx, dplan, C.hot_counts, D, (unsigned)nblk, stride, span);
+  const unsigned hgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((D + 255) / 256, (uint64_t)ctx->cus * 4));
+  hipLaunchKernelGGL(k_hot_hist, dim3(hgrid), dim3(256), 0, ctx->stream, C.hot_counts, D, C.hot_hist);
+  const unsigned cgrid =
+      (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((D + kHotSelectThreads - 1) / kHotSelectThreads, (uint64_t)ctx->cus));
+  // about twice as many candidates as the table has entries
+  const uint32_t want = (uint32_t)std::min<int64_t>(kHotList, 2 * (int64_t)P.lds_entries);
+  hipLaunchKernelGGL(k_hot_collect, dim3(cgrid), dim3(kHotSelectThreads), 0, ctx->stream, C.hot_counts, D,
+                     C.hot_hist, want, C.hot_list);
+  hipLaunchKernelGGL(k_hot_select, dim3(1), dim3(kHotSelectThreads), 0, ctx->stream, C.hot_list, total, (int64_t)0,
+                     D, P.hot_image, P.lds_entries, (float)T.hot_share / 100.0f, C.hot_out);
+  ctx->obs.launches += 7;
+  HIP_TRY(hipGetLastError());
+  return DFR_OK;
+}
+
+// The seeded entries' states over pass A's workgroups, into the group table.
+int launch_hot_reduce(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
+  const DevPlan &P = C.plan;
+  const int groups = std::max(1, std::min(16, P.n_segs));
+  const uint64_t ew = (uint64_t)P.lds_entries * (uint64_t)P.words;
+  hipLaunchKernelGGL(k_hot_reduce, dim3((unsigned)((ew * groups + 255) / 256)), dim3(256), 0, ctx->stream, dplan,
+                     groups, 0);
+  hipLaunchKernelGGL(k_hot_reduce, dim3((unsigned)((ew + 255) / 256)), dim3(256), 0, ctx->stream, dplan, groups, 1);
+  ctx->obs.launches += 2;
+  HIP_TRY(hipGetLastError());
+  return DFR_OK;
+}
+
 // Passes A, B and C of the partitioned aggregation. `events`: record the
 // phase boundary (off under graph capture).
 int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error, bool events = true) {
   const DevPlan &P = C.plan;
   int st;
+  if (P.hot_image && (st = launch_hot_select(ctx, C, dplan, error))) return st;
   switch (P.lds_mode) {
     case LDS_DIRECT: st = launch_scan_cols<LDS_DIRECT>(ctx, C, dplan, error); break;
     case LDS_HASHED: st = launch_scan_cols<LDS_HASHED>(ctx, C, dplan, error); break;
@@ -999,6 +1140,7 @@ int launch_partitioned(dfr_context *ctx, const Compiled &C, const DevPlan *dplan
       hipLaunchKernelGGL(k_part_aggregate, dim3(P.n_parts), dim3(pb), C.c_lds_bytes, ctx->stream, dplan);
     }
     ctx->obs.launches += 1;
+    if (P.hot_image && (st = launch_hot_reduce(ctx, C, dplan, error))) return st;
   }
   HIP_TRY(hipGetLastError());
   return DFR_OK;
@@ -1201,6 +1343,29 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
           (st = S.get(S_ * (uint64_t)P.seg_waves, &P.cnt_raw, error)) || (st = S.get(S_, &P.cnt_par, error)) ||
           (st = S.get(NP + 1, &P.base_raw, error)) || (st = S.get(NP + 1, &P.base_par, error)))
         return st;
+      P.hot_image = P.hot_states = nullptr;
+      // a sole integer or dictionary key with a hand-off list, else a sample
+      const dfr_column *kc = P.n_keys == 1 && !P.keys[0].is_float ? C.cols[0] : nullptr;
+      const bool col_list = kc && kc->hot;
+      if (hot_wanted(ctx, C, dense, col_list)) {
+        const uint64_t E = (uint64_t)P.lds_entries;
+        C.hot_sampled = !col_list;
+        if (col_list) {
+          C.hot_list = kc->hot;
+          C.hot_hist = (uint32_t *)(kc->hot + kHotList + 1);
+          C.hot_delta = kc->hot_min - P.keys[0].minv;
+          C.hot_limit = P.keys[0].radix - (uint64_t)P.keys[0].nullable;
+        } else {
+          C.hot_delta = 0;
+          C.hot_limit = P.domain;
+          if ((st = S.get(P.domain + 2, &C.hot_counts, error)) || (st = S.get(kHotBins + 2, &C.hot_hist, error)) ||
+              (st = S.get(kHotList + 1, &C.hot_list, error)))
+            return st;
+        }
+        if ((st = S.get(1, &C.hot_out, error)) || (st = S.get(E, &P.hot_image, error)) ||
+            (st = S.get(S_ * E * W, &P.hot_states, error)))
+          return st;
+      }
       if (dense) {
         // second level: the spill segments are dead after the first
         // scatter and take the sub-partition order
@@ -1277,9 +1442,12 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
     uint64_t *hs = (uint64_t *)ctx->staging;
     HIP_TRY(hipMemcpyAsync(hs, total, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(hs + 1, fault, 16, hipMemcpyDeviceToHost, ctx->stream));
+    hs[3] = 0;
+    if (P.hot_image && C.rows > 0) HIP_TRY(hipMemcpyAsync(hs + 3, C.hot_out, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->obs.waits++;
     ctx->obs.completed = ctx->obs.launches;
+    ctx->obs.hot_keys += (int64_t)(uint32_t)hs[3];
     const uint64_t G = hs[0];
     const uint32_t *hf = (const uint32_t *)(hs + 1);
     if (hf[3]) {

@@ -95,7 +95,8 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF, int 
     << "  L.part_raw = R.part_raw; L.part_par = R.part_par; L.state = R.state; L.keys_tab = R.keys_tab;\n"
     << "  L.mode = R.mode; L.cap = R.cap; L.cap_log2 = R.cap_log2; L.max_probe = R.max_probe; L.fault = R.fault;\n"
     << "  L.n_sub = R.n_sub; L.sub_chunks = R.sub_chunks; L.base2_raw = R.base2_raw; L.base2_par = R.base2_par;\n"
-    << "  L.part2_raw = R.part2_raw; L.part2_par = R.part2_par; L.gcount = R.gcount;\n";
+    << "  L.part2_raw = R.part2_raw; L.part2_par = R.part2_par; L.gcount = R.gcount;\n"
+    << "  L.hot_image = R.hot_image; L.hot_states = R.hot_states;\n";
   // a column without a validity bitmap has a null one in the shape, so its
   // validity code folds away
   for (int c = 0; c < P.n_cols; ++c)

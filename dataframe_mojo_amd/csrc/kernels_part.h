@@ -184,13 +184,35 @@ __device__ __forceinline__ uint32_t wave_append(bool take, uint32_t *counter) {
   return base + (uint32_t)rank;
 }
 
+// A row's packed group key (key columns first in `vals`).
+template <int NC, class PV>
+__device__ __forceinline__ uint64_t pack_key(const PV &P, const uint64_t (&vals)[NC], uint32_t vbits) {
+  if (P.n_keys == 1 && P.keys[0].is_float) return (vbits & 1u) ? canon_key_bits(vals[0]) : kNullFloatKey;
+  if (P.n_keys == 1) return (vbits & 1u) ? (vals[0] - (uint64_t)P.keys[0].minv) : (P.keys[0].radix - 1);
+  uint64_t packed = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxKeys && k < NC; ++k) {
+    if (k >= P.n_keys) continue;
+    const DevKey K = cget(P.keys[k]);
+    const uint64_t digit = ((vbits >> k) & 1u) ? (vals[k] - (uint64_t)K.minv) : (K.radix - 1);
+    packed += digit * K.stride;
+  }
+  return packed;
+}
+
+// The 4-slot bucket of a key in pass A's hashed LDS table of nb buckets.
+__host__ __device__ __forceinline__ uint32_t bucket_of(uint64_t key, uint32_t nb) {
+  const uint32_t k32 = (uint32_t)key ^ (uint32_t)(key >> 32);
+  return (uint32_t)(((uint64_t)(k32 * 0x9E3779B1u) * nb) >> 32);
+}
+
 // Scan LDS table: buckets of 4 key slots, read together; a key goes to the
 // first free slot of its bucket (first come), and a row whose bucket is
 // full spills. One LDS round trip per lookup, hit or miss.
 template <int LDSM, class PV>
 __device__ __forceinline__ int bucket_find(const PV &P, uint64_t *lkeys, uint64_t key) {
   if (LDSM == LDS_DIRECT) return (int)key;
-  const int b = (int)(((uint64_t)(key32(key) * 0x9E3779B1u) * (uint32_t)P.lds_buckets) >> 32);
+  const int b = (int)bucket_of(key, (uint32_t)P.lds_buckets);
   uint64_t *kb = lkeys + 4 * b;
   uint64_t k[4];
 #pragma unroll
@@ -217,19 +239,7 @@ __device__ __forceinline__ void spill_row(const PV &P, uint64_t *lkeys, uint64_t
   uint64_t packed = 0;
   int e = -1;
   if (kept) {
-    if (P.n_keys == 1 && P.keys[0].is_float) {
-      packed = (vbits & 1u) ? canon_key_bits(vals[0]) : kNullFloatKey;
-    } else if (P.n_keys == 1) {
-      packed = (vbits & 1u) ? (vals[0] - (uint64_t)P.keys[0].minv) : (P.keys[0].radix - 1);
-    } else {
-#pragma unroll
-      for (int k = 0; k < kMaxKeys && k < NC; ++k) {
-        if (k >= P.n_keys) continue;
-        const DevKey K = cget(P.keys[k]);
-        const uint64_t digit = ((vbits >> k) & 1u) ? (vals[k] - (uint64_t)K.minv) : (K.radix - 1);
-        packed += digit * K.stride;
-      }
-    }
+    packed = pack_key<NC>(P, vals, vbits);
     if (LDSM != LDS_NONE) e = bucket_find<LDSM>(P, lkeys, packed);
     if (e >= 0)
       fold_full<NC>(P, lst + (int64_t)e * P.lds_stride, row, [&](int c) { return vals[c]; },
@@ -276,9 +286,11 @@ __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
   uint32_t *hraw = (uint32_t *)(lst + (LDSM == LDS_NONE ? 0 : (size_t)E * S));
   uint32_t *hpar = hraw;
   uint32_t *cnt = hraw + H * NP;  // [0] raw records, [1] partial records
+  // (hashed table: the seeded hot keys, when there are, at their entries)
+  const DFR_GLOBAL uint64_t *image = LDSM == LDS_HASHED && P.hot_image ? gp(P.hot_image) : nullptr;
   if (LDSM != LDS_NONE) {
     for (int i = threadIdx.x; i < E; i += blockDim.x) {
-      if (LDSM == LDS_HASHED) lkeys[i] = kEmpty;
+      if (LDSM == LDS_HASHED) lkeys[i] = image ? image[i] : kEmpty;
       if constexpr (RegPlan<PV>::value) {
 #pragma unroll
         for (int w = 0; w < kMaxWords; ++w)
@@ -346,7 +358,13 @@ __device__ __forceinline__ void scan_spill_body(const PV &P, int64_t chunk) {
     const int Eround = (E + 63) & ~63;
     for (int e = threadIdx.x; e < Eround; e += blockDim.x) {
       const uint64_t *st = lst + (int64_t)e * S;
-      const bool occ = e < E && st[0] != 0;
+      // a seeded entry's state goes to the dense hot states, whatever it holds
+      const bool seeded = image && e < E && image[e] != kEmpty;
+      if (seeded) {
+        DFR_GLOBAL uint64_t *hs = gpw(P.hot_states) + ((uint64_t)s * E + e) * W;
+        for (int w = 0; w < W; ++w) hs[w] = st[w];
+      }
+      const bool occ = e < E && !seeded && st[0] != 0;
       const uint64_t key = (LDSM == LDS_DIRECT) ? (uint64_t)e : (e < E ? lkeys[e] : 0);
       const uint32_t at = wave_append(occ, &cnt[1]);
       if (occ) {
@@ -373,6 +391,357 @@ __global__ void __launch_bounds__(1024) k_scan_spill(const DevPlan *__restrict__
 }
 
 #if !DFR_JIT
+// ---------------------------------------------------------- hot keys
+//
+// Heavy-hitter keys, found from a sample of the kept rows and seeded into
+// every pass A workgroup's LDS table at the same entries. On skewed keys
+// (Zipf) they carry most rows: seeded, those rows never spill, and their
+// states leave pass A as dense per-workgroup arrays merged by one coalesced
+// reduction instead of as partial-state records that pass C would fold into
+// the same LDS entry from every workgroup. Any seeding gives the same
+// result: the seeded keys only decide which rows take the LDS table.
+
+constexpr int kHotSampleThreads = 256;
+constexpr int kHotCountEntries = 2048;  // per-block LDS pre-count table
+
+// Block b counts the packed keys (counts[key], key < domain) of the kept
+// rows in rows [b * stride, b * stride + span), through a small LDS table so
+// that the hottest keys take one device atomic per block.
+template <int NC>
+__global__ void __launch_bounds__(kHotSampleThreads) k_hot_count(const DevPlan *__restrict__ plan, uint32_t *counts,
+                                                                 uint64_t domain, int64_t stride, int64_t span) {
+  __shared__ uint64_t tkeys[kHotCountEntries];
+  __shared__ uint32_t tcnt[kHotCountEntries];
+  PlanRef P = plan_ref(plan);
+  for (int i = threadIdx.x; i < kHotCountEntries; i += blockDim.x) {
+    tkeys[i] = kEmpty;
+    tcnt[i] = 0;
+  }
+  __syncthreads();
+  auto sink = [&](bool kept, int64_t, const uint64_t(&vals)[NC], uint32_t vbits) {
+    if (!kept) return;
+    const uint64_t key = pack_key<NC>(P, vals, vbits);
+    if (key >= domain) return;  // (every packed key of the column ranges is below it)
+    int h = (int)((key * kGolden) >> (64 - 11));
+    for (int p = 0; p < 4; ++p, h = (h + 1) & (kHotCountEntries - 1)) {
+      const uint64_t k = tkeys[h];
+      if (k == key) {
+        atomicAdd(&tcnt[h], 1u);
+        return;
+      }
+      if (k == kEmpty) {
+        const unsigned long long old =
+            atomicCAS((unsigned long long *)&tkeys[h], (unsigned long long)kEmpty, (unsigned long long)key);
+        if (old == kEmpty || old == key) {
+          atomicAdd(&tcnt[h], 1u);
+          return;
+        }
+      }
+    }
+    atomicAdd(&counts[key], 1u);
+  };
+  const int64_t b0 = (int64_t)blockIdx.x * stride;
+  const int64_t b1 = min(b0 + span, P.rows);
+  const int64_t even = b0 + (max(b1 - b0, (int64_t)0) & ~(int64_t)1);
+  for (int64_t base = b0; base + 1 < even; base += 2 * (int64_t)blockDim.x) {
+    ScanTile<NC, 1> t;
+    load_tile<NC, 1>(P, t, base, even);
+    fold_tile<NC, 1, true>(P, t, base, sink);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHotCountEntries; i += blockDim.x)
+    if (tkeys[i] != kEmpty && tcnt[i]) atomicAdd(&counts[tkeys[i]], tcnt[i]);
+}
+
+// The column hand-off's sample: block b counts the valid values v of rows
+// [b * stride, b * stride + span) of one integer or dictionary column as
+// counts[v - minv] (v - minv < domain), through the same LDS pre-count.
+__global__ void __launch_bounds__(kHotSampleThreads) k_col_hot_count(DevCol C, int64_t n, int64_t minv,
+                                                                     uint64_t domain, uint32_t *counts,
+                                                                     int64_t stride, int64_t span) {
+  __shared__ uint64_t tkeys[kHotCountEntries];
+  __shared__ uint32_t tcnt[kHotCountEntries];
+  for (int i = threadIdx.x; i < kHotCountEntries; i += blockDim.x) {
+    tkeys[i] = kEmpty;
+    tcnt[i] = 0;
+  }
+  __syncthreads();
+  const int64_t b0 = (int64_t)blockIdx.x * stride;
+  const int64_t b1 = min(b0 + span, n);
+  for (int64_t r = b0 + threadIdx.x; r < b1; r += blockDim.x) {
+    if (C.valid && !((C.valid[(C.vbit + r) >> 6] >> ((C.vbit + r) & 63)) & 1ull)) continue;
+    const int64_t v = C.dtype == T_INT64 ? ((const int64_t *)C.values)[r]
+                      : C.dtype == T_INT32 ? (int64_t)((const int32_t *)C.values)[r]
+                                           : (int64_t)((const uint32_t *)C.values)[r];
+    const uint64_t key = (uint64_t)v - (uint64_t)minv;
+    if (key >= domain) continue;
+    int h = (int)((key * kGolden) >> (64 - 11));
+    bool done = false;
+    for (int p = 0; p < 4 && !done; ++p, h = (h + 1) & (kHotCountEntries - 1)) {
+      const uint64_t k = tkeys[h];
+      if (k == key) {
+        atomicAdd(&tcnt[h], 1u);
+        done = true;
+      } else if (k == kEmpty) {
+        const unsigned long long old =
+            atomicCAS((unsigned long long *)&tkeys[h], (unsigned long long)kEmpty, (unsigned long long)key);
+        if (old == kEmpty || old == key) {
+          atomicAdd(&tcnt[h], 1u);
+          done = true;
+        }
+      }
+    }
+    if (!done) atomicAdd(&counts[key], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHotCountEntries; i += blockDim.x)
+    if (tkeys[i] != kEmpty && tcnt[i]) atomicAdd(&counts[tkeys[i]], tcnt[i]);
+}
+
+constexpr int kHotBins = 4096;   // sample counts binned 0..4095 (the last bin: 4095 and more)
+constexpr int kHotList = 4096;   // candidates, at most
+constexpr int kHotSelectThreads = 1024;
+
+// hist[min(c, kHotBins - 1)] += 1 for every sampled count c >= 2 of the key
+// domain (a key sampled once is no candidate), and hist[kHotBins] (two
+// words) += every c: the kept rows sampled. Small counts, which most keys
+// share, are added once per wave and value (a ballot per distinct value),
+// not once per lane on one LDS word.
+__global__ void k_hot_hist(const uint32_t *counts, uint64_t domain, uint32_t *hist) {
+  __shared__ uint32_t h[kHotBins];
+  __shared__ unsigned long long total;
+  for (int i = threadIdx.x; i < kHotBins; i += blockDim.x) h[i] = 0;
+  if (threadIdx.x == 0) total = 0;
+  __syncthreads();
+  unsigned long long mine = 0;
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // every lane of a wave runs the same number of rounds (ballots below)
+  const uint64_t rounds = (domain + step - 1) / step;
+  for (uint64_t r = 0; r < rounds; ++r) {
+    const uint64_t k = first + r * step;
+    const uint32_t c = k < domain ? counts[k] : 0;
+    mine += c;
+    bool small = c >= 2 && c < 64;
+    if (c >= 64) atomicAdd(&h[min(c, (uint32_t)kHotBins - 1)], 1u);
+    uint64_t pend = __ballot(small);
+    while (pend) {
+      const int leader = __ffsll((unsigned long long)pend) - 1;
+      const uint32_t v = (uint32_t)__shfl((int)c, leader);
+      const uint64_t m = __ballot(small && c == v);
+      if (__lane_id() == leader) atomicAdd(&h[v], (uint32_t)__popcll(m));
+      pend &= ~m;
+    }
+  }
+  atomicAdd(&total, mine);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHotBins; i += blockDim.x)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+  if (threadIdx.x == 0) atomicAdd((unsigned long long *)(hist + kHotBins), total);
+}
+
+// The smallest count t >= 2 whose keys (count >= t) number `want` at most,
+// from hist; every thread of the (kHotSelectThreads) workgroup returns it.
+__device__ uint32_t hot_threshold(const uint32_t *hist, uint32_t want, uint32_t *part) {
+  const int per = kHotBins / kHotSelectThreads;
+  const int t = threadIdx.x;
+  uint32_t own = 0;
+  for (int j = 0; j < per; ++j) own += hist[t * per + j];
+  part[t] = own;
+  __syncthreads();
+  // suffix sums: part[t] = keys in bins >= t * per
+  for (int d = 1; d < kHotSelectThreads; d <<= 1) {
+    const uint32_t add = t + d < kHotSelectThreads ? part[t + d] : 0;
+    __syncthreads();
+    part[t] += add;
+    __syncthreads();
+  }
+  __shared__ uint32_t best;
+  if (t == 0) best = kHotBins;
+  __syncthreads();
+  uint32_t above = t + 1 < kHotSelectThreads ? part[t + 1] : 0;
+  for (int j = per - 1; j >= 0; --j) {
+    above += hist[t * per + j];
+    if (above <= want) atomicMin(&best, (uint32_t)(t * per + j));
+  }
+  __syncthreads();
+  return max(best, 2u);
+}
+
+// Keys whose sampled count reaches the threshold, appended to list as
+// (count << 32 | key) (keys below 2^32); the 32-bit counter at
+// list[kHotList] counts the appended (one atomic per wave).
+__global__ void __launch_bounds__(kHotSelectThreads) k_hot_collect(const uint32_t *counts, uint64_t domain,
+                                                                   const uint32_t *hist, uint32_t want,
+                                                                   uint64_t *list) {
+  __shared__ uint32_t part[kHotSelectThreads];
+  const uint32_t thresh = hot_threshold(hist, want, part);
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < domain; k += step) {
+    const uint32_t c = counts[k];
+    const bool take = c >= thresh;
+    const uint32_t at = wave_append(take, (uint32_t *)&list[kHotList]);
+    if (take && at < (uint32_t)kHotList) list[at] = ((uint64_t)c << 32) | (uint32_t)k;
+  }
+}
+
+// One workgroup: the candidates placed into an image of pass A's bucket
+// table (`entries` keys, entries / 4 buckets of 4), in bands of descending
+// sampled count (band b: counts in [2^b, 2^(b+1))) so that a full bucket
+// keeps the hotter keys; a candidate takes the next free slot of its bucket,
+// or none. A candidate's packed key is its listed key + delta, and only
+// packed keys in [0, limit) are placed (a column's list, whose keys are
+// offsets from the column's minimum, for a query whose key range is a
+// window of it). The image stays empty when the placed keys carry less than
+// min_share of the `total` sampled rows (no heavy hitters: seeding would not
+// pay). hot_out[0] = keys placed.
+__global__ void __launch_bounds__(kHotSelectThreads) k_hot_select(const uint64_t *list,
+                                                                  const unsigned long long *total_rows,
+                                                                  int64_t delta, uint64_t limit, uint64_t *image,
+                                                                  int entries, float min_share, uint32_t *hot_out) {
+  constexpr int PER = kHotList / kHotSelectThreads;
+  __shared__ uint32_t fill[4096];  // per bucket (entries <= 16384)
+  __shared__ unsigned long long covered;
+  __shared__ uint32_t placed, top;
+  const uint32_t n = min(*(const uint32_t *)&list[kHotList], (uint32_t)kHotList);
+  const int nb = entries / 4;
+  uint64_t mine[PER];
+  int band[PER];
+  for (int j = 0; j < PER; ++j) {
+    const uint32_t i = threadIdx.x + j * kHotSelectThreads;
+    mine[j] = i < n ? list[i] : 0;
+    const uint64_t key = (uint64_t)((int64_t)(uint32_t)mine[j] + delta);
+    band[j] = (i < n && key < limit) ? 31 - __clz((uint32_t)(mine[j] >> 32)) : -1;
+  }
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) fill[i] = 0;
+  for (int i = threadIdx.x; i < entries; i += blockDim.x) image[i] = kEmpty;
+  if (threadIdx.x == 0) {
+    covered = 0;
+    placed = 0;
+    top = 0;
+  }
+  __syncthreads();
+  for (int j = 0; j < PER; ++j)
+    if (band[j] >= 0) atomicMax(&top, (uint32_t)band[j]);
+  __syncthreads();
+  for (int b = (int)top; b >= 1; --b) {
+    for (int j = 0; j < PER; ++j) {
+      if (band[j] != b) continue;
+      const uint64_t key = (uint64_t)((int64_t)(uint32_t)mine[j] + delta);
+      const uint32_t bk = bucket_of(key, (uint32_t)nb);
+      const uint32_t slot = atomicAdd(&fill[bk], 1u);
+      if (slot < 4) {
+        image[4 * bk + slot] = key;
+        atomicAdd(&covered, (unsigned long long)(mine[j] >> 32));
+        atomicAdd(&placed, 1u);
+      }
+    }
+    __syncthreads();
+  }
+  const unsigned long long total = *total_rows;
+  const bool keep = covered > 0 && (double)covered >= (double)min_share * (double)total;
+  if (!keep)
+    for (int i = threadIdx.x; i < entries; i += blockDim.x) image[i] = kEmpty;
+  if (threadIdx.x == 0) hot_out[0] = keep ? placed : 0;
+}
+
+// One word of a merge, by its operation (an n-word add: all n words).
+__device__ __forceinline__ void hot_merge_word(int8_t op, int n, uint64_t (&acc)[4], const uint64_t (&x)[4]) {
+  switch (op) {
+    case W_ADD: acc[0] += x[0]; break;
+    case W_MIN_S: acc[0] = (int64_t)x[0] < (int64_t)acc[0] ? x[0] : acc[0]; break;
+    case W_MAX_S: acc[0] = (int64_t)x[0] > (int64_t)acc[0] ? x[0] : acc[0]; break;
+    case W_MIN_U: acc[0] = x[0] < acc[0] ? x[0] : acc[0]; break;
+    case W_MAX_U: acc[0] = x[0] > acc[0] ? x[0] : acc[0]; break;
+    case W_OR: acc[0] |= x[0]; break;
+    default: {  // an n-word two's-complement add
+      uint64_t carry = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i >= n) break;
+        const uint64_t t = acc[i] + x[i];
+        const uint64_t c1 = t < x[i] ? 1ull : 0ull;
+        const uint64_t t2 = t + carry;
+        const uint64_t c2 = t2 < t ? 1ull : 0ull;
+        acc[i] = t2;
+        carry = c1 | c2;
+      }
+    }
+  }
+}
+
+// Words of the merge a state word leads: 2 to 4 for a multi-word add, else 1.
+__device__ __forceinline__ int hot_words(int8_t op) {
+  return op == W_ADD128 ? 2 : op == W_ADD192 ? 3 : op == W_ADD256 ? 4 : 1;
+}
+
+// The seeded entries' states merged over pass A's workgroups in two steps.
+// Step 0, thread (group g, entry e, word w): workgroups [g * n / groups,
+// (g + 1) * n / groups) of hot_states ([workgroup][entry][word]) into the
+// first of them. Step 1, thread (e, w): the groups' results into the group
+// table at the key's slot (plain stores: seeded keys never reach pass C),
+// when the key has kept rows. A multi-word add is merged by its low word's
+// thread; the others return. Loads are issued 8 at a time.
+__global__ void k_hot_reduce(const DevPlan *__restrict__ plan, int groups, int step) {
+  PlanRef P = plan_ref(plan);
+  const int W = P.words, E = P.lds_entries, NS = P.n_segs;
+  const uint64_t ew = (uint64_t)E * W;
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ew * (step == 0 ? (uint64_t)groups : 1)) return;
+  const int g = (int)(idx / ew);
+  const int e = (int)((idx % ew) / W), w = (int)(idx % W);
+  const uint64_t key = P.hot_image[e];
+  const int8_t op = P.wop[w];
+  if (key == kEmpty || op == W_HI) return;
+  const int n = hot_words(op);
+  uint64_t acc[4];
+  for (int i = 0; i < 4; ++i) acc[i] = i < n ? P.ident[w + i] : 0;
+  const DFR_GLOBAL uint64_t *src = gp(P.hot_states) + (uint64_t)e * W + w;
+  // the workgroups merged: a range (step 0) or the groups' first ones (step 1)
+  int s0 = 0, cnt = groups;
+  if (step == 0) {
+    s0 = (int)((int64_t)g * NS / groups);
+    cnt = (int)((int64_t)(g + 1) * NS / groups) - s0;
+    if (cnt <= 0) return;
+  }
+  auto seg_of = [&](int i) -> int { return step == 0 ? s0 + i : (int)((int64_t)i * NS / groups); };
+  auto live = [&](int i) -> bool {
+    return step == 0 || (int)((int64_t)(i + 1) * NS / groups) > (int)((int64_t)i * NS / groups);
+  };
+  uint64_t rows = 0;  // (step 1) the key's kept rows: word 0 of each group
+  constexpr int B = 8;
+  for (int i0 = 0; i0 < cnt; i0 += B) {
+    uint64_t x[B][4];
+    uint64_t c0[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const int i = i0 + j;
+      const bool ok = i < cnt && live(i);
+      const DFR_GLOBAL uint64_t *v = src + (uint64_t)(ok ? seg_of(i) : 0) * ew;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) x[j][q] = (ok && q < n) ? v[q] : 0;
+      c0[j] = (ok && step == 1) ? v[-w] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const int i = i0 + j;
+      if (!(i < cnt && live(i))) continue;
+      hot_merge_word(op, n, acc, x[j]);
+      rows += c0[j];
+    }
+  }
+  if (step == 0) {
+    DFR_GLOBAL uint64_t *d = gpw(P.hot_states) + (uint64_t)s0 * ew + (uint64_t)e * W + w;
+    for (int i = 0; i < n; ++i) d[i] = acc[i];
+    return;
+  }
+  if (rows == 0) return;  // no kept row of the key (a group only exists with one)
+  const uint64_t slot = global_slot(P, key);
+  if (slot == kEmpty) return;
+  uint64_t *d = P.state + slot * (uint64_t)W + w;
+  for (int i = 0; i < n; ++i) d[i] = acc[i];
+}
+
 // Pass B, step 1: per partition p, exclusive scan over segments of the
 // histograms (in place), partition totals into base_*[p].
 __global__ void k_part_offsets(const DevPlan *__restrict__ plan) {

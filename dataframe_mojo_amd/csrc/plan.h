@@ -233,6 +233,15 @@ struct DevPlan {
   uint64_t *part2_raw;
   uint64_t *part2_par;
   uint64_t *gcount;
+  // seeded hot keys (one partition level, hashed pass A table): every pass A
+  // workgroup starts its LDS table from hot_image (lds_entries keys, empty
+  // where nothing is seeded), so the seeded keys sit at the same entries in
+  // every workgroup and never spill; at the end a workgroup writes its
+  // seeded entries' states densely to hot_states[workgroup][entry][words],
+  // and k_hot_reduce merges them across workgroups into the group table.
+  // Null: no seeding (first-come entries only).
+  uint64_t *hot_image;
+  uint64_t *hot_states;
   // outputs of finalize
   void *out_keys[kMaxKeys];
   uint64_t *out_key_valid[kMaxKeys];

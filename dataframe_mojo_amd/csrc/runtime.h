@@ -61,6 +61,11 @@ struct Tuning {
   int scatter_slices = 1;      // pass A workgroup's waves split into this many scatter workgroups
   int scatter_chunk = 16384;   // records a scatter workgroup sorts at once (8192 or 16384)
   int scatter_stage = 1;       // 1: the scatters stage whole records of 2 to 4 words in LDS
+  int hot_keys = -1;           // pass A tables seeded with sampled heavy-hitter keys: -1 from hot_rows rows, 0 never, 1 always
+  int64_t hot_rows = 1 << 24;  // rows from which hot_keys = -1 seeds
+  int64_t hot_domain = 1 << 24;  // largest packed key domain that is sampled (a count per key)
+  int hot_sample = 1 << 19;    // rows sampled
+  int hot_share = 30;          // percent of the sampled kept rows the seeded keys must carry
 };
 
 }  // namespace dfr
@@ -120,6 +125,12 @@ struct dfr_column {
   void *owned_valid = nullptr;
   dfr::ColStats stats = {};
   int32_t stats_window = 1;  // 0: the statistics are a parent's (a slice), bounds of this window's
+  // heavy hitters of a sample of the values (integer and dictionary columns
+  // of many rows): candidate words (count << 32 | value - hot_min), their
+  // counter and the sample's histogram with its row total; device memory of
+  // the column's context, or null
+  uint64_t *hot = nullptr;
+  int64_t hot_min = 0;
   dfr::DevCol dev() const {
     dfr::DevCol c;
     c.values = values;

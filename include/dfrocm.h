@@ -105,6 +105,7 @@ typedef struct dfr_observation {
     int64_t queries;           /* group-by queries executed */
     int64_t jit_compiles;      /* pass A kernels compiled at run time for a query shape */
     int64_t jit_launches;      /* pass A launches of such kernels */
+    int64_t hot_keys;          /* heavy-hitter keys seeded into pass A tables, summed over queries */
 } dfr_observation;
 int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
 
@@ -157,7 +158,15 @@ void dfr_context_trim(dfr_context *ctx);
  * "scatter_chunk" (records a scatter workgroup sorts at once: 8192 or
  * 16384), "scatter_stage" (1, the default: the scatters load records of 2
  * to 4 words whole, sort them through LDS and stream them out; 0: they sort
- * indices and gather the records again). */
+ * indices and gather the records again), "hot_keys" (the partitioned pass A
+ * seeds every workgroup's hashed table with the heavy-hitter keys of a
+ * sample of the kept rows, whose states then merge in one dense reduction
+ * instead of through the partitions: -1, the default, from "hot_rows" rows
+ * (default 2^24) when the packed key domain is at most "hot_domain"
+ * (default 2^24) and the query has one partition level; 1 whenever that
+ * domain allows; 0 never), "hot_sample" (rows sampled, default 2^19),
+ * "hot_share" (percent of the sampled kept rows the seeded keys must carry,
+ * else nothing is seeded; default 30). Seeding changes no result. */
 int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
 
 /* ---- device columns (the column hand-off) ------------------------------ */

@@ -182,7 +182,8 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
                  "scan_prefetch": 1, "lds_odd_stride": 0, "part_odd_stride": 1, "part_block": 1024, "scan_block": 1024,
                  "scan_wg_per_cu": 1, "scan_lds_bytes": 159744, "two_level_parts": 512,
                  "scatter_slices": 1, "scatter_chunk": 16384, "scatter_stage": 1, "jit_wait": 1,
-                 "sub_wg_per_cu": -1, "sub_fill": 50}
+                 "sub_wg_per_cu": -1, "sub_fill": 50, "hot_keys": -1, "hot_rows": 1 << 24, "hot_domain": 1 << 24,
+                 "hot_sample": 1 << 19, "hot_share": 30}
 
 
 @pytest.mark.parametrize("setting", [
@@ -216,6 +217,8 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
     {"scatter_slices": 4, "scan_block": 320, "two_level": 1},
     {"scatter_stage": 0}, {"scatter_stage": 0, "two_level": 1, "scatter_slices": 4},
     {"scatter_stage": 1, "jit": 2, "scan_block": 128, "two_level": 1},
+    # pass A tables seeded with sampled heavy-hitter keys
+    {"hot_keys": 1, "lds_mode": 2}, {"hot_keys": 1, "lds_mode": 2, "jit": 2, "hot_share": 0},
 ])
 def test_table_modes_give_identical_results(setting, gpu_ctx):
     df = random_frame(30000, 5, key_card=400)
@@ -533,3 +536,114 @@ def test_distinct_estimate_bounds_the_groups(kind, card, gpu_ctx):
     est = col_.distinct()
     # (HyperLogLog with 4096 registers: 1.6% standard error; the estimate adds three of them and 16)
     assert true <= est <= 1.12 * true + 24, (kind, card, true, est)
+
+
+def _skewed_series(n, seed, domain=200_000):
+    """Zipf(1.1)-skewed Int64 keys over [0, domain) with nulls, and value
+    columns whose Float64 magnitudes span 1e-8 to 1e20 (wide fixed-point
+    accumulators)."""
+    rng = np.random.default_rng(seed)
+    ranks = np.arange(1, domain + 1, dtype=np.float64)
+    p = ranks ** -1.1
+    keys = rng.choice(domain, size=n, p=p / p.sum())
+    w = rng.normal(size=n) * 10.0 ** rng.uniform(-8, 20, n)
+    return [Series("k", Column.int64(keys, rng.random(n) > 0.01)),
+            Series("a", Column.int64(rng.integers(-1000, 1000, n))),
+            Series("x", Column.float64(rng.normal(size=n), rng.random(n) > 0.05)),
+            Series("w", Column.float64(w)),
+            Series("y", Column.int32(rng.integers(-1000, 1000, n)))]
+
+
+HOT_AGGS = [col("y").sum().alias("y_sum"), col("y").count().alias("y_cnt"), col("y").min().alias("y_min"),
+            col("y").max().alias("y_max"), col("x").sum().alias("x_sum"), col("x").mean().alias("x_mean"),
+            col("x").var().alias("x_var"), col("x").min().alias("x_min"), col("x").max().alias("x_max"),
+            col("w").sum().alias("w_sum"), col("w").std().alias("w_std"), len_().alias("n")]
+
+
+@pytest.mark.parametrize("setting", [{}, {"jit": 2}, {"scan_block": 256, "scan_wg_per_cu": 3},
+                                     {"hot_sample": 4096, "hot_share": 0}, {"lds_odd_stride": 1, "jit": 2},
+                                     {"scan_lds_bytes": 30000},
+                                     # the key column's hand-off list instead of a per-query sample
+                                     {"hot_rows": 1000}, {"hot_rows": 1000, "jit": 2, "hot_share": 0}])
+@pytest.mark.parametrize("mo", [False, True])
+def test_hot_key_seeding(setting, mo, gpu_ctx):
+    """Pass A tables seeded with the sampled heavy hitters of skewed keys
+    (their states merged densely, apart from the partitions) give the
+    unseeded results bit for bit, and the oracle's."""
+    from test_gpu_batch import frames_equal_bitwise
+    df = DataFrame(_skewed_series(1_500_000, 21))
+    lf = df.filter(col("a") > -600).group_by("k", maintain_order=mo).agg(HOT_AGGS)
+    try:
+        gpu_ctx.tune("hot_keys", 0)
+        base = lf.collect(engine="accel")
+        for k, v in setting.items():
+            gpu_ctx.tune(k, v)
+        gpu_ctx.tune("hot_keys", 1)
+        before = gpu_ctx.observe()["hot_keys"]
+        got = lf.collect(engine="accel")
+        seeded = gpu_ctx.observe()["hot_keys"] - before
+    finally:
+        for k, v in TUNE_DEFAULTS.items():
+            gpu_ctx.tune(k, v)
+    assert seeded > 0  # the skew left heavy hitters to seed
+    frames_equal_bitwise(base, got, 1)
+    if mo or not setting:
+        exp, _ = oracle.collect(lf)
+        assert_same(got, exp, mo, ("x_min", "x_max"), 1)
+
+
+def test_hot_key_seeding_multi_key_and_filtered_hot_keys(gpu_ctx):
+    """Two keys (dictionary codes and Int64) packed into one domain, and a
+    filter that drops the hottest keys' rows: seeding follows the kept rows."""
+    from test_gpu_batch import frames_equal_bitwise
+    rng = np.random.default_rng(5)
+    n = 1_200_000
+    codes = rng.integers(0, 7, n).astype(np.int32)
+    df = DataFrame(_skewed_series(n, 22, domain=5000) +
+                   [Series("d", Column.dictionary(codes, [f"c{i}" for i in range(7)]))])
+    lf = df.filter((col("k") > 3) & (col("a") > -900)).group_by("d", "k").agg(HOT_AGGS)
+    try:
+        gpu_ctx.tune("hot_keys", 0)
+        base = lf.collect(engine="accel")
+        gpu_ctx.tune("hot_keys", 1)
+        before = gpu_ctx.observe()["hot_keys"]
+        got = lf.collect(engine="accel")
+        seeded = gpu_ctx.observe()["hot_keys"] - before
+    finally:
+        for k, v in TUNE_DEFAULTS.items():
+            gpu_ctx.tune(k, v)
+    assert seeded > 0
+    frames_equal_bitwise(base, got, 2)
+    exp, _ = oracle.collect(lf)
+    assert_same(got, exp, False, ("x_min", "x_max"), 2)
+
+
+def test_hot_key_seeding_dictionary_key_list(gpu_ctx):
+    """A dictionary key column's hand-off heavy-hitter list seeds pass A
+    (skewed codes over 3000 labels), with the unseeded results bit for bit."""
+    from test_gpu_batch import frames_equal_bitwise
+    rng = np.random.default_rng(9)
+    n, labels = 1_000_000, 3000
+    p = np.arange(1, labels + 1, dtype=np.float64) ** -1.2
+    codes = rng.choice(labels, size=n, p=p / p.sum()).astype(np.int32)
+    df = DataFrame([Series("d", Column.dictionary(codes, [f"s{i}" for i in range(labels)], rng.random(n) > 0.02)),
+                    Series("a", Column.int64(rng.integers(-1000, 1000, n))),
+                    Series("x", Column.float64(rng.normal(size=n), rng.random(n) > 0.05)),
+                    Series("y", Column.int32(rng.integers(-1000, 1000, n)))])
+    lf = df.filter(col("a") > 0).group_by("d").agg(HOT_AGGS[:9])
+    try:
+        gpu_ctx.tune("hot_keys", 0)
+        base = lf.collect(engine="accel")
+        gpu_ctx.tune("hot_keys", 1)
+        gpu_ctx.tune("hot_rows", 1000)
+        gpu_ctx.tune("lds_mode", 2)
+        before = gpu_ctx.observe()["hot_keys"]
+        got = lf.collect(engine="accel")
+        seeded = gpu_ctx.observe()["hot_keys"] - before
+    finally:
+        for k, v in TUNE_DEFAULTS.items():
+            gpu_ctx.tune(k, v)
+    assert seeded > 0
+    frames_equal_bitwise(base, got, 1)
+    exp, _ = oracle.collect(lf)
+    assert_same(got, exp, False, ("x_min", "x_max"), 1)
