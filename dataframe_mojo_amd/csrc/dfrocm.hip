@@ -352,6 +352,8 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.hot_sample = (int)value;
   else if (k == "hot_share" && value >= 0 && value <= 100)
     t.hot_share = (int)value;
+  else if (k == "rec_pack" && (value == 0 || value == 1))
+    t.rec_pack = (int)value;
   else if (k == "scatter_chunk" && (value == 8192 || value == 16384))
     t.scatter_chunk = (int)value;
   else if (k == "two_level_parts" && value >= 1 && value <= 4096 && (value & (value - 1)) == 0)

@@ -156,6 +156,7 @@ Tuning default_tuning(dfr_context *ctx) {
   t.scatter_chunk = env_int("DFR_SCATTER_CHUNK", 16384) == 8192 ? 8192 : 16384;
   t.scatter_stage = env_int("DFR_SCATTER_STAGE", 1) ? 1 : 0;
   t.hot_keys = std::max(-1, std::min(1, env_int("DFR_HOT_KEYS", -1)));
+  t.rec_pack = env_int("DFR_REC_PACK", 1) ? 1 : 0;
   t.batch_part_rows = env_int("DFR_BATCH_PART_ROWS", 1 << 22);
   t.scan_block = env_int("DFR_SCAN_BLOCK", 1024);
   t.scan_wg_per_cu = env_int("DFR_SCAN_WG_PER_CU", 1);
@@ -711,7 +712,35 @@ int compile_query(dfr_context *ctx, const dfr_query *q, Compiled &C, char **erro
   P.rec_meta = P.vmask_in_key ? -1 : rw++;
   P.rec_row = need_row ? rw++ : -1;
   P.rec_val0 = rw;
-  P.rw = rw + P.n_vals;
+  // narrow integer values ride in word 0's free bits (narrowest first)
+  for (int c = 0; c < kMaxCols; ++c) {
+    P.val_word[c] = -1;
+    P.pk_shift[c] = P.pk_bits[c] = 0;
+    P.pk_base[c] = 0;
+  }
+  int shift = P.key_bits + P.n_vals;
+  bool packed[kMaxCols] = {};
+  while (P.vmask_in_key && ctx->tune.rec_pack && shift < 64) {
+    int best = -1, best_bits = 65;
+    for (int c = 0; c < P.n_cols; ++c) {
+      const dfr_column *col = C.cols[c];
+      if (P.val_of[c] < 0 || packed[c] || col->dtype == DFR_FLOAT64) continue;
+      const int bits = col->stats.valid > 0 ? std::max(1, bitlen_u64((uint64_t)col->stats.imax - (uint64_t)col->stats.imin)) : 1;
+      if (bits < best_bits) {
+        best = c;
+        best_bits = bits;
+      }
+    }
+    if (best < 0 || shift + best_bits > 64) break;
+    packed[best] = true;
+    P.pk_shift[best] = (int8_t)shift;
+    P.pk_bits[best] = (int8_t)best_bits;
+    P.pk_base[best] = C.cols[best]->stats.valid > 0 ? C.cols[best]->stats.imin : 0;
+    shift += best_bits;
+  }
+  for (int c = 0; c < P.n_cols; ++c)
+    if (P.val_of[c] >= 0 && !packed[c]) P.val_word[c] = (int8_t)rw++;
+  P.rw = rw;
   return DFR_OK;
 }
 

@@ -136,7 +136,9 @@ std::string shape_source(const DevPlan &P, int LDSM, int NC, int U, int PF, int 
   for (int c = 0; c < P.n_cols; ++c) {
     const DevColAgg &A = P.cagg[c];
     o << "  L.cols[" << c << "].dtype = " << P.cols[c].dtype << "; L.cols[" << c << "].is_float = " << P.cols[c].is_float
-      << "; L.val_of[" << c << "] = " << (int)P.val_of[c] << ";\n"
+      << "; L.val_of[" << c << "] = " << (int)P.val_of[c] << "; L.val_word[" << c << "] = " << (int)P.val_word[c]
+      << "; L.pk_shift[" << c << "] = " << (int)P.pk_shift[c] << "; L.pk_bits[" << c << "] = " << (int)P.pk_bits[c]
+      << "; L.pk_base[" << c << "] = R.pk_base[" << c << "];\n"
       << "  L.cagg[" << c << "].mask = " << A.mask << "u; L.cagg[" << c << "].gbase = " << A.gbase << "; L.cagg[" << c
       << "].lbase = " << A.lbase << "; L.cagg[" << c << "].int_as_float = " << A.int_as_float << "; L.cagg[" << c
       << "].nfw = " << (int)A.nfw << "; L.cagg[" << c << "].nqw = " << (int)A.nqw << ";\n"

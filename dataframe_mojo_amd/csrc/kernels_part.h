@@ -252,14 +252,17 @@ __device__ __forceinline__ void spill_row(const PV &P, uint64_t *lkeys, uint64_t
   wcur += (uint32_t)__popcll(sm);
   if (spill) {
     DFR_GLOBAL uint64_t *rec = gpw(seg) + (uint64_t)at * P.rw;
-    uint64_t vmask = 0;
+    uint64_t vmask = 0, narrow = 0;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       if (c >= P.n_cols || P.val_of[c] < 0) continue;
       vmask |= (uint64_t)((vbits >> c) & 1u) << P.val_of[c];
-      rec[P.rec_val0 + P.val_of[c]] = vals[c];
+      if (P.val_word[c] >= 0)
+        rec[P.val_word[c]] = vals[c];
+      else
+        narrow |= ((vals[c] - (uint64_t)P.pk_base[c]) & ((1ull << P.pk_bits[c]) - 1)) << P.pk_shift[c];
     }
-    rec[0] = P.vmask_in_key ? (packed | (vmask << P.key_bits)) : packed;
+    rec[0] = P.vmask_in_key ? (packed | (vmask << P.key_bits) | narrow) : packed;
     if (P.rec_meta >= 0) rec[P.rec_meta] = vmask;
     if (P.rec_row >= 0) rec[P.rec_row] = (uint64_t)row;
     atomicAdd(&hraw[part_of(P, packed)], 1u);
@@ -1183,7 +1186,10 @@ __device__ __forceinline__ void fold_recs(const PV &P, uint64_t *lkeys, uint64_t
     const uint64_t key = rec[0] & keymask;
     const uint64_t vmask = P.vmask_in_key ? (rec[0] >> P.key_bits) : (P.rec_meta >= 0 ? pick(P.rec_meta) : ~0ull);
     const int64_t row = P.rec_row >= 0 ? (int64_t)pick(P.rec_row) : 0;
-    auto val = [&](int c) { return pick(P.rec_val0 + P.val_of[c]); };
+    auto val = [&](int c) {
+      return P.val_word[c] >= 0 ? pick(P.val_word[c])
+                                : ((rec[0] >> P.pk_shift[c]) & ((1ull << P.pk_bits[c]) - 1)) + (uint64_t)P.pk_base[c];
+    };
     auto ok = [&](int c) { return ((vmask >> P.val_of[c]) & 1ull) != 0; };
     const int e = part_find(P, lkeys, key, placed);
     if (e >= 0) {

@@ -182,6 +182,13 @@ struct DevPlan {
   int32_t rec_meta;      // word index of the validity mask word or -1
   int32_t rec_val0;      // first value word
   int8_t val_of[kMaxCols];  // record value index of each column or -1
+  // A value's record word, or -1 when it rides in word 0 above the key and
+  // the validity mask: (value - pk_base) in pk_bits bits from bit pk_shift
+  // (the column's range fits; a null's bits are masked and never read).
+  int8_t val_word[kMaxCols];
+  int8_t pk_shift[kMaxCols];
+  int8_t pk_bits[kMaxCols];
+  int64_t pk_base[kMaxCols];
   int8_t wop[kMaxWords];    // merge op of each state word
   int32_t part_log2;     // partitions = 1 << part_log2 (hash mode) ...
   int32_t n_parts;       // number of partitions

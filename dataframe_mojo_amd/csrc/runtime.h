@@ -66,6 +66,7 @@ struct Tuning {
   int64_t hot_domain = 1 << 24;  // largest packed key domain that is sampled (a count per key)
   int hot_sample = 1 << 19;    // rows sampled
   int hot_share = 30;          // percent of the sampled kept rows the seeded keys must carry
+  int rec_pack = 1;            // 1: narrow integer values of spilled records ride in the key word
 };
 
 }  // namespace dfr

@@ -166,7 +166,10 @@ void dfr_context_trim(dfr_context *ctx);
  * (default 2^24) and the query has one partition level; 1 whenever that
  * domain allows; 0 never), "hot_sample" (rows sampled, default 2^19),
  * "hot_share" (percent of the sampled kept rows the seeded keys must carry,
- * else nothing is seeded; default 30). Seeding changes no result. */
+ * else nothing is seeded; default 30). Seeding changes no result.
+ * "rec_pack" (1, the default: integer values of spilled row records whose
+ * column range fits ride in the record's key word, which shortens the
+ * records the partitions move; 0: every value has its own word). */
 int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
 
 /* ---- device columns (the column hand-off) ------------------------------ */

@@ -183,7 +183,7 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
                  "scan_wg_per_cu": 1, "scan_lds_bytes": 159744, "two_level_parts": 512,
                  "scatter_slices": 1, "scatter_chunk": 16384, "scatter_stage": 1, "jit_wait": 1,
                  "sub_wg_per_cu": -1, "sub_fill": 50, "hot_keys": -1, "hot_rows": 1 << 24, "hot_domain": 1 << 24,
-                 "hot_sample": 1 << 19, "hot_share": 30}
+                 "hot_sample": 1 << 19, "hot_share": 30, "rec_pack": 1}
 
 
 @pytest.mark.parametrize("setting", [
@@ -219,6 +219,8 @@ TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26
     {"scatter_stage": 1, "jit": 2, "scan_block": 128, "two_level": 1},
     # pass A tables seeded with sampled heavy-hitter keys
     {"hot_keys": 1, "lds_mode": 2}, {"hot_keys": 1, "lds_mode": 2, "jit": 2, "hot_share": 0},
+    # every record value in its own word
+    {"rec_pack": 0}, {"rec_pack": 0, "two_level": 1, "jit": 2},
 ])
 def test_table_modes_give_identical_results(setting, gpu_ctx):
     df = random_frame(30000, 5, key_card=400)
@@ -426,6 +428,7 @@ def test_staged_scatter_record_widths(width, two_level, gpu_ctx):
     try:
         gpu_ctx.tune("scan_block", 64)
         gpu_ctx.tune("two_level", two_level)
+        gpu_ctx.tune("rec_pack", 0)  # (records of `width` words)
         for stage in (0, 1):
             gpu_ctx.tune("scatter_stage", stage)
             got.append(lf.collect(engine="accel"))
@@ -647,3 +650,41 @@ def test_hot_key_seeding_dictionary_key_list(gpu_ctx):
     frames_equal_bitwise(base, got, 1)
     exp, _ = oracle.collect(lf)
     assert_same(got, exp, False, ("x_min", "x_max"), 1)
+
+
+@pytest.mark.parametrize("two_level", [-1, 1])
+@pytest.mark.parametrize("mo", [False, True])
+def test_narrow_values_ride_in_the_key_word(two_level, mo, gpu_ctx):
+    """Spilled records carry narrow integer values in the key word's free
+    bits (offsets from the column minimum, negative minimums included);
+    null slots holding values far outside the valid range are masked. The
+    results equal the one-word-per-value records bit for bit, and the
+    oracle's."""
+    from test_gpu_batch import frames_equal_bitwise
+    rng = np.random.default_rng(31)
+    n = 2_000_000
+    v_valid = rng.random(n) > 0.1
+    v = rng.integers(-700, 300, n)
+    v[~v_valid] = 1 << 40  # a null slot's value: never read
+    df = DataFrame([Series("k", Column.int64(rng.integers(-5000, 200000, n))),
+                    Series("v", Column.int64(v, v_valid)),
+                    Series("y", Column.int32(rng.integers(-(1 << 31), (1 << 31) - 1, n))),
+                    Series("z", Column.int64(rng.integers(-(1 << 62), 1 << 62, n))),
+                    Series("u", Column.int32(np.full(n, 7, dtype=np.int32))),
+                    Series("x", Column.float64(rng.normal(size=n), rng.random(n) > 0.05))])
+    aggs = [col("v").sum().alias("v_sum"), col("v").min().alias("v_min"), col("v").var().alias("v_var"),
+            col("y").max().alias("y_max"), col("y").min().alias("y_min"), col("z").max().alias("z_max"),
+            col("u").sum().alias("u_sum"), col("x").sum().alias("x_sum"), len_().alias("n")]
+    lf = df.filter(col("y") > -(1 << 30)).group_by("k", maintain_order=mo).agg(aggs)
+    got = []
+    try:
+        gpu_ctx.tune("two_level", two_level)
+        for pack in (0, 1):
+            gpu_ctx.tune("rec_pack", pack)
+            got.append(lf.collect(engine="accel"))
+    finally:
+        for k, val in TUNE_DEFAULTS.items():
+            gpu_ctx.tune(k, val)
+    frames_equal_bitwise(got[0], got[1], 1)
+    exp, _ = oracle.collect(lf)
+    assert_same(got[1], exp, mo, (), 1)
