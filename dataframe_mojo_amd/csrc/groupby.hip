@@ -1271,8 +1271,14 @@ int launch_aggregate(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, 
 
 }  // namespace
 
-// the plan, then read-backs (group count, fault flags) share the staging bytes
-static_assert(sizeof(DevPlan) + 64 + 32 <= kStagingBytes, "staging holds a plan and its read-backs");
+// Pinned staging bytes: a plan on its way to the device (hipMemcpyAsync reads
+// it when the stream reaches the copy, which can be after the host has
+// enqueued the whole query) in [0, kReadback); every read-back (group
+// count, fault flags, record totals) at kReadback or later. The host writes
+// the plan bytes only after a stream synchronisation.
+constexpr size_t kReadback = (sizeof(DevPlan) + 255) / 256 * 256;
+static_assert(kReadback + 64 + 32 <= kStagingBytes, "staging holds a plan and its read-backs");
+static uint64_t *readback(dfr_context *ctx) { return (uint64_t *)(ctx->staging + kReadback); }
 
 int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
   DevPlan &P = C.plan;
@@ -1427,7 +1433,7 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
       if (C.rows == 0) HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
       if (dense && C.rows > 0) {
         // the first level's record totals size the second level
-        uint64_t *tot = (uint64_t *)ctx->staging;
+        uint64_t *tot = readback(ctx);
         HIP_TRY(hipMemcpyAsync(tot, P.base_raw + P.n_parts, 8, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipMemcpyAsync(tot + 1, P.base_par + P.n_parts, 8, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1468,7 +1474,7 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
                          ctx->stream, dplan, occ, occ_prefix, order);
       ctx->obs.launches++;
     }
-    uint64_t *hs = (uint64_t *)ctx->staging;
+    uint64_t *hs = readback(ctx);
     HIP_TRY(hipMemcpyAsync(hs, total, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(hs + 1, fault, 16, hipMemcpyDeviceToHost, ctx->stream));
     hs[3] = 0;
@@ -1531,7 +1537,7 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
       }
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-    uint32_t *hf2 = (uint32_t *)(ctx->staging + sizeof(DevPlan) + 64);
+    uint32_t *hf2 = (uint32_t *)(readback(ctx) + 8);
     HIP_TRY(hipMemcpyAsync(hf2, fault, 16, hipMemcpyDeviceToHost, ctx->stream));
     hipError_t e = hipStreamSynchronize(ctx->stream);
     ctx->obs.waits++;

@@ -154,7 +154,7 @@ extern "C" int dfr_groupby_into(dfr_context *ctx, const dfr_query *query, const 
     hipLaunchKernelGGL(k_compact, dim3((unsigned)((P.cap + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
                        dplan, occ, occ_prefix, order);
     ctx->obs.launches++;
-    uint64_t *hs = (uint64_t *)ctx->staging;
+    uint64_t *hs = readback(ctx);
     HIP_TRY(hipMemcpyAsync(hs, occ_sums + occ_nb, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(hs + 1, fault, 16, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -204,7 +204,7 @@ extern "C" int dfr_groupby_into(dfr_context *ctx, const dfr_query *query, const 
       ctx->obs.launches++;
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-    uint32_t *hf2 = (uint32_t *)(ctx->staging + sizeof(DevPlan) + 64);
+    uint32_t *hf2 = (uint32_t *)(readback(ctx) + 8);
     HIP_TRY(hipMemcpyAsync(hf2, fault, 16, hipMemcpyDeviceToHost, ctx->stream));
     const hipError_t e = hipStreamSynchronize(ctx->stream);
     ctx->obs.waits++;
