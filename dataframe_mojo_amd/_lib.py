@@ -50,7 +50,10 @@ EXPORTS = [
     "dfr_context_destroy",
     "dfr_context_observe",
     "dfr_context_tune",
+    "dfr_context_tune_get",
     "dfr_context_trim",
+    "dfr_build_id",
+    "dfr_context_calibration",
     "dfr_column_upload",
     "dfr_column_wrap",
     "dfr_column_release",
@@ -198,7 +201,11 @@ def load(required: bool = True):
     lib.dfr_context_destroy.argtypes = [vp]
     lib.dfr_context_observe.argtypes = [vp, ctypes.POINTER(Observation)]
     lib.dfr_context_tune.argtypes = [vp, ctypes.c_char_p, i64, errp]
+    lib.dfr_context_tune_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(i64), errp]
     lib.dfr_context_trim.argtypes = [vp]
+    lib.dfr_build_id.argtypes = []
+    lib.dfr_build_id.restype = ctypes.c_char_p
+    lib.dfr_context_calibration.argtypes = [vp, ctypes.POINTER(vp), errp]
     lib.dfr_column_upload.argtypes = [vp, i32, vp, i64, i64, vp, i64, i64, ctypes.POINTER(vp), errp]
     lib.dfr_column_wrap.argtypes = [vp, i32, vp, i64, i64, vp, i64, i64, ctypes.POINTER(vp), errp]
     lib.dfr_column_release.argtypes = [vp]

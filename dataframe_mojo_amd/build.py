@@ -14,6 +14,7 @@ Run as ``python -m dataframe_mojo_amd.build`` or through
 from __future__ import annotations
 
 import concurrent.futures
+import hashlib
 import os
 import subprocess
 import sys
@@ -42,7 +43,7 @@ COMMON = [
 
 LIBS = {
     "libdfrocm.so": {
-        "sources": ["dfrocm.hip", "groupby.hip", "server.hip", "jit.hip"],
+        "sources": ["dfrocm.hip", "groupby.hip", "server.hip", "jit.hip", "calib.hip"],
         "extra": [f"-Wl,--version-script={os.path.join(CSRC, 'libdfrocm.map')}", "-lhiprtc", "-ldl"],
     },
     "libdfrgen.so": {"sources": ["datagen.hip"], "extra": []},
@@ -58,6 +59,21 @@ def _run(cmd: list[str], verbose: bool, what: str) -> None:
         raise RuntimeError(f"{what} failed (exit {proc.returncode})")
 
 
+def build_id() -> str:
+    """The identity of this source tree (kernels, runtime, headers): a
+    calibration profile measured with other code is not applied."""
+    h = hashlib.sha256()
+    root = os.path.dirname(HERE)
+    paths = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC))
+    paths.append(os.path.join(root, "include", "dfrocm.h"))
+    for path in paths:
+        if os.path.isfile(path):
+            h.update(os.path.basename(path).encode())
+            with open(path, "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def build(verbose: bool = False) -> list[str]:
     """Compiles every library; returns their paths. Raises on failure.
 
@@ -67,11 +83,14 @@ def build(verbose: bool = False) -> list[str]:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     jobs = []
+    bid = build_id()
     for name, spec in LIBS.items():
         for src in spec["sources"]:
             path = os.path.join(CSRC, src)
             obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
             flags = [f for f in COMMON if f != "-shared"]
+            if src == "calib.hip":
+                flags.append(f'-DDFR_BUILD_ID="{bid}"')
             jobs.append(([HIPCC, *flags, "-c", "-o", obj, path], f"compiling {src}"))
     workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
     with concurrent.futures.ThreadPoolExecutor(workers) as pool:

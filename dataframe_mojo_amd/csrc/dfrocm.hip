@@ -231,6 +231,15 @@ extern "C" int dfr_context_create(int32_t device, int64_t budget_bytes, dfr_cont
     return dfr_fail(error, DFR_DEVICE, "context set-up: %s", hipGetErrorString(e));
   }
   ctx->tune = default_tuning(ctx);
+  {
+    dfr_device_info info;
+    char *ierr = nullptr;
+    if (dfr_device_info_get(device, &info, &ierr) != DFR_OK) {
+      free(ierr);
+      snprintf(info.name, sizeof info.name, "%s", prop.name);
+    }
+    apply_calibration(ctx, prop, info.name);
+  }
   int st = noalloc_init(device, error);
   if (st) {
     dfr_context_destroy(ctx);
@@ -374,6 +383,30 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.jit_wait = (int)value;
   else
     return dfr_fail(error, DFR_INVALID, "bad tuning %s=%lld", key, (long long)value);
+  return DFR_OK;
+}
+
+extern "C" int dfr_context_tune_get(const dfr_context *ctx, const char *key, int64_t *value, char **error) {
+  if (!ctx || !key || !value) return dfr_fail(error, DFR_INVALID, "null argument");
+  const dfr::Tuning &t = ctx->tune;
+  const std::map<std::string, int64_t> v = {
+      {"block", t.block}, {"wg_per_cu", t.wg_per_cu}, {"lds_bytes", t.lds_bytes},
+      {"scan_block", t.scan_block}, {"scan_wg_per_cu", t.scan_wg_per_cu}, {"scan_lds_bytes", t.scan_lds_bytes},
+      {"lds_probe", t.lds_probe}, {"lds_mode", t.lds_mode}, {"direct_max_slots", t.direct_max_slots},
+      {"hash_probe", t.hash_probe}, {"hash_slots", t.hash_slots}, {"agg_mode", t.agg_mode},
+      {"part_lds_bytes", t.part_lds_bytes}, {"scan_unroll", t.scan_unroll}, {"two_level", t.two_level},
+      {"max_parts", t.max_parts}, {"sub_lds_bytes", t.sub_lds_bytes}, {"sub_block", t.sub_block},
+      {"sub_fill", t.sub_fill}, {"sub_wg_per_cu", t.sub_wg_per_cu}, {"batch_tile", t.batch_tile},
+      {"batch_graph", t.batch_graph}, {"batch_part_rows", t.batch_part_rows},
+      {"scatter_slices", t.scatter_slices}, {"scatter_stage", t.scatter_stage}, {"hot_keys", t.hot_keys},
+      {"hot_rows", t.hot_rows}, {"hot_domain", t.hot_domain}, {"hot_sample", t.hot_sample},
+      {"hot_share", t.hot_share}, {"rec_pack", t.rec_pack}, {"scatter_chunk", t.scatter_chunk},
+      {"two_level_parts", t.two_level_parts}, {"part_block", t.part_block}, {"lds_odd_stride", t.lds_odd_stride},
+      {"part_odd_stride", t.part_odd_stride}, {"scan_prefetch", t.scan_prefetch}, {"jit", t.jit},
+      {"jit_rows", t.jit_rows}, {"jit_wait", t.jit_wait}};
+  auto it = v.find(key);
+  if (it == v.end()) return dfr_fail(error, DFR_INVALID, "unknown tuning key %s", key);
+  *value = it->second;
   return DFR_OK;
 }
 

@@ -4,6 +4,7 @@
 
 #include <atomic>
 #include <map>
+#include <string>
 #include <memory>
 #include <mutex>
 #include <vector>
@@ -106,6 +107,8 @@ struct dfr_context {
   std::mutex hold_mu;
   int64_t holds = 0;
   bool retired = false;
+  // what became of the device's calibration profile at creation
+  std::string calibration;
 };
 
 // Counts a result of a lane context (see dfr_context::holds).
@@ -172,4 +175,7 @@ int dfr_fail(char **error, int status, const char *fmt, ...);
 // Looks up a device's address of the no-alloc scratch array (once per device).
 int noalloc_init(int device, char **error);
 dfr::Tuning default_tuning(dfr_context *ctx);
+// Applies the device's identity-matched calibration profile (if any) to a
+// new context's tuning.
+void apply_calibration(dfr_context *ctx, const hipDeviceProp_t &prop, const char *device_name);
 int column_stats(dfr_context *ctx, dfr_column *col, char **error);

@@ -124,6 +124,12 @@ def describe(plan: LazyFrame) -> str:
         f"  FILTER {len(prog.leaves)} leaves, truth table {[hex(t) for t in prog.truth[: max(1, (1 << len(prog.leaves)) // 32)]]}",
         f"  AGG {', '.join(dplan.agg_names)}",
     ]
+    prof = calibration.load_profile(0)
+    if prof is not None:
+        rows = plan.source.height()
+        est = calibration.estimate_ns(prof, plan, PipelineFeatures.rows(rows))
+        lines.append(f"  PLACE device 0: {est / 1e6:.3f} ms for {rows} resident rows (calibrated "
+                     f"{prof['identity']['device']}, build {prof['identity']['build_id']})")
     return "\n".join(lines) + "\n"
 
 
@@ -200,12 +206,12 @@ def pipeline_estimate(pipeline: LazyFrame, device: int, features: PipelineFeatur
 
 
 def pipeline_costs(pipeline: LazyFrame, device: int, features: PipelineFeatures) -> tuple[float, float]:
-    if not pipeline_supports(pipeline, device):
-        return (-1.0, -1.0)
-    prof = calibration.load_profile(device)
-    if prof is None:
-        return (-1.0, -1.0)
-    return (calibration.cpu_estimate_ns(prof, pipeline, features), calibration.estimate_ns(prof, pipeline, features))
+    """(CPU, device) nanoseconds. The device side is the calibrated model;
+    the CPU side is -1: this provider measures no CPU executor, so the
+    planner keeps its generic CPU calibration (accelerator-contract.md:47,
+    "A provider without matched CPU measurements can return
+    (-1, pipeline_estimate(...))")."""
+    return (-1.0, pipeline_estimate(pipeline, device, features))
 
 
 def run_pipeline(pipeline: LazyFrame, device: int) -> DataFrame:

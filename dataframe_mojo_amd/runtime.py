@@ -254,6 +254,23 @@ class Context:
         e = _err()
         _lib.check(self.lib.dfr_context_tune(self.handle, key.encode(), int(value), ctypes.byref(e)), e)
 
+    def tune_get(self, key: str) -> int:
+        v = ctypes.c_int64()
+        e = _err()
+        _lib.check(self.lib.dfr_context_tune_get(self.handle, key.encode(), ctypes.byref(v), ctypes.byref(e)), e)
+        return int(v.value)
+
+    def calibration(self) -> str:
+        """What became of the device's calibration profile at creation:
+        "applied: <path> (...)" or "none: <reason>"."""
+        t = ctypes.c_void_p()
+        e = _err()
+        _lib.check(self.lib.dfr_context_calibration(self.handle, ctypes.byref(t), ctypes.byref(e)), e)
+        try:
+            return ctypes.string_at(t.value).decode()
+        finally:
+            self.lib.dfr_free(t)
+
     def observe(self) -> dict:
         o = _lib.Observation()
         self.lib.dfr_context_observe(self.handle, ctypes.byref(o))

@@ -113,6 +113,23 @@ int dfr_context_observe(const dfr_context *ctx, dfr_observation *out);
  * and queries' scratch); blocks in use are not touched. */
 void dfr_context_trim(dfr_context *ctx);
 
+/* The identity of the sources this library was built from (a hash of its
+ * kernels, runtime and this header). A calibration profile records the
+ * build it was measured with and is applied only to that build (no
+ * counterpart in the reference's C ABI; the Metal provider's profiles carry
+ * the same kind of identity, dataframe_metal/calibration.mojo:29-41). */
+const char *dfr_build_id(void);
+
+/* What became of the device's calibration profile when the context was
+ * created: "applied: <path> (<n> settings)" or "none: <reason>". Profiles
+ * (JSON, written by the autotuner) are read from
+ * $DATAFRAME_ROCM_CALIBRATION ("off": none) or from
+ * calibration/<arch>-<CUs>cu.json beside the library; one is applied only
+ * when its identity (device name, arch, compute units, build id) matches,
+ * as the reference's identity-matched profiles
+ * (dataframe_metal/calibration.mojo:1-12). *text is freed with dfr_free. */
+int dfr_context_calibration(const dfr_context *ctx, char **text, char **error);
+
 /* Sets one launch or table parameter of the context (no counterpart in the
  * reference: the autotuner's knob). Keys: "block", "wg_per_cu",
  * "lds_bytes" (threads per workgroup, persistent workgroups per CU and LDS
@@ -171,6 +188,11 @@ void dfr_context_trim(dfr_context *ctx);
  * column range fits ride in the record's key word, which shortens the
  * records the partitions move; 0: every value has its own word). */
 int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value, char **error);
+
+/* The current value of a dfr_context_tune key (the built-in default, the
+ * calibration profile's, or the last dfr_context_tune); no counterpart in
+ * the reference: the autotuner reads each knob before it varies it. */
+int dfr_context_tune_get(const dfr_context *ctx, const char *key, int64_t *value, char **error);
 
 /* ---- device columns (the column hand-off) ------------------------------ */
 typedef struct dfr_column dfr_column;
