@@ -184,3 +184,30 @@ def test_runtime_compiler_program_writes_a_code_object(tmp_path):
     bad.write_text('extern "C" __global__ void k(int *p) { p[threadIdx.x] = undefined_name; }\n')
     r = subprocess.run([exe, str(bad), str(tmp_path / "bad.co"), *JITC_OPTS], capture_output=True, text=True, timeout=300)
     assert r.returncode == 2 and "undefined_name" in r.stderr
+
+
+def test_mojo_provider_binds_only_declared_symbols_and_constants():
+    """The Mojo provider package (uncompiled here: no Mojo toolchain) names
+    only entry points include/dfrocm.h declares, and its status, type,
+    leaf, aggregate and buffer constants equal the header's."""
+    pkg = os.path.join(ROOT, "providers", "rocm", "dataframe_provider")
+    text = ""
+    for name in sorted(os.listdir(pkg)):
+        if name.endswith(".mojo"):
+            with open(os.path.join(pkg, name)) as f:
+                text += f.read()
+    declared = set(declared_symbols())
+    named = set(re.findall(r'"(dfr_[a-z_0-9]+)"', text))
+    assert named and named <= declared, sorted(named - declared)
+    with open(os.path.join(ROOT, "include", "dfrocm.h")) as f:
+        header = f.read()
+    defines = dict((k, int(v)) for k, v in re.findall(r"#define (DFR_[A-Z_0-9]+) (-?\d+)", header))
+    consts = dict((k, int(v)) for k, v in re.findall(r"comptime (DFR_[A-Z_0-9]+) = (-?\d+)", text))
+    assert len(consts) >= 30
+    wrong = {k: (v, defines.get(k)) for k, v in consts.items() if defines.get(k) != v}
+    assert not wrong, wrong
+    # the 13 provider functions of providers/cpu/dataframe_provider/__init__.mojo
+    for fn in ["installed", "execute", "describe", "execute_profiled", "handoff_supports", "handoff_execute",
+               "handoff_estimate", "devices", "transfer_model", "pipeline_supports", "pipeline_estimate",
+               "pipeline_costs", "run_pipeline"]:
+        assert re.search(rf"^def {fn}\(", text, flags=re.M), fn
