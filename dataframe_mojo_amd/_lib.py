@@ -64,6 +64,8 @@ EXPORTS = [
     "dfr_groupby_batch",
     "dfr_groupby_into",
     "dfr_column_slice",
+    "dfr_column_upload_selection",
+    "dfr_column_compute",
     "dfr_server_create",
     "dfr_server_submit",
     "dfr_server_wait",
@@ -92,6 +94,29 @@ class DeviceInfo(ctypes.Structure):
         ("memory_bytes", ctypes.c_int64),
         ("name", ctypes.c_char * 64),
         ("arch", ctypes.c_char * 32),
+    ]
+
+
+class Source(ctypes.Structure):
+    """dfr_source: a host source column window."""
+
+    _fields_ = [
+        ("values", ctypes.c_void_p),
+        ("offset", ctypes.c_int64),
+        ("length", ctypes.c_int64),
+        ("validity", ctypes.c_void_p),
+        ("bit_offset", ctypes.c_int64),
+    ]
+
+
+class Selection(ctypes.Structure):
+    """dfr_selection: one batch of a chunked or selected column."""
+
+    _fields_ = [
+        ("source", Source),
+        ("start", ctypes.c_int64),
+        ("rows", ctypes.c_void_p),
+        ("count", ctypes.c_int64),
     ]
 
 
@@ -131,6 +156,31 @@ class Predicate(ctypes.Structure):
         ("pad", ctypes.c_int32),
         ("truth", ctypes.c_uint32 * 8),
         ("leaves", Leaf * MAX_LEAVES),
+    ]
+
+
+MAX_VALUE_OPS = 32
+MAX_CONDITIONS = 4
+VAL_COLUMN, VAL_LITERAL, VAL_NULL, VAL_ADD, VAL_SUB, VAL_MUL, VAL_DIV, VAL_NEG, VAL_ABS, VAL_CASE = range(10)
+
+
+class ValueOp(ctypes.Structure):
+    _fields_ = [
+        ("op", ctypes.c_int32),
+        ("arg", ctypes.c_int32),
+        ("dtype", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("literal", ctypes.c_int64),
+    ]
+
+
+class ValueProgram(ctypes.Structure):
+    _fields_ = [
+        ("n_ops", ctypes.c_int32),
+        ("n_conditions", ctypes.c_int32),
+        ("ops", ValueOp * MAX_VALUE_OPS),
+        ("conditions", Predicate * MAX_CONDITIONS),
+        ("keep", Predicate),
     ]
 
 
@@ -218,6 +268,8 @@ def load(required: bool = True):
     lib.dfr_groupby.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(vp), errp]
     lib.dfr_groupby_into.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(Output), ctypes.POINTER(i64), errp]
     lib.dfr_column_slice.argtypes = [vp, i64, i64, ctypes.POINTER(vp), errp]
+    lib.dfr_column_upload_selection.argtypes = [vp, i32, ctypes.POINTER(Selection), i32, i64, ctypes.POINTER(vp), errp]
+    lib.dfr_column_compute.argtypes = [vp, ctypes.POINTER(ValueProgram), ctypes.POINTER(vp), i32, ctypes.POINTER(vp), errp]
     lib.dfr_server_create.argtypes = [vp, i32, i32, i64, ctypes.POINTER(vp), errp]
     lib.dfr_server_submit.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(i64), errp]
     lib.dfr_server_wait.argtypes = [vp, i64, i64, ctypes.POINTER(vp), ctypes.POINTER(i64), errp]

@@ -43,7 +43,7 @@ COMMON = [
 
 LIBS = {
     "libdfrocm.so": {
-        "sources": ["dfrocm.hip", "groupby.hip", "server.hip", "jit.hip", "calib.hip"],
+        "sources": ["dfrocm.hip", "groupby.hip", "server.hip", "jit.hip", "calib.hip", "selection.hip"],
         "extra": [f"-Wl,--version-script={os.path.join(CSRC, 'libdfrocm.map')}", "-lhiprtc", "-ldl"],
     },
     "libdfrgen.so": {"sources": ["datagen.hip"], "extra": []},

@@ -1,8 +1,10 @@
 """Expressions for the filter -> group_by -> agg path (expr.mojo:2869-3006).
 
 Nodes: col, lit, comparisons (== != < <= > >=), %, & | ~, is_null,
-is_not_null, is_in, is_between, and the reductions sum(min_count), count,
-len, min, max, mean, var(ddof), std(ddof), with alias.
+is_not_null, is_in, is_between, arithmetic (+ - * /, unary -, abs), CASE
+(when(...).then(...).otherwise(...), chained when/then), and the reductions
+sum(min_count), count, len, min, max, mean, var(ddof), std(ddof) of a
+column or of an arithmetic/CASE value, with alias.
 """
 from __future__ import annotations
 
@@ -67,6 +69,58 @@ class Expr:
     # arithmetic / logic --------------------------------------------------
     def __mod__(self, other) -> "Expr":
         return Expr("mod", (self, _wrap(other)))
+
+    def _arith(self, op: str, other, swap: bool = False) -> "Expr":
+        a, b = (_wrap(other), self) if swap else (self, _wrap(other))
+        return Expr("arith", (op, a, b))
+
+    def __add__(self, other) -> "Expr":
+        return self._arith("add", other)
+
+    def __radd__(self, other) -> "Expr":
+        return self._arith("add", other, True)
+
+    def __sub__(self, other) -> "Expr":
+        return self._arith("sub", other)
+
+    def __rsub__(self, other) -> "Expr":
+        return self._arith("sub", other, True)
+
+    def __mul__(self, other) -> "Expr":
+        return self._arith("mul", other)
+
+    def __rmul__(self, other) -> "Expr":
+        return self._arith("mul", other, True)
+
+    def __truediv__(self, other) -> "Expr":
+        """True division: Float64 (integers convert), IEEE by zero."""
+        return self._arith("div", other)
+
+    def __rtruediv__(self, other) -> "Expr":
+        return self._arith("div", other, True)
+
+    def __neg__(self) -> "Expr":
+        return Expr("neg", (self,))
+
+    def abs(self) -> "Expr":
+        return Expr("abs", (self,))
+
+    # CASE ----------------------------------------------------------------
+    def otherwise(self, value) -> "Expr":
+        """The ELSE of a when/then chain (without it, unmatched rows are
+        null, expr.mojo:901-904)."""
+        if self.kind != "case":
+            raise ValueError("otherwise() follows when(...).then(...)")
+        cond, then, other = self.args
+        if other.kind == "case":
+            return Expr("case", (cond, then, other.otherwise(value)), self.name)
+        return Expr("case", (cond, then, _wrap(value)), self.name)
+
+    def when(self, condition) -> "When":
+        """A further WHEN of a chain: when(a).then(x).when(b).then(y)."""
+        if self.kind != "case":
+            raise ValueError("when() chains after when(...).then(...)")
+        return When(_wrap(condition), self)
 
     def __and__(self, other) -> "Expr":
         return Expr("and", (self, _wrap(other)))
@@ -145,6 +199,26 @@ class Expr:
         else:
             s = f"{self.kind}({', '.join(repr(a) for a in self.args)})"
         return s if self.name is None else f"{s}.alias({self.name!r})"
+
+
+class When:
+    """when(condition) awaiting its then(value) (expr.mojo WHEN: left =
+    predicate, right = then, extra = otherwise)."""
+
+    def __init__(self, condition: Expr, chain: Expr | None = None):
+        self.condition = condition
+        self.chain = chain
+
+    def then(self, value) -> Expr:
+        case = Expr("case", (self.condition, _wrap(value), lit(None)))
+        if self.chain is None:
+            return case
+        # a chained WHEN is the ELSE of the previous one
+        return self.chain.otherwise(case)
+
+
+def when(condition) -> When:
+    return When(_wrap(condition))
 
 
 def _wrap(v) -> Expr:

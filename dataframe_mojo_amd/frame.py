@@ -115,6 +115,47 @@ class Column:
         return np.packbits(self.valid, bitorder="little")
 
 
+class ChunkedColumn(Column):
+    """A column held as several chunks of one dtype (a chunked Series,
+    series.mojo chunks(); String chunks share one dictionary, as the
+    reference groups chunked String keys only when every chunk carries
+    codes into one dictionary, series.mojo:337-370). The device path
+    uploads the chunks as they are (dfr_column_upload_selection, one
+    transfer, joined on the device); `values` and `valid` join them on the
+    host only when host code reads them."""
+
+    def __init__(self, chunks: Sequence[Column]):
+        chunks = list(chunks)
+        if not chunks:
+            raise ValueError("a chunked column needs at least one chunk")
+        self.dtype = chunks[0].dtype
+        if any(c.dtype != self.dtype for c in chunks):
+            raise ValueError("chunks of one column share a dtype")
+        if self.dtype == STRING and any(c.labels != chunks[0].labels for c in chunks):
+            raise ValueError("String chunks share one dictionary")
+        self.chunks = chunks
+        self.labels = chunks[0].labels
+        self._joined = None
+
+    def _join(self):
+        if self._joined is None:
+            vals = np.concatenate([c.values for c in self.chunks])
+            valid = np.concatenate([c.is_valid() for c in self.chunks])
+            self._joined = (vals, None if valid.all() else valid)
+        return self._joined
+
+    @property
+    def values(self):
+        return self._join()[0]
+
+    @property
+    def valid(self):
+        return self._join()[1]
+
+    def __len__(self) -> int:
+        return sum(len(c) for c in self.chunks)
+
+
 @dataclass
 class Series:
     name: str

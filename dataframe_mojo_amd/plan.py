@@ -59,6 +59,18 @@ class Program:
 
 
 @dataclass
+class ValueSpec:
+    """An aggregate's expression as a device value program: computed by
+    dfr_column_compute into the column `name` of DevicePlan.columns."""
+
+    name: str
+    inputs: list  # input column names (program input i = inputs[i])
+    ops: list  # (op, arg, dtype code, literal bits), postfix
+    conditions: list  # Programs whose leaves name inputs
+    dtype: str  # the result type
+
+
+@dataclass
 class DevicePlan:
     keys: list
     columns: list  # names of the query's input columns
@@ -66,6 +78,7 @@ class DevicePlan:
     aggs: list  # (op, column index or -1, ddof, min_count)
     agg_names: list
     maintain_order: bool
+    values: list = field(default_factory=list)  # ValueSpec of each derived column
 
     @property
     def output_names(self) -> list:
@@ -224,6 +237,176 @@ class _Compiler:
         return self.leaf(Leaf(name, _CMP[op], flags, int(value), modulus))
 
 
+# ------------------------------------------------------------ value programs
+#
+# Types follow the reference's promotion (docs/expressions.md:57-84): two
+# integer operands compute in the wider (Int32 with Int32 stays Int32,
+# checked at 32 bits); an integer with Float64 is Float64; `/` is Float64. A
+# bare literal takes the other operand's type when it fits, else Int32, or
+# Int64 when it needs more; a float literal beside an integer is Float64; a
+# literal with a literal is Int64 (Float64 for a float).
+
+_RANK = {INT32: 0, INT64: 1, FLOAT64: 2}
+_VAL_CODE = {INT32: _lib.DFR_INT32, INT64: _lib.DFR_INT64, FLOAT64: _lib.DFR_FLOAT64}
+_VAL_OP = {"add": _lib.VAL_ADD, "sub": _lib.VAL_SUB, "mul": _lib.VAL_MUL, "div": _lib.VAL_DIV}
+
+
+def _is_bare_lit(e: Expr) -> bool:
+    return e.kind == "lit"
+
+
+def _lit_own_type(v) -> str:
+    if isinstance(v, float):
+        return FLOAT64
+    return INT32 if -(1 << 31) <= v < (1 << 31) else INT64
+
+
+class _ValueCompiler:
+    def __init__(self, frame, name: str):
+        self.frame = frame
+        self.name = name
+        self.inputs: list[str] = []
+        self.ops: list = []
+        self.conditions: list = []
+
+    def input(self, name: str) -> int:
+        if name not in self.inputs:
+            self.inputs.append(name)
+        return self.inputs.index(name)
+
+    def type_of(self, e: Expr) -> str:
+        """The node's type; "int?"/"float?"/"null" for a bare literal."""
+        k = e.kind
+        if k == "col":
+            try:
+                dt = self.frame.column(e.args[0]).dtype
+            except KeyError:
+                raise Unsupported(f"no column named {e.args[0]!r}") from None
+            if dt not in _RANK:
+                raise Unsupported(f"{dt} columns are not values on the device path")
+            return dt
+        if k == "lit":
+            v = e.args[0]
+            if v is None:
+                return "null"
+            if isinstance(v, bool) or not isinstance(v, (int, float)):
+                raise Unsupported(f"literal {v!r} is not a numeric value")
+            if isinstance(v, int) and not (-(1 << 63) <= v < (1 << 63)):
+                raise Unsupported("literal outside Int64")
+            return "float?" if isinstance(v, float) else "int?"
+        if k == "arith":
+            op, a, b = e.args
+            if op == "div":
+                self.type_of(a), self.type_of(b)
+                return FLOAT64
+            return self._combine(a, b)
+        if k in ("neg", "abs"):
+            t = self.type_of(e.args[0])
+            return {"int?": INT64, "float?": FLOAT64, "null": INT64}.get(t, t)
+        if k == "case":
+            return self._combine(e.args[1], e.args[2])
+        raise Unsupported(f"{e!r} is not a value on the device path")
+
+    def _combine(self, a: Expr, b: Expr) -> str:
+        ta, tb = self.type_of(a), self.type_of(b)
+        return self._bind_pair(a, ta, b, tb)[2]
+
+    def _bind_pair(self, a, ta, b, tb):
+        """(type of a, type of b, node type) after literal binding."""
+        def bound(e, t, other):
+            if t in _RANK:
+                return t
+            if t == "null":
+                return other if other in _RANK else None
+            if t == "float?":
+                return FLOAT64
+            # an untyped integer literal
+            if other == FLOAT64 or other == "float?":
+                return FLOAT64
+            if other in (INT32, INT64) and _is_bare_lit(e):
+                v = e.args[0]
+                lo, hi = (-(1 << 31), 1 << 31) if other == INT32 else (-(1 << 63), 1 << 63)
+                return other if lo <= v < hi else _lit_own_type(v)
+            return INT64
+        ba, bb = bound(a, ta, tb), bound(b, tb, ta)
+        if ba is None and bb is None:
+            ba = bb = INT64
+        ba = ba or bb
+        bb = bb or ba
+        node = ba if _RANK[ba] >= _RANK[bb] else bb
+        return ba, bb, node
+
+    def emit(self, e: Expr, t: str):
+        """Postfix ops of `e` computed as type `t` (its bound type)."""
+        k = e.kind
+        if k == "col":
+            self.ops.append((_lib.VAL_COLUMN, self.input(e.args[0]), _VAL_CODE[self.type_of(e)], 0))
+            return
+        if k == "lit":
+            v = e.args[0]
+            if v is None:
+                self.ops.append((_lib.VAL_NULL, 0, _VAL_CODE[t], 0))
+            elif t == FLOAT64:
+                self.ops.append((_lib.VAL_LITERAL, 0, _VAL_CODE[t], _f64_bits(float(v))))
+            else:
+                self.ops.append((_lib.VAL_LITERAL, 0, _VAL_CODE[t], int(v)))
+            return
+        if k == "arith":
+            op, a, b = e.args
+            ba, bb, node = self._bind_pair(a, self.type_of(a), b, self.type_of(b))
+            if op == "div":
+                node = FLOAT64
+            self.emit(a, ba)
+            self.emit(b, bb)
+            self.ops.append((_VAL_OP[op], 0, _VAL_CODE[node], 0))
+            return
+        if k in ("neg", "abs"):
+            x = e.args[0]
+            tx = self.type_of(x)
+            bt = {"int?": INT64, "float?": FLOAT64, "null": INT64}.get(tx, tx)
+            self.emit(x, bt)
+            self.ops.append((_lib.VAL_NEG if k == "neg" else _lib.VAL_ABS, 0, _VAL_CODE[bt], 0))
+            return
+        if k == "case":
+            cond, then, other = e.args
+            bt, bo, node = self._bind_pair(then, self.type_of(then), other, self.type_of(other))
+            if len(self.conditions) == _lib.MAX_CONDITIONS:
+                raise Unsupported(f"more than {_lib.MAX_CONDITIONS} CASE conditions in one value")
+            prog = compile_predicate(self.frame, [cond])
+            for leaf in prog.leaves:
+                self.input(leaf.column)
+                if leaf.column2 is not None:
+                    self.input(leaf.column2)
+            self.conditions.append(prog)
+            ci = len(self.conditions) - 1
+            self.emit(then, bt)
+            self.emit(other, bo)
+            self.ops.append((_lib.VAL_CASE, ci, _VAL_CODE[node], 0))
+            return
+        raise Unsupported(f"{e!r} is not a value on the device path")
+
+    def compile(self, e: Expr, filter_program: Program) -> ValueSpec:
+        t = self.type_of(e)
+        t = {"int?": INT64, "float?": FLOAT64, "null": INT64}.get(t, t)
+        self.emit(e, t)
+        for leaf in filter_program.leaves:
+            self.input(leaf.column)
+            if leaf.column2 is not None:
+                self.input(leaf.column2)
+        if len(self.ops) > _lib.MAX_VALUE_OPS:
+            raise Unsupported(f"a value program has at most {_lib.MAX_VALUE_OPS} nodes")
+        if len(self.inputs) > 8:
+            raise Unsupported("a value program reads at most 8 columns (its filter's included)")
+        depth = peak = 0
+        for op, _, _, _ in self.ops:
+            depth += 1 if op in (_lib.VAL_COLUMN, _lib.VAL_LITERAL, _lib.VAL_NULL) else (
+                0 if op in (_lib.VAL_NEG, _lib.VAL_ABS) else -1)
+            peak = max(peak, depth)
+        if peak > 8:
+            raise Unsupported("a value program needs more than 8 stack entries")
+        return ValueSpec(self.name, self.inputs, self.ops, self.conditions, t)
+
+
 def _eval(f, bits: int) -> bool:
     t = f[0]
     if t == "leaf":
@@ -274,6 +457,8 @@ def compile_plan(plan: LazyFrame) -> DevicePlan:
         if leaf.column2 is not None:
             colindex(leaf.column2)
     aggs, names = [], []
+    values: list[ValueSpec] = []
+    value_exprs: list[str] = []
     for e in plan.aggs:
         if not isinstance(e, Expr) or e.kind != "agg":
             raise Unsupported(f"{e!r} is not a reduction")
@@ -281,9 +466,17 @@ def compile_plan(plan: LazyFrame) -> DevicePlan:
         kw = dict(kw)
         if target is None:
             aggs.append((_lib.AGG_LEN, -1, 0, 0))
+        elif target.kind != "col":
+            # an arithmetic or CASE value: a derived column (one per distinct
+            # expression) computed on the device for the kept rows
+            vname = next((v.name for v, ex in zip(values, value_exprs) if ex == repr(target)), None)
+            if vname is None:
+                vname = f"\x00value{len(values)}"
+                values.append(_ValueCompiler(frame, vname).compile(target, program))
+                value_exprs.append(repr(target))
+            columns.append(vname) if vname not in columns else None
+            aggs.append((_AGG[op], columns.index(vname), int(kw.get("ddof", 1)), int(kw.get("min_count", 0))))
         else:
-            if target.kind != "col":
-                raise Unsupported("reductions of expressions are not on the device path")
             name = target.args[0]
             dt = frame.column(name).dtype
             if dt == STRING and op not in ("count", "len"):
@@ -293,4 +486,4 @@ def compile_plan(plan: LazyFrame) -> DevicePlan:
     out = list(plan.keys) + names
     if len(set(out)) != len(out):
         raise ValueError(f"duplicate output names {out}")
-    return DevicePlan(list(plan.keys), columns, program, aggs, names, plan.maintain_order)
+    return DevicePlan(list(plan.keys), columns, program, aggs, names, plan.maintain_order, values)

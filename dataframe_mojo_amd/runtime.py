@@ -15,7 +15,7 @@ import threading
 import numpy as np
 
 from . import _lib
-from .frame import FLOAT64, INT32, INT64, STRING, Column, DataFrame, Series
+from .frame import FLOAT64, INT32, INT64, STRING, ChunkedColumn, Column, DataFrame, Series
 from .plan import DTYPE_CODE, DevicePlan
 
 MEMORY_LIMIT_ENV = "DATAFRAME_ACCEL_MEMORY_LIMIT"
@@ -231,6 +231,37 @@ def build_query(plan: DevicePlan, keys: list, columns: list):
     return q, (key_arr, col_arr, aggs)
 
 
+def _fill_predicate(pred, prog, names: list):
+    """A dfr_predicate from a compiled Program whose leaves name columns of
+    `names` (their index is the leaf's column)."""
+    pred.n_leaves = len(prog.leaves)
+    for i in range(8):
+        pred.truth[i] = prog.truth[i] & 0xFFFFFFFF
+    for i, leaf in enumerate(prog.leaves):
+        L = pred.leaves[i]
+        L.column = names.index(leaf.column)
+        L.op = leaf.op
+        L.flags = leaf.flags
+        L.literal = names.index(leaf.column2) if leaf.column2 is not None else leaf.literal
+        L.modulus = leaf.modulus
+
+
+def build_value_program(spec, keep) -> "_lib.ValueProgram":
+    """The dfr_value_program of a ValueSpec; `keep` is the query's filter
+    Program (rows it drops are neither evaluated nor faulted)."""
+    vp = _lib.ValueProgram()
+    vp.n_ops = len(spec.ops)
+    for i, (op, arg, dtype, literal) in enumerate(spec.ops):
+        o = vp.ops[i]
+        o.op, o.arg, o.dtype = op, arg, dtype
+        o.literal = literal
+    vp.n_conditions = len(spec.conditions)
+    for c, prog in enumerate(spec.conditions):
+        _fill_predicate(vp.conditions[c], prog, spec.inputs)
+    _fill_predicate(vp.keep, keep, spec.inputs)
+    return vp
+
+
 class Context:
     def __init__(self, device: int = 0, budget: int = 0):
         self.lib = _lib.load()
@@ -279,6 +310,12 @@ class Context:
     # columns -------------------------------------------------------------
     def upload(self, column: Column) -> DeviceColumn:
         dt = DTYPE_CODE[column.dtype]
+        if isinstance(column, ChunkedColumn):
+            # every chunk in one transfer, joined on the device
+            from .selection import upload_pieces
+
+            h = upload_pieces(self, dt, [(c, 0, None, len(c)) for c in column.chunks], len(column.labels or []))
+            return DeviceColumn(self, h, dt, len(column), column.labels)
         vals = np.ascontiguousarray(column.values)
         bits = column.validity_bytes()
         h = ctypes.c_void_p()
@@ -335,6 +372,18 @@ class Context:
         del keep
         return Result(self, h, len(keys), len(plan.aggs))
 
+
+    def compute(self, spec, keep, inputs: list) -> DeviceColumn:
+        """dfr_column_compute: a ValueSpec's column over device `inputs`
+        (one per spec.inputs), for the rows `keep` (a Program) keeps."""
+        vp = build_value_program(spec, keep)
+        arr = (ctypes.c_void_p * len(inputs))(*[c.handle for c in inputs])
+        h = ctypes.c_void_p()
+        e = _err()
+        with self.lock:
+            _lib.check(self.lib.dfr_column_compute(self.handle, ctypes.byref(vp), arr, len(inputs), ctypes.byref(h),
+                                                   ctypes.byref(e)), e)
+        return DeviceColumn(self, h, DTYPE_CODE[spec.dtype], inputs[0].length if inputs else 0)
 
     def groupby_into(self, plan: DevicePlan, keys: list, columns: list, out) -> int:
         """dfr_groupby_into: the query into caller-owned device buffers,
@@ -515,7 +564,12 @@ def run_plan(ctx: Context, frame: DataFrame, plan: DevicePlan) -> tuple[DataFram
 
     def dev(name):
         if name not in uploaded:
-            uploaded[name] = ctx.upload(frame.column(name).column)
+            spec = next((v for v in plan.values if v.name == name), None)
+            if spec is None:
+                uploaded[name] = ctx.upload(frame.column(name).column)
+            else:
+                # an aggregate's expression, computed for the kept rows
+                uploaded[name] = ctx.compute(spec, plan.program, [dev(i) for i in spec.inputs])
         return uploaded[name]
 
     keys = [dev(k) for k in plan.keys]

@@ -217,6 +217,37 @@ int dfr_column_wrap(dfr_context *ctx, int32_t dtype, const void *device_values,
                     const uint64_t *device_validity, int64_t bit_offset,
                     int64_t dict_size, dfr_column **out, char **error);
 
+/* ---- chunked and selected inputs (streaming hand-off) -------------------
+ * A column that reaches the device as several pieces: the chunks of a
+ * chunked Series (series.mojo chunks()), or the batches a CPU pipeline
+ * hands over with a selection instead of a copy (SelectionBatch,
+ * vector/selection_batches.mojo:13-75: each batch keeps its source column,
+ * a start and an optional row list). Batch b contributes `count` rows,
+ * after the rows of batches 0..b-1; its row i is row start + rows[i] of its
+ * source window (rows NULL: start + i). The sources are host memory. Only
+ * the span of source rows a batch selects is copied, in one transfer for
+ * the whole column, and a device kernel gathers the rows and their
+ * validity bits into one device column (which then has statistics as an
+ * upload does). A row outside its source window is DFR_INVALID before any
+ * transfer, as SelectionBatch.validate rejects it
+ * (vector/selection_batches.mojo:34-58). */
+typedef struct dfr_source {
+    const void *values;      /* host: the source column's value buffer */
+    int64_t offset;          /* element offset of its window */
+    int64_t length;          /* rows of its window */
+    const uint8_t *validity; /* LSB-first bitmap, bit bit_offset + i is window row i; NULL: all valid */
+    int64_t bit_offset;
+} dfr_source;
+typedef struct dfr_selection {
+    dfr_source source;
+    int64_t start;           /* first window row the batch reads */
+    const uint32_t *rows;    /* count row numbers relative to start; NULL: contiguous */
+    int64_t count;           /* rows the batch contributes */
+} dfr_selection;
+int dfr_column_upload_selection(dfr_context *ctx, int32_t dtype, const dfr_selection *batches,
+                                int32_t n_batches, int64_t dict_size, dfr_column **out,
+                                char **error);
+
 /* A window [offset, offset + length) of a column, without device work: it
  * borrows the parent's buffers (the parent must outlive it) and takes the
  * parent's statistics as bounds, so it costs no launch; the reference's
@@ -312,6 +343,64 @@ typedef struct dfr_query {
     int32_t maintain_order; /* groups in order of their first kept row */
     const dfr_agg *aggs;
 } dfr_query;
+
+/* ---- value programs: arithmetic and CASE inside aggregates --------------
+ * An aggregate may reduce an expression instead of a column, e.g.
+ * (col("price") * col("quantity")).sum() or
+ * when(col("q") > 0).then(col("p") * 2).otherwise(0).mean()
+ * (docs/apple-metal.md:104-120, 635-650, 711-725). dfr_column_compute
+ * evaluates such an expression, a postfix program, for every row the `keep`
+ * predicate keeps (the query's filter: other rows are null and never
+ * evaluated) into a new device column that a query then aggregates.
+ * Semantics (docs/expressions.md:50-100): nulls propagate through
+ * arithmetic; integer + - * and unary - / abs are checked at the node's
+ * width (Int32 nodes compute wider and range-check) and an overflow in a
+ * kept row is a data fault with the reference's text ("Int64 sum
+ * overflow", "Int64 subtraction overflow", "Int64 multiplication
+ * overflow", "int32 addition overflow", ..., kernels/binary.mojo:108-122,
+ * 415-421, 1620-1635), also when a later operand makes the result null;
+ * when several nodes overflow, the first in program order is reported, as
+ * the CPU evaluates node by node. `/` is IEEE Float64 division. CASE takes
+ * its THEN value where its condition is true and its ELSE value otherwise
+ * (a null condition takes ELSE); only the taken branch's overflow counts. */
+#define DFR_VAL_COLUMN 0   /* push input `arg` */
+#define DFR_VAL_LITERAL 1  /* push `literal` (int64 value or double bits) */
+#define DFR_VAL_NULL 2     /* push a null */
+#define DFR_VAL_ADD 3
+#define DFR_VAL_SUB 4
+#define DFR_VAL_MUL 5
+#define DFR_VAL_DIV 6      /* Float64 only */
+#define DFR_VAL_NEG 7
+#define DFR_VAL_ABS 8
+#define DFR_VAL_CASE 9     /* pops ELSE, then THEN; pushes condition `arg` ? THEN : ELSE */
+#define DFR_MAX_VALUE_OPS 32
+#define DFR_MAX_VALUE_INPUTS 8
+#define DFR_MAX_CONDITIONS 4
+#define DFR_MAX_VALUE_STACK 8
+
+typedef struct dfr_value_op {
+    int32_t op;
+    int32_t arg;      /* COLUMN: input index; CASE: condition index */
+    int32_t dtype;    /* the node's type: DFR_INT32, DFR_INT64 or DFR_FLOAT64; operands convert to it */
+    int32_t pad;
+    int64_t literal;  /* LITERAL: an int64 value or the bits of a double */
+} dfr_value_op;
+
+typedef struct dfr_value_program {
+    int32_t n_ops;
+    int32_t n_conditions;
+    dfr_value_op ops[DFR_MAX_VALUE_OPS];
+    dfr_predicate conditions[DFR_MAX_CONDITIONS];  /* leaf columns index the inputs */
+    dfr_predicate keep;                            /* rows evaluated; the others are null */
+} dfr_value_program;
+
+/* Computes `program` over `inputs` (equal lengths; Int32, Int64, Float64, and
+ * dictionary columns in conditions only) into a new column of the last
+ * node's type. DFR_DATA_FAULT (reference text) on a checked overflow in a
+ * kept row; DFR_INVALID for a malformed program (checked before any device
+ * work: stack depth, operand indexes, types). */
+int dfr_column_compute(dfr_context *ctx, const dfr_value_program *program, dfr_column *const *inputs,
+                       int32_t n_inputs, dfr_column **out, char **error);
 
 /* ---- results (device resident until copied) ---------------------------- */
 typedef struct dfr_result dfr_result;

@@ -167,6 +167,162 @@ def evaluate(frame, e):
     raise OracleError(f"oracle cannot evaluate {e!r}")
 
 
+# ------------------------------------------------------------ values
+#
+# Aggregate values (arithmetic, CASE), restated from docs/expressions.md:
+# 50-100 (nulls propagate; integer + - * and unary -/abs checked at the
+# operand width, Int32 computed wider and range-checked; `/` is Float64;
+# promotion: the wider integer, an integer with a float is the float; a
+# bare literal takes the other operand's type when it fits, else Int32, or
+# Int64 when it needs more; two literals are Int64) and the reference's
+# error texts (kernels/binary.mojo:108-122 "Int64 sum overflow", "Int64
+# subtraction overflow", "Int64 multiplication overflow"; :415-421 "<dtype>
+# <operation> overflow" below 64 bits; :1620-1635 negation and abs). Only
+# rows a node is evaluated on can fault: the filter's kept rows, within a
+# CASE branch the rows that take it (docs/apple-metal.md:711-725; a null
+# condition takes ELSE), and only where the operands are valid. Exact
+# Python integers decide every overflow.
+
+_BOUNDS = {INT32: (-(1 << 31), (1 << 31) - 1), INT64: (-(1 << 63), (1 << 63) - 1)}
+_RANKS = {INT32: 0, INT64: 1, FLOAT64: 2}
+_WORD = {("add", INT64): "Int64 sum overflow", ("sub", INT64): "Int64 subtraction overflow",
+         ("mul", INT64): "Int64 multiplication overflow", ("neg", INT64): "Int64 negation overflow",
+         ("abs", INT64): "Int64 abs overflow", ("add", INT32): "int32 addition overflow",
+         ("sub", INT32): "int32 subtraction overflow", ("mul", INT32): "int32 multiplication overflow",
+         ("neg", INT32): "int32 negation overflow", ("abs", INT32): "int32 abs overflow"}
+
+
+class ValueFault(OracleError):
+    """A checked overflow: the reference raises its text."""
+
+
+def _vtype(frame, e):
+    """A node's type, or ("lit", value) for a bare numeric literal, or
+    "null" for a null literal."""
+    if e.kind == "col":
+        return frame.column(e.args[0]).dtype
+    if e.kind == "lit":
+        return "null" if e.args[0] is None else ("lit", e.args[0])
+    if e.kind == "arith":
+        op, a, b = e.args
+        return FLOAT64 if op == "div" else _pair_types(frame, a, b)[2]
+    if e.kind in ("neg", "abs"):
+        t = _vtype(frame, e.args[0])
+        return _alone(t)
+    if e.kind == "case":
+        return _pair_types(frame, e.args[1], e.args[2])[2]
+    raise OracleError(f"oracle cannot type {e!r}")
+
+
+def _alone(t):
+    if isinstance(t, tuple):
+        return FLOAT64 if isinstance(t[1], float) else INT64
+    return INT64 if t == "null" else t
+
+
+def _lit_as(v, other):
+    """A literal's type beside an operand of type `other`."""
+    if isinstance(v, float) or other == FLOAT64 or (isinstance(other, tuple) and isinstance(other[1], float)):
+        return FLOAT64
+    if other in _BOUNDS:
+        lo, hi = _BOUNDS[other]
+        if lo <= v <= hi:
+            return other
+    if isinstance(other, tuple) or other == "null":
+        return INT64
+    return INT32 if _BOUNDS[INT32][0] <= v <= _BOUNDS[INT32][1] else INT64
+
+
+def _pair_types(frame, a, b):
+    ta, tb = _vtype(frame, a), _vtype(frame, b)
+    ba = _lit_as(ta[1], tb) if isinstance(ta, tuple) else ta
+    bb = _lit_as(tb[1], ta) if isinstance(tb, tuple) else tb
+    if ba == "null" and bb == "null":
+        ba = bb = INT64
+    ba = bb if ba == "null" else ba
+    bb = ba if bb == "null" else bb
+    return ba, bb, (ba if _RANKS[ba] >= _RANKS[bb] else bb)
+
+
+def _as_type(vals, t):
+    if t == FLOAT64:
+        return np.array([float(v) for v in vals], dtype=np.float64)
+    return np.array([int(v) for v in vals], dtype=object)
+
+
+def evaluate_value(frame, e, rows: np.ndarray, as_type=None):
+    """(values, valid, type) of a value expression on the rows `rows`
+    selects (other rows: invalid). Raises ValueFault for a checked overflow
+    on an evaluated row with valid operands."""
+    n = frame.height()
+    k = e.kind
+    t = as_type or _alone(_vtype(frame, e))
+    if k == "col":
+        c = frame.column(e.args[0]).column
+        vals = c.values.astype(np.float64) if t == FLOAT64 else c.values.astype(object)
+        return vals, c.is_valid() & rows, t
+    if k == "lit":
+        v = e.args[0]
+        if v is None:
+            return np.zeros(n, dtype=np.float64 if t == FLOAT64 else object), np.zeros(n, dtype=bool), t
+        return (np.full(n, float(v)) if t == FLOAT64 else np.full(n, int(v), dtype=object)), rows.copy(), t
+    if k == "arith":
+        op, a, b = e.args
+        ta, tb, node = _pair_types(frame, a, b)
+        if op == "div":
+            node = FLOAT64
+        av, am, _ = evaluate_value(frame, a, rows, ta)
+        bv, bm, _ = evaluate_value(frame, b, rows, tb)
+        ok = am & bm
+        if node == FLOAT64:
+            av, bv = av.astype(np.float64), bv.astype(np.float64)
+            with np.errstate(all="ignore"):
+                r = {"add": np.add, "sub": np.subtract, "mul": np.multiply, "div": np.true_divide}[op](av, bv)
+            return r, ok, node
+        r = np.array([0] * n, dtype=object)
+        lo, hi = _BOUNDS[node]
+        for i in np.nonzero(ok)[0]:
+            x, y = int(av[i]), int(bv[i])
+            z = x + y if op == "add" else (x - y if op == "sub" else x * y)
+            if z < lo or z > hi:
+                raise ValueFault(_WORD[(op, node)])
+            r[i] = z
+        return r, ok, node
+    if k in ("neg", "abs"):
+        xv, xm, _ = evaluate_value(frame, e.args[0], rows, t)
+        if t == FLOAT64:
+            return (-xv if k == "neg" else np.abs(xv)), xm, t
+        r = np.array([0] * n, dtype=object)
+        lo, _ = _BOUNDS[t]
+        for i in np.nonzero(xm)[0]:
+            x = int(xv[i])
+            if x == lo:
+                raise ValueFault(_WORD[(k, t)])
+            r[i] = -x if (k == "neg" or x < 0) else x
+        return r, xm, t
+    if k == "case":
+        cond, then, other = e.args
+        tt, to, node = _pair_types(frame, then, other)
+        cv, cm, _ = evaluate(frame, cond)
+        take = cm & cv.astype(bool)
+        tv, tm, _ = evaluate_value(frame, then, rows & take, tt)
+        ov, om, _ = evaluate_value(frame, other, rows & ~take, to)
+        tv, ov = _as_type(tv, node), _as_type(ov, node)
+        return np.where(take, tv, ov), np.where(take, tm, om) & rows, node
+    raise OracleError(f"oracle cannot evaluate the value {e!r}")
+
+
+def value_column(frame, e, keep: np.ndarray) -> Column:
+    vals, valid, t = evaluate_value(frame, e, keep)
+    if t == FLOAT64:
+        out = np.where(valid, vals.astype(np.float64), 0.0)
+    else:
+        out = np.array([int(v) if ok else 0 for v, ok in zip(vals, valid)], dtype=np.int64)
+        if t == INT32:
+            out = out.astype(np.int32)
+    return Column(t, out, valid)
+
+
 def kept_rows(frame, filters) -> np.ndarray:
     keep = np.ones(frame.height(), dtype=bool)
     for f in filters:
@@ -273,11 +429,19 @@ def collect(plan, threads: int = 1):
     frame = plan.source
     keep = kept_rows(frame, plan.filters)
     specs, names = [], []
+    derived = []
     for e in plan.aggs:
         op, target, kw = e.args
         kw = dict(kw)
-        specs.append((op, None if target is None else target.args[0], int(kw.get("ddof", 1)), int(kw.get("min_count", 0))))
+        name = None if target is None else (target.args[0] if target.kind == "col" else None)
+        if target is not None and target.kind != "col":
+            # an aggregate's value: computed for the kept rows, then grouped
+            name = f"\x00value{len(derived)}"
+            derived.append(Series(name, value_column(frame, target, keep)))
+        specs.append((op, name, int(kw.get("ddof", 1)), int(kw.get("min_count", 0))))
         names.append(e.output_name())
+    if derived:
+        frame = DataFrame([frame.column(c) for c in frame.columns] + derived)
     key_cols, agg_cols, first = groupby(frame, keep, list(plan.keys), specs, plan.maintain_order, threads)
     series = [Series(k, c) for k, c in key_cols] + [Series(nm, c) for nm, c in zip(names, agg_cols)]
     return DataFrame(series), first
