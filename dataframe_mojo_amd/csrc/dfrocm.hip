@@ -363,6 +363,8 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.hot_share = (int)value;
   else if (k == "rec_pack" && (value == 0 || value == 1))
     t.rec_pack = (int)value;
+  else if (k == "fuse_one_level" && (value == 0 || value == 1))
+    t.fuse_one_level = (int)value;
   else if (k == "scatter_chunk" && (value == 8192 || value == 16384))
     t.scatter_chunk = (int)value;
   else if (k == "two_level_parts" && value >= 1 && value <= 4096 && (value & (value - 1)) == 0)
@@ -400,7 +402,7 @@ extern "C" int dfr_context_tune_get(const dfr_context *ctx, const char *key, int
       {"batch_graph", t.batch_graph}, {"batch_part_rows", t.batch_part_rows},
       {"scatter_slices", t.scatter_slices}, {"scatter_stage", t.scatter_stage}, {"hot_keys", t.hot_keys},
       {"hot_rows", t.hot_rows}, {"hot_domain", t.hot_domain}, {"hot_sample", t.hot_sample},
-      {"hot_share", t.hot_share}, {"rec_pack", t.rec_pack}, {"scatter_chunk", t.scatter_chunk},
+      {"hot_share", t.hot_share}, {"rec_pack", t.rec_pack}, {"fuse_one_level", t.fuse_one_level}, {"scatter_chunk", t.scatter_chunk},
       {"two_level_parts", t.two_level_parts}, {"part_block", t.part_block}, {"lds_odd_stride", t.lds_odd_stride},
       {"part_odd_stride", t.part_odd_stride}, {"scan_prefetch", t.scan_prefetch}, {"jit", t.jit},
       {"jit_rows", t.jit_rows}, {"jit_wait", t.jit_wait}};

@@ -157,6 +157,7 @@ Tuning default_tuning(dfr_context *ctx) {
   t.scatter_stage = env_int("DFR_SCATTER_STAGE", 1) ? 1 : 0;
   t.hot_keys = std::max(-1, std::min(1, env_int("DFR_HOT_KEYS", -1)));
   t.rec_pack = env_int("DFR_REC_PACK", 1) ? 1 : 0;
+  t.fuse_one_level = env_int("DFR_FUSE_ONE_LEVEL", 1) ? 1 : 0;
   t.batch_part_rows = env_int("DFR_BATCH_PART_ROWS", 1 << 22);
   t.scan_block = env_int("DFR_SCAN_BLOCK", 1024);
   t.scan_wg_per_cu = env_int("DFR_SCAN_WG_PER_CU", 1);
@@ -1131,6 +1132,10 @@ int launch_hot_reduce(dfr_context *ctx, const Compiled &C, const DevPlan *dplan,
                      groups, 0);
   hipLaunchKernelGGL(k_hot_reduce, dim3((unsigned)((ew + 255) / 256)), dim3(256), 0, ctx->stream, dplan, groups, 1);
   ctx->obs.launches += 2;
+  if (P.fused) {
+    hipLaunchKernelGGL(k_hot_emit, dim3((unsigned)((P.lds_entries + 255) / 256)), dim3(256), 0, ctx->stream, dplan);
+    ctx->obs.launches += 1;
+  }
   HIP_TRY(hipGetLastError());
   return DFR_OK;
 }
@@ -1291,12 +1296,27 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
   DevPlan &P = C.plan;
   int64_t hash_slots = ctx->tune.hash_slots;
   int sub_boost = 0;
+  bool no_fuse = false;  // a fused one-level pass C's table overflowed: run unfused
   for (int attempt = 0;; ++attempt) {
     choose_layout(ctx, C, hash_slots, sub_boost);
     const bool dense = P.two_level != 0;
     // unordered second level: pass C writes the output rows itself, so
     // there are no state or key tables and no finalize pass
     const bool fused = dense && !C.maintain_order;
+    // unordered one level: pass C and the seeded keys' merge put each group
+    // at the next slot of dense key and state tables sized by the most
+    // groups the query can have (its packed key domain, at most its rows):
+    // no table set-up, occupancy scan or compaction, and the finalize
+    // reads the tables in slot order
+    const uint64_t one_cap = std::min<uint64_t>(P.domain, (uint64_t)std::max<int64_t>(C.rows, 0));
+    const bool fused1 = !dense && !C.maintain_order && P.agg_mode == AGG_PARTITION && C.rows > 0 &&
+                        ctx->tune.fuse_one_level && !no_fuse && one_cap > 0 && one_cap <= ((uint64_t)1 << 28) &&
+                        one_cap * (uint64_t)(P.words + 1) * 8 <=
+                            std::min<uint64_t>((uint64_t)1 << 31, (uint64_t)ctx->pool.budget / 8);
+    if (fused1) {
+      P.mode = MODE_HASH;  // (keys from keys_tab: the finalize decodes them there)
+      P.cap = one_cap;
+    }
     P.fused = 0;
     Scratch S(ctx);
     int st;
@@ -1404,10 +1424,12 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
               (st = S.get(kHotList + 1, &C.hot_list, error)))
             return st;
         }
+        // (a fused query merges the seeded states into one more segment)
         if ((st = S.get(1, &C.hot_out, error)) || (st = S.get(E, &P.hot_image, error)) ||
-            (st = S.get(S_ * E * W, &P.hot_states, error)))
+            (st = S.get((S_ + (fused1 ? 1 : 0)) * E * W, &P.hot_states, error)))
           return st;
       }
+      if (fused1 && (st = S.get(1, &P.gcount, error))) return st;
       if (dense) {
         // second level: the spill segments are dead after the first
         // scatter and take the sub-partition order
@@ -1420,14 +1442,16 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
     const uint64_t occ_words = (P.cap + 63) / 64;
     uint64_t *occ = nullptr, *occ_prefix = nullptr, *order = nullptr;
     const uint64_t max_groups = std::min<uint64_t>(P.cap, (uint64_t)std::max<int64_t>(C.rows, 1));
-    if (!dense && ((st = S.get(occ_words, &occ, error)) || (st = S.get(occ_words, &occ_prefix, error)) ||
-                   (st = S.get(max_groups + 1, &order, error))))
+    if (!dense && !fused1 &&
+        ((st = S.get(occ_words, &occ, error)) || (st = S.get(occ_words, &occ_prefix, error)) ||
+         (st = S.get(max_groups + 1, &order, error))))
       return st;
+    if (fused1) P.fused = 1;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     memcpy(ctx->staging, &P, sizeof P);
     HIP_TRY(hipMemcpyAsync(dplan, ctx->staging, sizeof P, hipMemcpyHostToDevice, ctx->stream));
     fill(ctx, (uint64_t *)fault, 2, 0);
-    if (dense) {
+    if (fused1 || dense) {
       fill(ctx, P.gcount, 1, 0);  // dense tables need no set-up: pass C writes every word
     } else {
       int blocks = (int)std::min<uint64_t>((P.cap + kBlock - 1) / kBlock, (uint64_t)ctx->cus * 8);
@@ -1469,7 +1493,7 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
       HIP_TRY(hipEventRecord(ctx->evp[1], ctx->stream));
     }
     uint64_t *total;
-    if (dense) {
+    if (dense || fused1) {
       total = P.gcount;  // slots 0..G-1, in slot order
     } else {
       uint64_t threads = occ_words * 64;
@@ -1492,6 +1516,11 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
     ctx->obs.hot_keys += (int64_t)(uint32_t)hs[3];
     const uint64_t G = hs[0];
     const uint32_t *hf = (const uint32_t *)(hs + 1);
+    if (hf[3] && fused1) {
+      // a partition's keys outgrew its LDS table with no HBM table behind it
+      no_fuse = true;
+      continue;  // (the guard releases this attempt's result)
+    }
     if (hf[3]) {
       // a sub-partition outgrew its LDS table: twice the sub-partitions
       if (sub_boost >= 6 || P.n_sub >= 8192)
@@ -1508,7 +1537,7 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
       continue;
     }
     P.groups = total;
-    P.order = dense ? nullptr : order;
+    P.order = (dense || fused1) ? nullptr : order;
     if (C.maintain_order && G > 0) {
       const uint64_t rwords = ((uint64_t)C.rows + 63) / 64;
       uint64_t *rowbits, *rowprefix, *order2;

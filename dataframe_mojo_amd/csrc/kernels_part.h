@@ -26,6 +26,10 @@ namespace dfr {
 constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
 constexpr int kMaxPartEntries = 16384;  // entries of a pass C LDS table, at most
 
+// A group's output row (keys decoded, aggregates finished), defined below.
+template <int NA, class PV>
+__device__ __forceinline__ void emit_group(const PV &P, uint64_t i, const uint64_t *slot, uint64_t packed);
+
 __device__ __forceinline__ uint32_t key32(uint64_t key) { return (uint32_t)key ^ (uint32_t)(key >> 32); }
 
 template <class PV>
@@ -738,11 +742,38 @@ __global__ void k_hot_reduce(const DevPlan *__restrict__ plan, int groups, int s
     for (int i = 0; i < n; ++i) d[i] = acc[i];
     return;
   }
+  if (P.fused) {
+    // the merged state, whole, into the spare segment NS (k_hot_emit)
+    DFR_GLOBAL uint64_t *d = gpw(P.hot_states) + (uint64_t)NS * ew + (uint64_t)e * W + w;
+    for (int i = 0; i < n; ++i) d[i] = acc[i];
+    return;
+  }
   if (rows == 0) return;  // no kept row of the key (a group only exists with one)
   const uint64_t slot = global_slot(P, key);
   if (slot == kEmpty) return;
   uint64_t *d = P.state + slot * (uint64_t)W + w;
   for (int i = 0; i < n; ++i) d[i] = acc[i];
+}
+
+// A fused query's seeded keys with kept rows (their merged states in
+// hot_states segment n_segs) at consecutive dense slots: one reservation
+// per wave.
+__global__ void k_hot_emit(const DevPlan *__restrict__ plan) {
+  PlanRef P = plan_ref(plan);
+  const int W = P.words, E = P.lds_entries;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t *st = P.hot_states + ((uint64_t)P.n_segs * E + (uint64_t)(e < E ? e : 0)) * W;
+  const uint64_t key = e < E ? P.hot_image[e] : kEmpty;
+  const bool live = key != kEmpty && st[0] != 0;  // word 0: the group's kept rows
+  const uint64_t ball = __ballot(live);
+  const int lane = threadIdx.x & 63;
+  uint64_t base = 0;
+  if (lane == 0 && ball) base = atomicAdd((unsigned long long *)P.gcount, (unsigned long long)__popcll(ball));
+  base = __shfl(base, 0, 64);
+  if (!live) return;
+  const uint64_t slot = base + (uint64_t)__popcll(ball & ((1ull << lane) - 1ull));
+  P.keys_tab[slot] = key;
+  for (int w = 0; w < W; ++w) P.state[slot * (uint64_t)W + w] = st[w];
 }
 
 // Pass B, step 1: per partition p, exclusive scan over segments of the
@@ -1194,7 +1225,8 @@ __device__ __forceinline__ void fold_recs(const PV &P, uint64_t *lkeys, uint64_t
     const int e = part_find(P, lkeys, key, placed);
     if (e >= 0) {
       fold_full(P, lst + (int64_t)e * P.c_stride, row, val, ok);
-    } else if (DENSE) {
+    } else if (DENSE || P.fused) {
+      // no HBM table to take the key: the host runs the query again
       atomicOr(&P.fault[3], 1u);
     } else {
       const uint64_t g = global_slot(P, key);
@@ -1225,7 +1257,7 @@ __device__ __forceinline__ void merge_one(const PV &P, uint64_t *lkeys, uint64_t
   const int e = part_find(P, lkeys, key, placed);
   if (e >= 0) {
     merge_full(P, lst + (int64_t)e * P.c_stride, s);
-  } else if (DENSE) {
+  } else if (DENSE || P.fused) {
     atomicOr(&P.fault[3], 1u);
   } else {
     const uint64_t g = global_slot(P, key);
@@ -1256,9 +1288,70 @@ __device__ __forceinline__ void merge_partials(const PV &P, uint64_t *lkeys, uin
 }
 
 // Pass C: partition p.
+// A fused pass C's groups (the occupied entries of its LDS table) placed at
+// consecutive slots of the dense key and state tables: each 64-entry
+// chunk's occupied count, a prefix over the chunks (wave 0), one
+// reservation of the partition's total, then each group's key and words
+// (plain stores; k_finalize finishes the tables in slot order). Finishing
+// the groups here instead would put the finish code into this kernel,
+// whose register copy of the plan then no longer stays in registers.
+template <class PV>
+__device__ __forceinline__ void place_table(const PV &P, const uint64_t *lkeys, const uint64_t *lst, int E, int S,
+                                            uint32_t *wsum, uint64_t &gbase) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int nchunks = (E + 63) >> 6;
+  const int W = P.words;
+  for (int c = wave; c < nchunks; c += nwaves) {
+    const int e = c * 64 + lane;
+    const uint64_t ball = __ballot(e < E && lkeys[e] != kEmpty);
+    if (lane == 0) wsum[c] = (uint32_t)__popcll(ball);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const int per = (nchunks + 63) >> 6;
+    uint32_t own = 0;
+    for (int j = 0; j < per; ++j) {
+      const int c = lane * per + j;
+      if (c < nchunks) own += wsum[c];
+    }
+    uint32_t inc = own;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += y;
+    }
+    uint32_t run = inc - own;
+    for (int j = 0; j < per; ++j) {
+      const int c = lane * per + j;
+      if (c < nchunks) {
+        const uint32_t x = wsum[c];
+        wsum[c] = run;
+        run += x;
+      }
+    }
+    const uint32_t total = __shfl(inc, 63, 64);
+    if (lane == 0) gbase = total ? atomicAdd((unsigned long long *)P.gcount, (unsigned long long)total) : 0;
+  }
+  __syncthreads();
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int c = wave; c < nchunks; c += nwaves) {
+    const int e = c * 64 + lane;
+    const uint64_t key = e < E ? lkeys[e] : kEmpty;
+    const uint64_t ball = __ballot(key != kEmpty);
+    if (key == kEmpty) continue;
+    const uint64_t slot = gbase + wsum[c] + (uint64_t)__popcll(ball & lt);
+    P.keys_tab[slot] = key;
+    uint64_t *d = P.state + slot * (uint64_t)W;
+    const uint64_t *src = lst + (int64_t)e * S;
+    for (int w = 0; w < W; ++w) d[w] = src[w];
+  }
+}
+
 template <class PV>
 __device__ __forceinline__ void part_aggregate_body(const PV &P) {
   extern __shared__ uint64_t smem[];
+  __shared__ uint32_t wsum[kMaxPartEntries / 64];
+  __shared__ uint64_t gbase;
   const int p = blockIdx.x, E = P.c_entries, W = P.words, S = P.c_stride;
   uint64_t *lkeys = smem;
   uint64_t *lst = smem + E;
@@ -1284,6 +1377,11 @@ __device__ __forceinline__ void part_aggregate_body(const PV &P) {
     part_fold_records<kMaxCols + 3, 1, false>(P, lkeys, lst, keymask, recs, b0, b1);
   merge_partials<false>(P, lkeys, lst, gp(P.part_par), P.base_par[p], P.base_par[p + 1]);
   __syncthreads();
+  if (P.fused) {
+    // unordered: the partition's groups go to consecutive dense slots
+    place_table(P, lkeys, lst, E, S, wsum, gbase);
+    return;
+  }
   // every key of this partition lives only here: plain stores
   for (int e = threadIdx.x; e < E; e += blockDim.x) {
     const uint64_t key = lkeys[e];

@@ -68,6 +68,7 @@ struct Tuning {
   int hot_sample = 1 << 19;    // rows sampled
   int hot_share = 30;          // percent of the sampled kept rows the seeded keys must carry
   int rec_pack = 1;            // 1: narrow integer values of spilled records ride in the key word
+  int fuse_one_level = 1;      // 1: an unordered one-level partitioned query's pass C fills dense slots
 };
 
 }  // namespace dfr

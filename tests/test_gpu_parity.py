@@ -176,6 +176,23 @@ def test_random_queries(seed, fi, ki):
         both(lf, exact_float=("x_min", "x_max"))
 
 
+class tuned:
+    """Tune settings for a block; the context's previous values (built-in or
+    from a calibration profile) come back afterwards."""
+
+    def __init__(self, ctx, setting):
+        self.ctx, self.setting = ctx, setting
+
+    def __enter__(self):
+        self.saved = {k: self.ctx.tune_get(k) for k in self.setting}
+        for k, v in self.setting.items():
+            self.ctx.tune(k, v)
+
+    def __exit__(self, *exc):
+        for k, v in self.saved.items():
+            self.ctx.tune(k, v)
+
+
 TUNE_DEFAULTS = {"lds_mode": -1, "lds_bytes": 65536, "direct_max_slots": 1 << 26, "hash_slots": 0, "block": 512,
                  "wg_per_cu": 2, "lds_probe": 4, "scan_unroll": 1, "agg_mode": 1, "two_level": -1,
                  "max_parts": 4096, "part_lds_bytes": 131072, "sub_lds_bytes": 49152, "sub_block": 512, "jit": 1,
@@ -226,13 +243,8 @@ def test_table_modes_give_identical_results(setting, gpu_ctx):
     df = random_frame(30000, 5, key_card=400)
     lf = df.filter(col("a") > -500000).group_by("k", "s", maintain_order=True).agg(ALL_AGGS)
     base = lf.collect()
-    try:
-        for k, v in setting.items():
-            gpu_ctx.tune(k, v)
+    with tuned(gpu_ctx, setting):
         got = lf.collect()
-    finally:
-        for k, v in TUNE_DEFAULTS.items():
-            gpu_ctx.tune(k, v)
     # bitwise identical: every state is order independent
     for name in base.columns:
         a, b = base.column(name).column, got.column(name).column
@@ -688,3 +700,41 @@ def test_narrow_values_ride_in_the_key_word(two_level, mo, gpu_ctx):
     frames_equal_bitwise(got[0], got[1], 1)
     exp, _ = oracle.collect(lf)
     assert_same(got[1], exp, mo, (), 1)
+
+
+def _sorted_rows(frame, n_keys):
+    """Every column's (values bits, validity) with rows in key order."""
+    cols = [frame.column(c).column for c in frame.columns]
+    keys = []
+    for c in reversed(cols[:n_keys]):
+        raw = c.values.view(np.int64) if c.values.dtype.kind == "f" else c.values.astype(np.int64)
+        keys.append(np.where(c.is_valid(), raw, np.iinfo(np.int64).max))
+        keys.append(~c.is_valid())
+    order = np.lexsort(keys)
+    return [(name, c.values[order].view(np.uint8), c.is_valid()[order]) for name, c in zip(frame.columns, cols)]
+
+
+@pytest.mark.parametrize("setting", [
+    {}, {"jit": 2}, {"hot_keys": 1, "lds_mode": 2}, {"hot_keys": 1, "lds_mode": 2, "jit": 2},
+    {"rec_pack": 0}, {"part_block": 256}, {"lds_mode": 0},
+    # one partition whose LDS table cannot hold the groups: the dense pass C
+    # raises its capacity flag and the query runs again through the table
+    {"max_parts": 1, "part_lds_bytes": 4096},
+])
+def test_unordered_one_level_dense_slots(setting, gpu_ctx):
+    """An unordered one-level query whose pass C (and the seeded keys'
+    merge) puts each group at the next dense slot equals the same query
+    through the key-addressed table, occupancy scan and compaction, bit for
+    bit, and the oracle."""
+    df = random_frame(40000, 9, key_card=3000)
+    lf = df.filter(col("a") > -700000).group_by("k", "k32", maintain_order=False).agg(ALL_AGGS)
+    with tuned(gpu_ctx, dict(setting, fuse_one_level=0)):
+        unfused = lf.collect()
+    with tuned(gpu_ctx, dict(setting, fuse_one_level=1)):
+        fused = lf.collect()
+    assert fused.height() == unfused.height()
+    for (name, a, ma), (_, b, mb) in zip(_sorted_rows(fused, 2), _sorted_rows(unfused, 2)):
+        assert np.array_equal(ma, mb), name
+        assert np.array_equal(a.reshape(len(ma), -1)[ma], b.reshape(len(mb), -1)[mb]), name
+    exp, _ = oracle.collect(lf)
+    assert_same(fused, exp, False, ("x_min", "x_max"), 2)
