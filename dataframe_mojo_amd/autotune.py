@@ -121,23 +121,36 @@ class Workload:
         raise NotImplementedError
 
 
+_MIX = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix(x: np.ndarray) -> np.ndarray:
+    """SplitMix64's finaliser over uint64 (wrapping)."""
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def _bits(vals: np.ndarray, valid: np.ndarray) -> np.ndarray:
+    b = vals.view(np.uint32 if vals.itemsize == 4 else np.uint64).astype(np.uint64)
+    return np.where(valid, b, np.uint64(0xFFFFFFFFFFFFFFFF))
+
+
 def _digest_result(res, n_keys: int, n_aggs: int) -> str:
-    """Group count, keys and every aggregate's bits, rows in key order."""
-    h = hashlib.sha256(str(res.groups).encode())
-    keys = [res.key(k) for k in range(n_keys)]
-    sort = []
-    for _, kv, km in reversed(keys):
-        sort.append(kv.view(np.uint32 if kv.itemsize == 4 else np.uint64))
-        sort.append(~km)
-    order = np.lexsort(sort) if sort else np.arange(res.groups)
-    for _, kv, km in keys:
-        h.update(kv[order].tobytes())
-        h.update(km[order].tobytes())
-    for a in range(n_aggs):
-        _, v, m = res.agg(a)
-        h.update(np.where(m[order], v[order].view(np.uint8).reshape(len(order), -1).T, 0).tobytes())
-        h.update(m[order].tobytes())
-    return h.hexdigest()
+    """Group count, keys and every aggregate's bits, independent of row
+    order and coupled by row: each row's hash of its key words is mixed
+    with each of its values, and the wrapping sums over rows are hashed
+    (a value moved to another group's row changes the digest)."""
+    with np.errstate(over="ignore"):
+        row = np.zeros(res.groups, dtype=np.uint64)
+        for k in range(n_keys):
+            _, kv, km = res.key(k)
+            row = _mix(row * _MIX + _bits(kv, km))
+        sums = [res.groups, int(row.sum(dtype=np.uint64))]
+        for a in range(n_aggs):
+            _, v, m = res.agg(a)
+            sums.append(int(_mix(row + _bits(v, m) * _MIX + np.uint64(a)).sum(dtype=np.uint64)))
+    return hashlib.sha256(repr(sums).encode()).hexdigest()
 
 
 class OneQuery(Workload):
@@ -367,7 +380,7 @@ def groupby_costs(ctx, reps: int = 3, rows_list=(100_000, 1_000_000, 10_000_000,
 
 
 def calibrate(device: int = 0, classes=("one_level", "two_level", "batch"), reps: int = 3,
-              min_gain: float = 0.02, quick: bool = False) -> dict:
+              min_gain: float = 0.02, quick: bool = False, costs: bool = True) -> dict:
     ident = identity(device)
     log(f"identity {ident}")
     os.environ["DATAFRAME_ROCM_CALIBRATION"] = "off"  # tune from the built-in settings
@@ -387,19 +400,35 @@ def calibrate(device: int = 0, classes=("one_level", "two_level", "batch"), reps
         tuned[cls] = prov
         del work
         ctx.trim()
-    log("placement costs")
     prof = {
         "version": VERSION,
         "identity": ident,
         "tuning": tuning,
         "tuned": tuned,
-        "transfer": dict(transfer_costs(), startup_ns=startup_ns(device)),
-        "groupby": groupby_costs(ctx, rows_list=(100_000, 1_000_000, 10_000_000) if quick else
-                                 (100_000, 1_000_000, 10_000_000, 100_000_000)),
         "measured": time.strftime("%Y-%m-%dT%H:%M:%SZ", time.gmtime()),
     }
+    if costs:
+        log("placement costs")
+        prof["transfer"] = dict(transfer_costs(), startup_ns=startup_ns(device))
+        prof["groupby"] = groupby_costs(ctx, rows_list=(100_000, 1_000_000, 10_000_000) if quick else
+                                        (100_000, 1_000_000, 10_000_000, 100_000_000))
     ctx.close()
     return prof
+
+
+def merge_profiles(base: dict, new: dict) -> dict:
+    """`new` over `base`, both measured with the same identity: tuning and
+    provenance joined (new wins), costs taken from whichever has them."""
+    if base.get("identity") != new.get("identity") or base.get("version") != new.get("version"):
+        raise ValueError("profiles of different identities do not merge")
+    out = dict(base)
+    out.update({k: v for k, v in new.items() if k not in ("tuning", "tuned")})
+    out["tuning"] = {**base.get("tuning", {}), **new.get("tuning", {})}
+    out["tuned"] = {**base.get("tuned", {}), **new.get("tuned", {})}
+    for k in ("transfer", "groupby"):
+        if k not in new and k in base:
+            out[k] = base[k]
+    return out
 
 
 def main(argv=None) -> int:
@@ -410,9 +439,14 @@ def main(argv=None) -> int:
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--min-gain", type=float, default=0.02)
     ap.add_argument("--quick", action="store_true", help="smaller workloads (a smoke run, not a calibration)")
+    ap.add_argument("--no-costs", action="store_true", help="tune only (no placement cost capture)")
+    ap.add_argument("--merge", default="", help="profiles (comma separated) whose tuning and costs this run's extend")
     args = ap.parse_args(argv)
     prof = calibrate(args.device, tuple(c for c in args.classes.split(",") if c), args.reps, args.min_gain,
-                     args.quick)
+                     args.quick, not args.no_costs)
+    for path in [p for p in args.merge.split(",") if p]:
+        with open(path) as f:
+            prof = merge_profiles(json.load(f), prof)
     out = args.out or os.path.join(os.path.dirname(os.path.abspath(__file__)), "calibration",
                                    profile_name(prof["identity"]))
     os.makedirs(os.path.dirname(out) or ".", exist_ok=True)

@@ -121,3 +121,37 @@ def test_library_applies_only_a_matching_profile(tmp_path, monkeypatch):
         assert ctx.calibration() == "none: DATAFRAME_ROCM_CALIBRATION=off"
     finally:
         ctx.close()
+
+
+class _FakeResult:
+    def __init__(self, keys, aggs):
+        self.keys, self.aggs = keys, aggs
+        self.groups = len(keys[0][0])
+
+    def key(self, k):
+        return None, *self.keys[k]
+
+    def agg(self, a):
+        return None, *self.aggs[a]
+
+
+def test_result_digest_is_order_independent_and_row_coupled():
+    rng = np.random.default_rng(3)
+    n = 1000
+    k = rng.integers(0, 10**6, n).astype(np.int64)
+    v = rng.normal(size=n)
+    valid = rng.random(n) > 0.1
+    ones = np.ones(n, dtype=bool)
+    base = autotune._digest_result(_FakeResult([(k, ones)], [(v, valid)]), 1, 1)
+    p = rng.permutation(n)
+    assert autotune._digest_result(_FakeResult([(k[p], ones)], [(v[p], valid[p])]), 1, 1) == base
+    # a value moved to another group's row, one bit flipped, one validity changed
+    v2 = v.copy()
+    v2[[0, 1]] = v2[[1, 0]]
+    assert autotune._digest_result(_FakeResult([(k, ones)], [(v2, valid)]), 1, 1) != base
+    v3 = v.copy()
+    v3[5] = np.nextafter(v3[5], 10.0)
+    assert autotune._digest_result(_FakeResult([(k, ones)], [(v3, valid)]), 1, 1) != base
+    m2 = valid.copy()
+    m2[7] = not m2[7]
+    assert autotune._digest_result(_FakeResult([(k, ones)], [(v, m2)]), 1, 1) != base
