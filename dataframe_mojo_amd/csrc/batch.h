@@ -33,20 +33,44 @@ __device__ __forceinline__ int64_t uniform_i64(int64_t x) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Block q sets query q's table to its identity words (and empty keys).
-__global__ void k_batch_init(const DevPlan *__restrict__ plans) {
-  PlanRef P = plan_ref(plans + blockIdx.x);
+// The per-table kernels (init, occupancy, compaction) give each query
+// blocks of kSlotSpan table slots: query q owns blocks [boff[q], boff[q + 1])
+// (at least one), so a table of 1e5 slots is not left to one workgroup.
+constexpr uint64_t kSlotSpan = 8 * kBlock;  // a multiple of 64 and of kBlock
+
+// The query of this block (the last q with boff[q] <= blockIdx.x) and the
+// first slot of its span.
+__device__ __forceinline__ int span_query(const uint64_t *__restrict__ boff, int n, uint64_t *s0) {
+  const uint64_t b = blockIdx.x;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (boff[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  *s0 = (b - boff[lo]) * kSlotSpan;
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// A span of a query's table set to its identity words (and empty keys).
+__global__ void k_batch_init(const DevPlan *__restrict__ plans, const uint64_t *__restrict__ boff, int n) {
+  uint64_t s0;
+  const int q = span_query(boff, n, &s0);
+  PlanRef P = plan_ref(plans + q);
+  if (s0 >= P.cap) return;
+  const uint64_t s1 = min(P.cap, s0 + kSlotSpan);
   const uint32_t W = (uint32_t)P.words;
-  const uint64_t total = P.cap * W;
+  const uint64_t total = (s1 - s0) * W;
+  uint64_t *base = P.state + s0 * W;  // slot-aligned: word j of the span is word j % W of its slot
   uint32_t w = threadIdx.x % W;
   const uint32_t stride_w = blockDim.x % W;
   for (uint64_t j = threadIdx.x; j < total; j += blockDim.x) {
-    P.state[j] = P.ident[w];
+    base[j] = P.ident[w];
     w += stride_w;
     if (w >= W) w -= W;
   }
   if (P.mode == MODE_HASH)
-    for (uint64_t j = threadIdx.x; j < P.cap; j += blockDim.x) P.keys_tab[j] = kEmpty;
+    for (uint64_t j = s0 + threadIdx.x; j < s1; j += blockDim.x) P.keys_tab[j] = kEmpty;
 }
 
 // Persistent workgroups: each takes the next work item until the queue is
@@ -74,17 +98,20 @@ __global__ void __launch_bounds__(512) k_batch_aggregate(const DevPlan *__restri
   }
 }
 
-// Block q: occupancy bits of query q's slots, at word woff[q].
+// A span's occupancy bits, at word woff[q] of query q's bit array onward.
 __global__ void k_batch_occupancy(const DevPlan *__restrict__ plans, const uint64_t *__restrict__ woff,
-                                  uint64_t *bits) {
-  PlanRef P = plan_ref(plans + blockIdx.x);
+                                  const uint64_t *__restrict__ boff, int n, uint64_t *bits) {
+  uint64_t s0;
+  const int q = span_query(boff, n, &s0);
+  PlanRef P = plan_ref(plans + q);
   const uint64_t nw = (P.cap + 63) / 64;
-  uint64_t *out = bits + woff[blockIdx.x];
-  for (uint64_t s0 = 0; s0 < nw * 64; s0 += blockDim.x) {
-    const uint64_t s = s0 + threadIdx.x;
+  const uint64_t w_end = min(nw, (s0 + kSlotSpan) >> 6);  // this span's words end here
+  uint64_t *out = bits + woff[q];
+  for (uint64_t sb = s0; (sb >> 6) < w_end; sb += blockDim.x) {
+    const uint64_t s = sb + threadIdx.x;
     const bool occ = s < P.cap && P.state[s * (uint64_t)P.words] != 0;
     const uint64_t ball = __ballot(occ);
-    if ((threadIdx.x & 63) == 0 && (s >> 6) < nw) out[s >> 6] = ball;
+    if ((threadIdx.x & 63) == 0 && (s >> 6) < w_end) out[s >> 6] = ball;
   }
 }
 
@@ -100,14 +127,17 @@ __global__ void k_batch_counts(const uint64_t *__restrict__ woff, const uint64_t
   pad_off[q] = (r1 - r0 + 63) / 64 * 64;
 }
 
-// Block q: order[global rank] = local slot of each occupied slot; query
+// order[global rank] = local slot of each occupied slot of a span; query
 // q's groups take ranks gbase[q] .. gbase[q] + groups[q] - 1.
 __global__ void k_batch_compact(const DevPlan *__restrict__ plans, const uint64_t *__restrict__ woff,
-                                const uint64_t *__restrict__ bits, const uint64_t *__restrict__ prefix,
-                                uint64_t *order) {
-  PlanRef P = plan_ref(plans + blockIdx.x);
-  const uint64_t w0 = woff[blockIdx.x];
-  for (uint64_t s = threadIdx.x; s < P.cap; s += blockDim.x) {
+                                const uint64_t *__restrict__ boff, int n, const uint64_t *__restrict__ bits,
+                                const uint64_t *__restrict__ prefix, uint64_t *order) {
+  uint64_t s0;
+  const int q = span_query(boff, n, &s0);
+  PlanRef P = plan_ref(plans + q);
+  const uint64_t w0 = woff[q];
+  const uint64_t s1 = min(P.cap, s0 + kSlotSpan);
+  for (uint64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
     const uint64_t w = bits[w0 + (s >> 6)];
     if (!((w >> (s & 63)) & 1)) continue;
     const uint64_t below = (s & 63) ? (w & ((1ull << (s & 63)) - 1)) : 0;
@@ -277,12 +307,40 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
     if (!parted[j]) lds_max = std::max(lds_max, c.lds_bytes);
   }
   woff[m] = occ_words;
+  // blocks of the per-table kernels, kSlotSpan slots each (woff and boff
+  // travel as one array)
+  woff.resize(2 * ((size_t)m + 1));
+  uint64_t *boff = woff.data() + m + 1;
+  uint64_t span_blocks = 0;
+  for (int j = 0; j < m; ++j) {
+    boff[j] = span_blocks;
+    span_blocks += std::max<uint64_t>(1, (C[live[j]].plan.cap + kSlotSpan - 1) / kSlotSpan);
+  }
+  boff[m] = span_blocks;
   const int block = std::min(512, ctx->tune.block);  // k_batch_aggregate's launch bound
-  const int64_t tile = std::max<int64_t>(2 * block, ctx->tune.batch_tile / (2 * block) * (2 * block));
+  const int grid = ctx->cus * std::max(1, ctx->tune.wg_per_cu);
+  // Work items are at most batch_tile rows. A batch of few rows is cut
+  // finer, so that its items still give every workgroup of the grid
+  // batch_fill of them (at batch_tile, a 1e6-row query alone would keep 8
+  // workgroups busy). A query's item never gets shorter than its LDS
+  // table: each item clears the table and merges its occupied entries into
+  // HBM once. More items per workgroup lose: every item's merge lands on
+  // the same few HBM entries of a low-cardinality query (a 1.8e6-row query
+  // of 21 groups: 0.21 ms at one item per workgroup, 0.35 ms at two).
+  const int64_t unit = 2 * (int64_t)block;
+  const int64_t tile_max = std::max<int64_t>(unit, ctx->tune.batch_tile / unit * unit);
+  int64_t rows_all = 0;
+  for (int j = 0; j < m; ++j)
+    if (!parted[j]) rows_all += C[live[j]].rows;
+  const int64_t want = (int64_t)ctx->tune.batch_fill * grid;
+  const int64_t tile_fill = want > 0 ? std::min(tile_max, std::max(unit, rows_all / want / unit * unit)) : tile_max;
   std::vector<WorkItem> items;
   for (int j = 0; j < m; ++j) {
     if (parted[j]) continue;
+    const DevPlan &P = C[live[j]].plan;
     const int64_t rows = C[live[j]].rows;
+    const int64_t table = P.lds_mode == LDS_NONE ? 0 : (P.lds_entries + unit - 1) / unit * unit;
+    const int64_t tile = std::min(tile_max, std::max(tile_fill, table));
     for (int64_t b = 0; b < rows; b += tile) items.push_back({j, 0, b, std::min(rows, b + tile)});
   }
   // larger items first: the queue's tail is made of the smallest
@@ -300,12 +358,13 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
   const uint64_t n_items = items.size();
   if ((st = S.get(m, &dplans, error)) || (st = S.get(std::max<uint64_t>(n_items, 1), &ditems, error)) ||
       (st = S.get(std::max<uint64_t>(state_words, 1), &state, error)) ||
-      (key_slots && (st = S.get(key_slots, &keys, error))) || (st = S.get(m + 1, &dwoff, error)) ||
+      (key_slots && (st = S.get(key_slots, &keys, error))) || (st = S.get(2 * ((uint64_t)m + 1), &dwoff, error)) ||
       (st = S.get(occ_words + 1, &occ, error)) || (st = S.get(occ_words + 1, &prefix, error)) ||
       (st = S.get(m, &counts, error)) || (st = S.get(m, &gbase, error)) || (st = S.get(m + 1, &pad_off, error)) ||
       (st = S.get(std::max<uint64_t>(pad_total, 1), &order, error)) || (st = S.get(1, &next, error)) ||
       (st = S.get(4 * (uint64_t)m, &faults, error)) || (st = S.get(scan_nb + 1, &sums, error)))
     return st;
+  const uint64_t *dboff = dwoff + m + 1;
   char *partblock = nullptr;
   if (part_bytes && (st = S.get(part_bytes, &partblock, error))) return st;
   if ((st = ctx->pool.alloc(std::max<uint64_t>(out_bytes, 256), &outblock, error))) return st;
@@ -394,7 +453,7 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
 
   // one pinned upload: plans, items, word offsets
   const size_t plan_bytes = sizeof(DevPlan) * (size_t)m, item_bytes = sizeof(WorkItem) * n_items,
-               woff_bytes = 8 * (size_t)(m + 1);
+               woff_bytes = 16 * (size_t)(m + 1);
   const size_t up_bytes = plan_bytes + item_bytes + woff_bytes;
   const size_t down_bytes = 8 * (size_t)m + 16 * (size_t)m;
   if ((st = ensure_batch_staging(ctx, std::max(up_bytes, down_bytes), error))) {
@@ -427,10 +486,9 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
     fill(ctx, next, 1, 0);
     fill(ctx, (uint64_t *)faults, 2 * (uint64_t)m, 0);
     fill(ctx, occ + occ_words, 1, 0);
-    hipLaunchKernelGGL(k_batch_init, dim3(m), dim3(kBlock), 0, ctx->stream, dplans);
+    hipLaunchKernelGGL(k_batch_init, dim3((unsigned)span_blocks), dim3(kBlock), 0, ctx->stream, dplans, dboff, m);
     if (!graph) hipEventRecord(ctx->evp[0], ctx->stream);
     if (n_items) {
-      const int grid = ctx->cus * std::max(1, ctx->tune.wg_per_cu);
       hipLaunchKernelGGL(kagg, dim3(grid), dim3(block), lds_max, ctx->stream, dplans,
                          ditems, (int64_t)n_items, (unsigned long long *)next);
     }
@@ -443,7 +501,8 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
       }
     }
     if (!graph) hipEventRecord(ctx->evp[1], ctx->stream);
-    hipLaunchKernelGGL(k_batch_occupancy, dim3(m), dim3(kBlock), 0, ctx->stream, dplans, dwoff, occ);
+    hipLaunchKernelGGL(k_batch_occupancy, dim3((unsigned)span_blocks), dim3(kBlock), 0, ctx->stream, dplans, dwoff,
+                       dboff, m, occ);
     hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)scan_nb), dim3(kBlock), 0, ctx->stream, occ, scan_n, sums);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, ctx->stream, sums, scan_nb);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)scan_nb), dim3(kBlock), 0, ctx->stream, occ, scan_n, sums,
@@ -451,7 +510,8 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
     hipLaunchKernelGGL(k_batch_counts, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, dwoff,
                        prefix, m, counts, gbase, pad_off);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, ctx->stream, pad_off, (uint64_t)m);
-    hipLaunchKernelGGL(k_batch_compact, dim3(m), dim3(kBlock), 0, ctx->stream, dplans, dwoff, occ, prefix, order);
+    hipLaunchKernelGGL(k_batch_compact, dim3((unsigned)span_blocks), dim3(kBlock), 0, ctx->stream, dplans, dwoff,
+                       dboff, m, occ, prefix, order);
     if (!graph) hipEventRecord(ctx->evp[2], ctx->stream);
     if (pad_total)
       hipLaunchKernelGGL(k_batch_finalize, dim3((unsigned)((pad_total + kBlock - 1) / kBlock)), dim3(kBlock), 0,

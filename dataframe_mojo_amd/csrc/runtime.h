@@ -48,6 +48,7 @@ struct Tuning {
   int sub_wg_per_cu = -1;      // its workgroups per CU (-1: as many as fit at once; 0: one per sub-partition)
   int two_level = -1;          // second partition level: -1 when groups may outgrow one level, 0 never, 1 always
   int batch_tile = 131072;     // rows of a batch work item
+  int batch_fill = 1;          // a batch of few rows is cut into this many items per workgroup (0: never)
   int batch_graph = 0;         // 1: a batch's launch sequence runs as one captured HIP graph
   int64_t batch_part_rows = 1 << 22;  // a batch query of this many rows whose groups outgrow the batch
                                       // kernel's LDS table takes the partitioned passes (0: never)

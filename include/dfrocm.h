@@ -150,7 +150,10 @@ int dfr_context_calibration(const dfr_context *ctx, char **text, char **error);
  * sub-partitions and runs the second level again), "sub_wg_per_cu"
  * (those workgroups per CU, each looping over sub-partitions: -1 as many as
  * fit on a CU at once, 0 one workgroup per sub-partition), "batch_tile"
- * (rows of a batch work item), "batch_graph" (1: a batch's launch sequence
+ * (rows of a batch work item, at most), "batch_fill" (a batch of fewer rows
+ * than batch_tile x this x the grid's workgroups is cut into shorter items,
+ * this many per workgroup, none shorter than a query's LDS table; 0: items of
+ * batch_tile rows only), "batch_graph" (1: a batch's launch sequence
  * is captured and run as one HIP graph), "batch_part_rows" (a batch query
  * with at least this many rows whose key domain outgrows the batch kernel's
  * LDS table runs the partitioned passes instead, after the batch kernel;

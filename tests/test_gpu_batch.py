@@ -73,6 +73,7 @@ def frames_equal_bitwise(a, b, n_keys):
 
 
 @pytest.mark.parametrize("setting", [{}, {"batch_graph": 1}, {"batch_tile": 2048}, {"block": 256, "lds_bytes": 16384},
+                                     {"batch_fill": 0}, {"batch_fill": 8, "batch_graph": 1},
                                      {"batch_part_rows": 0}, {"batch_part_rows": 1000},
                                      {"batch_part_rows": 1000, "block": 256, "lds_bytes": 16384}])
 def test_batch_equals_one_launch_per_query(setting, gpu_ctx):
@@ -82,7 +83,8 @@ def test_batch_equals_one_launch_per_query(setting, gpu_ctx):
             gpu_ctx.tune(k, v)
         batch = gpu_ctx.groupby_batch(jobs)
     finally:
-        for k, v in {**TUNE_DEFAULTS, "batch_graph": 0, "batch_tile": 131072, "batch_part_rows": 1 << 22}.items():
+        for k, v in {**TUNE_DEFAULTS, "batch_graph": 0, "batch_tile": 131072, "batch_fill": 1,
+                     "batch_part_rows": 1 << 22}.items():
             gpu_ctx.tune(k, v)
     for (plan, keys, cols), got in zip(jobs, batch):
         assert not isinstance(got, Exception), got
@@ -153,3 +155,30 @@ def test_empty_batch_and_empty_queries(gpu_ctx):
     dev = {name: gpu_ctx.upload(df.column(name).column) for name in {*plan.keys, *plan.columns}}
     (r,) = gpu_ctx.groupby_batch([(plan, [dev[k] for k in plan.keys], [dev[c] for c in plan.columns])])
     assert r.groups == 0
+
+
+@pytest.mark.parametrize("sizes", [[(300_000, 50)], [(300_000, 200_000)], [(1_000, 80_000), (20_000, 150_000)],
+                                   [(1_000_000, 3_000), (500, 1), (2_000, 120_000), (150_000, 40)]])
+def test_few_rows_cut_fine_and_wide_tables_spread(sizes, gpu_ctx):
+    # a batch of few rows is cut into items of fewer rows than batch_tile
+    # (down to one per workgroup of the grid, never below a query's LDS
+    # table); tables of up to 2e5 groups are set up, scanned and compacted
+    # in spans of 2048 slots, several workgroups per query
+    jobs, lfs = [], []
+    for i, (n, card) in enumerate(sizes):
+        df = frame(n, 77 + i, card)
+        lf = df.filter(col("v") > -900_000).group_by("k").agg(AGG_SETS[i % len(AGG_SETS)])
+        plan = compile_plan(lf)
+        dev = {name: gpu_ctx.upload(df.column(name).column) for name in {*plan.keys, *plan.columns}}
+        jobs.append((plan, [dev[k] for k in plan.keys], [dev[c] for c in plan.columns]))
+        lfs.append(lf)
+    for (plan, keys, cols), got, lf in zip(jobs, gpu_ctx.groupby_batch(jobs), lfs):
+        assert not isinstance(got, Exception), got
+        one = gpu_ctx.groupby(plan, keys, cols)
+        frames_equal_bitwise(result_frame(got, plan, keys), result_frame(one, plan, keys), len(plan.keys))
+        got.release()
+        one.release()
+    plan, keys, cols = jobs[-1]
+    exp, _ = oracle.collect(lfs[-1])
+    (got,) = gpu_ctx.groupby_batch([jobs[-1]])
+    assert_same(result_frame(got, plan, keys), exp, False, ("x_min",), len(plan.keys))
