@@ -181,6 +181,20 @@ def bench_batch(args) -> dict:
         release(out)
     batch_ms = statistics.mean(times) * 1e3
     log(f"batched: {batch_ms:.2f} ms/step (device {statistics.mean(dev_ms):.2f} ms), {groups} groups")
+    stream_ms = None
+    if args.batch_graph:
+        # the same batch launched kernel by kernel on the stream, for comparison
+        ctx.tune("batch_graph", 0)
+        release(batch.run(ctx))
+        st = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            out = batch.run(ctx)
+            st.append(time.perf_counter() - t0)
+            release(out)
+        stream_ms = statistics.mean(st) * 1e3
+        ctx.tune("batch_graph", 1)
+        log(f"batched without the graph: {stream_ms:.2f} ms/step")
 
     # plain: one launch sequence per query, the same queries, warm
     for q in prepared[:64]:
@@ -239,6 +253,7 @@ def bench_batch(args) -> dict:
                    "graph": bool(args.batch_graph)},
         "batched_vs_plain": plain_ms / batch_ms,
         "plain_ms": plain_ms,
+        "stream_ms": stream_ms,
         "identical_results_checked": checked,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
@@ -512,6 +527,7 @@ def extras(args, ctx) -> dict:
     b = bench_batch(b_args)
     out["batch_4096"] = {"batched_ms": b["ms_per_step"], "plain_ms": b["plain_ms"],
                          "batched_vs_plain": b["batched_vs_plain"], "graph": True, "rows": b["config"]["rows"],
+                         "stream_ms": b["stream_ms"],
                          "identical_results_checked": b["identical_results_checked"],
                          "rows_per_s": b["value"], "query_hbm_GBps": b["query_hbm"]["GB/s"]}
     gc.collect()

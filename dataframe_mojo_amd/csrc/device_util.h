@@ -217,6 +217,27 @@ DFR_HD void fixsq_words(uint64_t bits, int S2, uint64_t (&w)[N]) {
   for (int i = 0; i < N; ++i) w[i] = bits_at(p, 2, 64 * i - sh);
 }
 
+// Word i of a 4-word value for a run-time i (0 outside 0..3). A select
+// chain: an array indexed at run time would live in scratch (or LDS).
+DFR_HD uint64_t pick4(const uint64_t (&w)[4], int i) {
+  return i == 0 ? w[0] : i == 1 ? w[1] : i == 2 ? w[2] : i == 3 ? w[3] : 0ull;
+}
+
+// Correctly rounded value of the non-negative 4-word x times 2^-scale
+// (round to odd into 64 bits, then one rounding to double).
+DFR_HD double words4_to_f64(const uint64_t (&x)[4], int scale) {
+  const int top = x[3] ? 3 : x[2] ? 2 : x[1] ? 1 : 0;
+  if (top == 0) return ldexp(u64_to_f64(x[0]), -scale);
+  const uint64_t hi = pick4(x, top), lo = pick4(x, top - 1);
+  const int lz = __builtin_clzll(hi);
+  uint64_t t = lz ? ((hi << lz) | (lo >> (64 - lz))) : hi;
+  uint64_t sticky = lz ? (lo << lz) : lo;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) sticky |= (i <= top - 2) ? x[i] : 0ull;
+  t |= sticky ? 1 : 0;
+  return ldexp(u64_to_f64(t), 64 * top - lz - scale);
+}
+
 // Correctly rounded value of the non-negative n-word x times 2^-scale
 // (round to odd into 64 bits, then one rounding to double).
 DFR_HD double words_to_f64(const uint64_t *x, int n, int scale) {
@@ -232,13 +253,19 @@ DFR_HD double words_to_f64(const uint64_t *x, int n, int scale) {
   return ldexp(u64_to_f64(t), 64 * top - lz - scale);
 }
 
-// Correctly rounded value of the two's-complement n-word x times 2^-scale.
+// Correctly rounded value of the two's-complement n-word x (n <= 4) times
+// 2^-scale.
 DFR_HD double sword_to_f64(const uint64_t *x, int n, int scale) {
-  uint64_t a[4] = {0, 0, 0, 0};
-  for (int i = 0; i < n && i < 4; ++i) a[i] = x[i];
-  const bool neg = (int64_t)a[n - 1] < 0;
-  if (neg) neg_words<4>(a);
-  const double d = words_to_f64(a, n, scale);
+  uint64_t a[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) a[i] = i < n ? x[i] : 0ull;
+  const bool neg = (int64_t)pick4(a, n - 1) < 0;
+  if (neg) {
+    neg_words<4>(a);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = i < n ? a[i] : 0ull;  // the n-word magnitude
+  }
+  const double d = words4_to_f64(a, scale);
   return neg ? -d : d;
 }
 
@@ -303,14 +330,12 @@ DFR_HD U256 mul128(uint64_t alo, uint64_t ahi, uint64_t blo, uint64_t bhi) {
 }
 
 DFR_HD U256 shl256(U256 x, int s) {
-  U256 r = {{0, 0, 0, 0}};
-  int q = s >> 6, b = s & 63;
-  for (int i = 3; i >= 0; --i) {
-    int src = i - q;
-    if (src < 0) continue;
-    uint64_t v = x.w[src] << b;
-    if (b && src - 1 >= 0) v |= x.w[src - 1] >> (64 - b);
-    r.w[i] = v;
+  U256 r;
+  const int q = s >> 6, b = s & 63;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint64_t hi = pick4(x.w, i - q), lo = pick4(x.w, i - q - 1);
+    r.w[i] = b ? ((hi << b) | (lo >> (64 - b))) : hi;
   }
   return r;
 }
@@ -334,21 +359,25 @@ DFR_HD bool sub256(U256 a, U256 b, U256 &r) {
 }
 
 // Correctly rounded value of x * 2^-scale.
-DFR_HD double u256_to_f64(U256 x, int scale) {
-  int top = 3;
-  while (top > 0 && x.w[top] == 0) --top;
-  if (top == 0) return ldexp(u64_to_f64(x.w[0]), -scale);
-  int lz = __builtin_clzll(x.w[top]);
-  // 64 significant bits starting at the leading one
-  uint64_t hi = x.w[top], lo = x.w[top - 1];
-  uint64_t t = lz ? ((hi << lz) | (lo >> (64 - lz))) : hi;
-  uint64_t sticky = lz ? (lo << lz) : lo;
-  for (int i = top - 2; i >= 0; --i) sticky |= x.w[i];
-  t |= sticky ? 1 : 0;
-  // the leading one is bit 64*top + 63 - lz, so t's lowest bit weighs
-  // 2^(64*top - lz)
-  int exp = 64 * top - lz;
-  return ldexp(u64_to_f64(t), exp - scale);
+DFR_HD double u256_to_f64(U256 x, int scale) { return words4_to_f64(x.w, scale); }
+
+// The variance finish of two-word totals (sum T at scale S, two's
+// complement; sum of squares Q at scale S2 <= 2S): n * Q * 2^(2S - S2) - T^2
+// exactly, in units of 2^-2S, rounded once, over n (n - ddof).
+DFR_HD double var_2words(uint64_t tlo, uint64_t thi, int S, uint64_t qlo, uint64_t qhi, int S2, uint64_t n,
+                         uint64_t ddof) {
+  if ((int64_t)thi < 0) {
+    tlo = ~tlo + 1;
+    thi = ~thi + (tlo == 0 ? 1 : 0);
+  }
+  const U256 nq = mul128(qlo, qhi, n, 0);
+  const U256 a = shl256(nq, 2 * S - S2);
+  const U256 t2 = mul128(tlo, thi, tlo, thi);
+  U256 r;
+  sub256(a, t2, r);
+  const double num = u256_to_f64(r, 2 * S);
+  const double den = (double)n * (double)(n - ddof);
+  return num / den;
 }
 
 // ---- 512-bit unsigned arithmetic for the variance finish of accumulators
@@ -431,6 +460,14 @@ DFR_HD double var_words(const uint64_t *sum, int nf, int S, const uint64_t *sq, 
   const double num = words_to_f64(r.w, 8, U);
   const double den = (double)n * (double)(n - ddof);
   return num / den;
+}
+
+// var_words out of line, for device code: its run-time indexed arrays then
+// live in its own frame, and a kernel that never takes the wide path pays
+// for them neither in registers nor in LDS.
+static __host__ __device__ __attribute__((noinline)) double var_words_at(const uint64_t *sum, int nf, int S, const uint64_t *sq, int nq, int S2, uint64_t n,
+                    uint64_t ddof) {
+  return var_words(sum, nf, S, sq, nq, S2, n, ddof);
 }
 
 }  // namespace dfr

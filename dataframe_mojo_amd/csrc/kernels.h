@@ -265,22 +265,9 @@ __device__ __forceinline__ double finish_var(const DevColAgg &A, const uint64_t 
                                              int ddof, uint64_t flags) {
   if (flags & (FL_NAN | FL_PINF | FL_NINF)) return __builtin_nan("");
   if (A.nfw > 2 || A.nqw > 2)
-    return var_words(b + A.o_fsum, A.nfw, A.fshift, b + A.o_sq, A.nqw, A.fshift2, n, (uint64_t)ddof);
-  // n * sumsq * 2^(2S - S2) - sum^2, exactly, in units of 2^-2S
-  uint64_t qlo = b[A.o_sq], qhi = b[A.o_sq + 1];
-  uint64_t tlo = b[A.o_fsum], thi = b[A.o_fsum + 1];
-  if ((int64_t)thi < 0) {
-    tlo = ~tlo + 1;
-    thi = ~thi + (tlo == 0 ? 1 : 0);
-  }
-  U256 nq = mul128(qlo, qhi, n, 0);
-  U256 a = shl256(nq, 2 * A.fshift - A.fshift2);
-  U256 t2 = mul128(tlo, thi, tlo, thi);
-  U256 r;
-  sub256(a, t2, r);
-  double num = u256_to_f64(r, 2 * A.fshift);
-  double den = (double)n * (double)(n - (uint64_t)ddof);
-  return num / den;
+    return var_words_at(b + A.o_fsum, A.nfw, A.fshift, b + A.o_sq, A.nqw, A.fshift2, n, (uint64_t)ddof);
+  return var_2words(b[A.o_fsum], b[A.o_fsum + 1], A.fshift, b[A.o_sq], b[A.o_sq + 1], A.fshift2, n,
+                    (uint64_t)ddof);
 }
 
 // Column `col`'s aggregate block and whether it is Float64. A register

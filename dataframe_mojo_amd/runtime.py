@@ -106,9 +106,30 @@ class Result:
         self.n_keys = n_keys
         self.n_aggs = n_aggs
         self.groups = int(ctx.lib.dfr_result_groups(handle))
-        self.device_ms = float(ctx.lib.dfr_result_device_ms(handle))
-        self.device_bytes = int(ctx.lib.dfr_result_device_bytes(handle))
-        self.phase_ms = [float(ctx.lib.dfr_result_phase_ms(handle, p)) for p in range(4)]
+        self._timing = None  # read on first use: a batch hands out thousands of results
+
+    def _times(self):
+        if self._timing is None:
+            if not self.handle:
+                raise ValueError("result released")
+            lib, h = self.ctx.lib, self.handle
+            self._timing = (float(lib.dfr_result_device_ms(h)), int(lib.dfr_result_device_bytes(h)),
+                            [float(lib.dfr_result_phase_ms(h, p)) for p in range(4)])
+        return self._timing
+
+    @property
+    def device_ms(self) -> float:
+        """Device time of the launch sequence that made this result (a
+        batch's whole sequence for a batched query)."""
+        return self._times()[0]
+
+    @property
+    def device_bytes(self) -> int:
+        return self._times()[1]
+
+    @property
+    def phase_ms(self) -> list:
+        return self._times()[2]
 
     def _copy(self, fn, i: int, dtype: int):
         n = self.groups

@@ -5,6 +5,9 @@
 //   sq <bits hex> <S2> <N>          the N-word image trunc(x^2 * 2^S2)
 //   tof <n> <scale> <words hex...>  an n-word two's-complement value * 2^-scale
 //   var <nf> <S> <nq> <S2> <n> <ddof> <sum words> <sq words>
+//   var2 <S> <S2> <n> <ddof> <sum lo hi> <sq lo hi>   the two-word finish
+//   u256 <scale> <4 words hex>      a non-negative 4-word value * 2^-scale
+//   shl <s> <4 words hex>           the 4-word value shifted left by s
 #include "../../dataframe_mojo_amd/csrc/device_util.h"
 #include <cstdio>
 #include <cstdlib>
@@ -44,6 +47,25 @@ int main() {
       double d = var_words(a, nf, S, b, nq, S2, n, dd);
       uint64_t bb; memcpy(&bb, &d, 8);
       printf("%016llx\n", (unsigned long long)bb);
+    } else if (!strcmp(op, "var2")) {
+      int S, S2; unsigned long long n, dd, tl, th, ql, qh;
+      scanf("%d %d %llu %llu %llx %llx %llx %llx", &S, &S2, &n, &dd, &tl, &th, &ql, &qh);
+      double d = var_2words(tl, th, S, ql, qh, S2, n, dd);
+      uint64_t bb; memcpy(&bb, &d, 8);
+      printf("%016llx\n", (unsigned long long)bb);
+    } else if (!strcmp(op, "u256") || !strcmp(op, "shl")) {
+      int k; scanf("%d", &k);
+      U256 x;
+      for (int i = 0; i < 4; ++i) { unsigned long long v; scanf("%llx", &v); x.w[i] = v; }
+      if (op[0] == 'u') {
+        double d = u256_to_f64(x, k);
+        uint64_t bb; memcpy(&bb, &d, 8);
+        printf("%016llx\n", (unsigned long long)bb);
+      } else {
+        U256 r = shl256(x, k);
+        for (int i = 0; i < 4; ++i) printf("%016llx ", (unsigned long long)r.w[i]);
+        printf("\n");
+      }
     }
     fflush(stdout);
   }

@@ -100,3 +100,30 @@ def test_variance_finish_and_its_error_bound(harness):
         true = sum((F(v) - mean) ** 2 for v in vals) / (n - ddof)
         bound = F(2) ** (ddof.bit_length() + 2) * max(F(2) ** -s2, F(2) ** (b + 1 - s))
         assert abs(F(got) - true) <= bound + abs(true) * F(1, 2**50)
+
+
+def test_two_word_variance_finish_and_256_bit_helpers(harness):
+    # the finish of two-word totals (the usual width), and the 256-bit
+    # shift and rounding it is made of, against exact rationals
+    rnd = random.Random(3)
+    for _ in range(400):
+        n = rnd.randint(2, 10**6) if rnd.random() < 0.5 else rnd.randint(2, 50)
+        b, L = rnd.randint(-40, 40), max(1, n.bit_length())
+        s, s2 = 126 - b - L, 126 - 2 * b - L
+        vals = [rnd.gauss(0, 1) * 2.0 ** (b - 1) for _ in range(min(n, 50))]
+        T = sum(trunc(F(v) * F(2) ** s) for v in vals) * (n // len(vals))
+        Q = sum(int(F(v) * F(v) * F(2) ** s2) for v in vals) * (n // len(vals))
+        ddof = rnd.choice([0, 1])
+        got = dbl(int(harness(f"var2 {s} {s2} {n} {ddof} {hexw(words(T, 2))} {hexw(words(Q, 2))}")[0], 16))
+        num = max(F(0), F(n) * F(Q) / F(2) ** s2 - (F(T) / F(2) ** s) ** 2)
+        assert got == float(num) / float(n * (n - ddof))
+    for _ in range(2000):
+        v = rnd.getrandbits(256) >> rnd.randint(0, 255)
+        if rnd.random() < 0.1:
+            v = rnd.choice([0, 1, 2**64, 2**128 - 1, 2**192, 2**255 + 7])
+        sc = rnd.randint(-100, 900)
+        exact = F(v) / F(2) ** sc
+        if exact < 2**1023:
+            assert dbl(int(harness(f"u256 {sc} {hexw(words(v, 4))}")[0], 16)) == float(exact)
+        sh = rnd.randint(0, 255)
+        assert [int(w, 16) for w in harness(f"shl {sh} {hexw(words(v, 4))}")] == words(v << sh, 4)
