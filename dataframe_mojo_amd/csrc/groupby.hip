@@ -1210,6 +1210,29 @@ void second_level_layout(dfr_context *ctx, Compiled &C, uint64_t records, int bo
 // The second level after the first scatter: per-chunk sub-partition counts,
 // their scans, the scatter into sub-partition order, then pass C per
 // sub-partition.
+// Whether the second-level pass C compiled at run time for this plan (its
+// fused form: each group's output row written by pass C) keeps more than a
+// few spill slots in scratch memory (kSubSpillBytes per lane). Such a kernel
+// can run many times slower than placing the groups in dense tables for the
+// finalize: when a register copy of the plan ended in scratch, 200M Zipf
+// rows with 8 aggregates took 63 ms in this pass. False when the
+// precompiled kernel would run.
+constexpr int kSubSpillBytes = 256;
+bool sub_fused_spills(dfr_context *ctx, const Compiled &C) {
+  const Tuning &T = ctx->tune;
+  if (!(T.jit == 2 || (T.jit == 1 && (C.rows >= T.jit_rows || C.jit_any_size)))) return false;
+  int ldsm, nc, u;
+  scan_shape(ctx, C, &ldsm, &nc, &u);
+  hipFunction_t fn = jit_function(ctx, C.plan, ldsm, nc, u, JIT_SUB);
+  if (!fn) return false;
+  int local = 0;
+  if (hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fn) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return local > kSubSpillBytes;
+}
+
 int launch_second_level(dfr_context *ctx, const Compiled &C, const DevPlan *dplan, char **error) {
   const DevPlan &P = C.plan;
   int st;
@@ -1302,7 +1325,7 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
     const bool dense = P.two_level != 0;
     // unordered second level: pass C writes the output rows itself, so
     // there are no state or key tables and no finalize pass
-    const bool fused = dense && !C.maintain_order;
+    bool fused = dense && !C.maintain_order;
     // unordered one level: pass C and the seeded keys' merge put each group
     // at the next slot of dense key and state tables sized by the most
     // groups the query can have (its packed key domain, at most its rows):
@@ -1477,12 +1500,22 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
             (st = S.get(nb + 1, &C.scan2_raw, error)) || (st = S.get(nb + 1, &C.scan2_par, error)))
           return st;
         if (fused) {
+          P.fused = 1;
+          if (sub_fused_spills(ctx, C)) {
+            // place the groups in dense tables and finish them afterwards
+            P.fused = 0;
+            fused = false;
+            if ((st = S.get(P.cap * (uint64_t)P.words, &P.state, error)) ||
+                (st = S.get(P.cap, &P.keys_tab, error)))
+              return st;
+          }
+        }
+        if (fused) {
           // every group has a record: the record total bounds the groups
           const uint64_t records = tot[0] + tot[1];
           if ((st = alloc_result(std::min<uint64_t>(records, P.cap)))) return st;
           for (int k = 0; k < P.n_keys; ++k) fill(ctx, P.out_key_valid[k], ((uint64_t)r->groups + 63) / 64 + 1, ~0ull);
           for (int a = 0; a < P.n_aggs; ++a) fill(ctx, P.aggs[a].out_valid, ((uint64_t)r->groups + 63) / 64 + 1, ~0ull);
-          P.fused = 1;
         }
         memcpy(ctx->staging, &P, sizeof P);
         HIP_TRY(hipMemcpyAsync(dplan, ctx->staging, sizeof P, hipMemcpyHostToDevice, ctx->stream));

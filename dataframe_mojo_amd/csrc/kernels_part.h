@@ -1547,6 +1547,18 @@ __global__ void __launch_bounds__(1024) k_sub_scatter(const DevPlan *__restrict_
 //
 // NA > 0 (a register copy of the plan) is the exact number of aggregates,
 // so their loop unrolls completely.
+// f(P.aggs[a]) for a = A .. NA - 1, each with a constant index: a loop over
+// them that the compiler declines to unroll (its body, the finish of any
+// aggregate, is large) indexes a register copy of the plan at run time,
+// which keeps the whole copy in scratch memory.
+template <int A, int NA, class PV, class F>
+__device__ __forceinline__ void each_agg(const PV &P, F &f) {
+  if constexpr (A < NA) {
+    f(P.aggs[A]);
+    each_agg<A + 1, NA>(P, f);
+  }
+}
+
 template <int NA, class PV>
 __device__ __forceinline__ void emit_group(const PV &P, uint64_t i, const uint64_t *slot, uint64_t packed) {
   const uint64_t clear = ~(1ull << (i & 63));
@@ -1576,8 +1588,7 @@ __device__ __forceinline__ void emit_group(const PV &P, uint64_t i, const uint64
     if (!o.valid) atomicAnd((unsigned long long *)&G.out_valid[i >> 6], (unsigned long long)clear);
   };
   if constexpr (RegPlan<PV>::value) {
-#pragma unroll
-    for (int a = 0; a < NA; ++a) one(P.aggs[a]);
+    each_agg<0, NA>(P, one);
   } else {
     for (int a = 0; a < P.n_aggs; ++a) one(cget(P.aggs[a]));
   }
