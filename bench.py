@@ -197,20 +197,21 @@ def bench_batch(args) -> dict:
         log(f"batched without the graph: {stream_ms:.2f} ms/step")
 
     # plain: one launch sequence per query, the same queries, warm
-    for q in prepared[:64]:
-        q.run().release()
-    log("timing the plain one-launch-per-query pass")
-    plain = []
-    t0 = time.perf_counter()
-    for q in prepared:
-        plain.append(q.run())
-    plain_ms = (time.perf_counter() - t0) * 1e3
-    log(f"plain: {plain_ms:.1f} ms for {len(prepared)} queries")
+    plain, plain_ms = [], None
+    if args.plain:
+        for q in prepared[:64]:
+            q.run().release()
+        log("timing the plain one-launch-per-query pass")
+        t0 = time.perf_counter()
+        for q in prepared:
+            plain.append(q.run())
+        plain_ms = (time.perf_counter() - t0) * 1e3
+        log(f"plain: {plain_ms:.1f} ms for {len(prepared)} queries")
 
     # identical results: every 16th query, batched vs plain, bit for bit
     out = batch.run(ctx)
     checked = 0
-    for i in range(0, len(jobs), 16):
+    for i in range(0, len(jobs) if plain else 0, 16):
         a, b = out[i], plain[i]
         if isinstance(a, Exception):
             continue
@@ -251,7 +252,7 @@ def bench_batch(args) -> dict:
                    "rows_per_query": "log-uniform [1e3, 1e7]", "key_cardinality": "log-uniform [1, 1e5]",
                    "filter": "value > t, t uniform in [-1e6, 1e6]", "aggs": "len, sum/max(value), sum/mean(x)",
                    "graph": bool(args.batch_graph)},
-        "batched_vs_plain": plain_ms / batch_ms,
+        "batched_vs_plain": plain_ms / batch_ms if plain_ms else None,
         "plain_ms": plain_ms,
         "stream_ms": stream_ms,
         "identical_results_checked": checked,
@@ -477,6 +478,8 @@ def main() -> int:
     ap.add_argument("--serve-graph", type=int, default=0, help="serve: lanes capture each batch as a HIP graph")
     ap.add_argument("--queries", type=int, default=4096, help="batch_4096: queries per batch")
     ap.add_argument("--batch-graph", type=int, default=0, help="batch_4096: run the batch as a captured HIP graph")
+    ap.add_argument("--plain", type=int, default=1,
+                    help="batch_4096: also time one dfr_groupby call per query and compare results (0: batch only)")
     ap.add_argument("--rows", type=int, default=0, help="rows (default: the workload's own)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=50_000_000)

@@ -15,6 +15,11 @@ Two uses, both from the repository root after a profiled run of bench.py:
         WRITE_DIR/run_counter_collection.csv --kernel dfr_scan_jit --copy-bytes BYTES \\
         --workload zipf_200m --rows 200000000 --command "..." > profiles/traffic_<workload>.json
 
+  every counter of a counter run, as the mean per dispatch of each kernel
+  whose name contains one of the given names:
+    python -m dataframe_mojo_amd.profiling counters DIR/run_counter_collection.csv \
+        --kernels k_batch_aggregate,dfr_scan_jit
+
 bench.py reports a workload's ``roofline.traffic`` from
 profiles/traffic_<workload>.json when its kernel is the one the bench ran.
 """
@@ -85,6 +90,22 @@ def traffic(fetch_csv: str, write_csv: str, kernel: str, copy_bytes: int) -> dic
     }
 
 
+def counter_means(path: str, names: list[str]) -> dict[str, dict]:
+    """Per name: the dispatches of the kernels whose name contains it, and
+    each counter's mean per dispatch over them."""
+    sums: dict[str, dict[str, float]] = {n: {} for n in names}
+    disp: dict[str, set] = {n: set() for n in names}
+    for r in csv.DictReader(open(path)):
+        hit = next((n for n in names if n in r["Kernel_Name"]), None)
+        if hit is None:
+            continue
+        c = r["Counter_Name"]
+        sums[hit][c] = sums[hit].get(c, 0.0) + float(r["Counter_Value"])
+        disp[hit].add(r["Dispatch_Id"])
+    return {n: {"dispatches": len(disp[n]),
+                "mean": {c: v / max(1, len(disp[n])) for c, v in sorted(sums[n].items())}} for n in names}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m dataframe_mojo_amd.profiling")
     sub = ap.add_subparsers(dest="what", required=True)
@@ -100,9 +121,15 @@ def main(argv=None) -> int:
     t.add_argument("--workload", required=True)
     t.add_argument("--rows", type=int, required=True)
     t.add_argument("--command", required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("csv")
+    c.add_argument("--kernels", required=True)
     a = ap.parse_args(argv)
     if a.what == "stats":
         sys.stdout.write(kernel_stats_md(a.csv, a.title, a.command))
+    elif a.what == "counters":
+        json.dump(counter_means(a.csv, a.kernels.split(",")), sys.stdout, indent=1)
+        sys.stdout.write("\n")
     else:
         out = traffic(a.fetch_csv, a.write_csv, a.kernel, a.copy_bytes)
         out["workload"] = a.workload
