@@ -193,6 +193,15 @@ int ensure_batch_staging(dfr_context *ctx, size_t bytes, char **error) {
   return DFR_OK;
 }
 
+// The context's first n side streams and the fork and join events.
+int ensure_side_streams(dfr_context *ctx, int n, char **error) {
+  for (int i = 0; i <= n; ++i)
+    if (!ctx->side_ev[i]) HIP_TRY(hipEventCreateWithFlags(&ctx->side_ev[i], hipEventDisableTiming));
+  for (int i = 0; i < n; ++i)
+    if (!ctx->side[i]) HIP_TRY(hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking));
+  return DFR_OK;
+}
+
 size_t out_col_bytes(int dtype, uint64_t cap) {
   const size_t b = (size_t)cap * ((dtype == DFR_INT32 || dtype == DFR_DICT) ? 4 : 8);
   return (b + 255) / 256 * 256 + ((cap + 63) / 64 + 1) * 8;
@@ -474,6 +483,15 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
     drop_all();
     return st;
   }
+  // the partitioned queries run on side streams, beside the batch kernel and
+  // each other (they write only their own tables and buffers)
+  int n_parted = 0;
+  for (int j = 0; j < m; ++j) n_parted += parted[j] && C[live[j]].rows > 0;
+  const int ns = std::min(std::min(ctx->tune.batch_streams, dfr_context::kSideStreams), n_parted);
+  if (ns >= 2 && (st = ensure_side_streams(ctx, ns, error))) {
+    drop_all();
+    return st;
+  }
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
@@ -488,16 +506,31 @@ extern "C" int dfr_groupby_batch(dfr_context *ctx, const dfr_query *queries, int
     fill(ctx, occ + occ_words, 1, 0);
     hipLaunchKernelGGL(k_batch_init, dim3((unsigned)span_blocks), dim3(kBlock), 0, ctx->stream, dplans, dboff, m);
     if (!graph) hipEventRecord(ctx->evp[0], ctx->stream);
+    hipStream_t main = ctx->stream;
+    if (ns >= 2) {
+      hipEventRecord(ctx->side_ev[0], main);  // after the tables' set-up
+      for (int k = 0; k < ns; ++k) hipStreamWaitEvent(ctx->side[k], ctx->side_ev[0], 0);
+    }
+    for (int j = 0, k = 0; j < m; ++j) {
+      if (!parted[j] || C[live[j]].rows == 0) continue;
+      if (ns >= 2) ctx->stream = ctx->side[k++ % ns];
+      st = launch_partitioned(ctx, C[live[j]], dplans + j, error, !graph && ns < 2);
+      ctx->stream = main;
+      if (st) {
+        hipStreamSynchronize(main);
+        for (int q = 0; q < ns; ++q) hipStreamSynchronize(ctx->side[q]);
+        drop_all();
+        return st;
+      }
+    }
     if (n_items) {
       hipLaunchKernelGGL(kagg, dim3(grid), dim3(block), lds_max, ctx->stream, dplans,
                          ditems, (int64_t)n_items, (unsigned long long *)next);
     }
-    for (int j = 0; j < m; ++j) {
-      if (!parted[j] || C[live[j]].rows == 0) continue;
-      if ((st = launch_partitioned(ctx, C[live[j]], dplans + j, error, !graph))) {
-        hipStreamSynchronize(ctx->stream);
-        drop_all();
-        return st;
+    if (ns >= 2) {
+      for (int k = 0; k < ns; ++k) {
+        hipEventRecord(ctx->side_ev[1 + k], ctx->side[k]);
+        hipStreamWaitEvent(main, ctx->side_ev[1 + k], 0);
       }
     }
     if (!graph) hipEventRecord(ctx->evp[1], ctx->stream);

@@ -270,6 +270,8 @@ extern "C" void dfr_context_destroy(dfr_context *ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  for (hipStream_t s : ctx->side)
+    if (s) hipStreamSynchronize(s);
   for (auto &kv : ctx->pool.live) hipFree(kv.first);
   ctx->pool.live.clear();
   ctx->pool.trim();
@@ -278,6 +280,10 @@ extern "C" void dfr_context_destroy(dfr_context *ctx) {
   hipEventDestroy(ctx->ev0);
   hipEventDestroy(ctx->ev1);
   for (hipEvent_t ev : ctx->evp) hipEventDestroy(ev);
+  for (hipStream_t s : ctx->side)
+    if (s) hipStreamDestroy(s);
+  for (hipEvent_t ev : ctx->side_ev)
+    if (ev) hipEventDestroy(ev);
   hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -343,6 +349,8 @@ extern "C" int dfr_context_tune(dfr_context *ctx, const char *key, int64_t value
     t.sub_wg_per_cu = (int)value;
   else if (k == "batch_tile" && value >= 2048 && value <= (1 << 30))
     t.batch_tile = (int)value;
+  else if (k == "batch_streams" && value >= 0 && value <= dfr_context::kSideStreams)
+    t.batch_streams = (int)value;
   else if (k == "batch_fill" && value >= 0 && value <= 64)
     t.batch_fill = (int)value;
   else if (k == "batch_graph" && (value == 0 || value == 1))
@@ -400,7 +408,7 @@ extern "C" int dfr_context_tune_get(const dfr_context *ctx, const char *key, int
       {"hash_probe", t.hash_probe}, {"hash_slots", t.hash_slots}, {"agg_mode", t.agg_mode},
       {"part_lds_bytes", t.part_lds_bytes}, {"scan_unroll", t.scan_unroll}, {"two_level", t.two_level},
       {"max_parts", t.max_parts}, {"sub_lds_bytes", t.sub_lds_bytes}, {"sub_block", t.sub_block},
-      {"sub_fill", t.sub_fill}, {"sub_wg_per_cu", t.sub_wg_per_cu}, {"batch_tile", t.batch_tile}, {"batch_fill", t.batch_fill},
+      {"sub_fill", t.sub_fill}, {"sub_wg_per_cu", t.sub_wg_per_cu}, {"batch_tile", t.batch_tile}, {"batch_fill", t.batch_fill}, {"batch_streams", t.batch_streams},
       {"batch_graph", t.batch_graph}, {"batch_part_rows", t.batch_part_rows},
       {"scatter_slices", t.scatter_slices}, {"scatter_stage", t.scatter_stage}, {"hot_keys", t.hot_keys},
       {"hot_rows", t.hot_rows}, {"hot_domain", t.hot_domain}, {"hot_sample", t.hot_sample},

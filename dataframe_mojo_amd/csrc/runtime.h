@@ -48,6 +48,7 @@ struct Tuning {
   int sub_wg_per_cu = -1;      // its workgroups per CU (-1: as many as fit at once; 0: one per sub-partition)
   int two_level = -1;          // second partition level: -1 when groups may outgrow one level, 0 never, 1 always
   int batch_tile = 131072;     // rows of a batch work item
+  int batch_streams = 4;       // streams a batch's partitioned queries are spread over (0, 1: the context's own)
   int batch_fill = 1;          // a batch of few rows is cut into this many items per workgroup (0: never)
   int batch_graph = 0;         // 1: a batch's launch sequence runs as one captured HIP graph
   int64_t batch_part_rows = 1 << 22;  // a batch query of this many rows whose groups outgrow the batch
@@ -101,6 +102,11 @@ struct dfr_context {
   size_t bstaging_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evp[4] = {nullptr, nullptr, nullptr, nullptr};  // phase boundaries
+  // side streams of a batch's partitioned queries and their fork and join
+  // events (created on first use)
+  static constexpr int kSideStreams = 8;
+  hipStream_t side[kSideStreams] = {};
+  hipEvent_t side_ev[kSideStreams + 1] = {};  // [0] fork, [1 + s] side stream s's join
   dfr_observation obs = {};
   dfr::Tuning tune;
   // A server's lane context outlives its server while results it produced

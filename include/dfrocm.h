@@ -153,7 +153,9 @@ int dfr_context_calibration(const dfr_context *ctx, char **text, char **error);
  * (rows of a batch work item, at most), "batch_fill" (a batch of fewer rows
  * than batch_tile x this x the grid's workgroups is cut into shorter items,
  * this many per workgroup, none shorter than a query's LDS table; 0: items of
- * batch_tile rows only), "batch_graph" (1: a batch's launch sequence
+ * batch_tile rows only), "batch_streams" (a batch's partitioned queries run on this many
+ * side streams beside the batch kernel, 0 to 8; 0 and 1: on the context's
+ * stream), "batch_graph" (1: a batch's launch sequence
  * is captured and run as one HIP graph), "batch_part_rows" (a batch query
  * with at least this many rows whose key domain outgrows the batch kernel's
  * LDS table runs the partitioned passes instead, after the batch kernel;

@@ -74,6 +74,8 @@ def frames_equal_bitwise(a, b, n_keys):
 
 @pytest.mark.parametrize("setting", [{}, {"batch_graph": 1}, {"batch_tile": 2048}, {"block": 256, "lds_bytes": 16384},
                                      {"batch_fill": 0}, {"batch_fill": 8, "batch_graph": 1},
+                                     {"batch_part_rows": 1000, "batch_streams": 1},
+                                     {"batch_part_rows": 1000, "batch_streams": 8, "batch_graph": 1},
                                      {"batch_part_rows": 0}, {"batch_part_rows": 1000},
                                      {"batch_part_rows": 1000, "block": 256, "lds_bytes": 16384}])
 def test_batch_equals_one_launch_per_query(setting, gpu_ctx):
@@ -83,7 +85,7 @@ def test_batch_equals_one_launch_per_query(setting, gpu_ctx):
             gpu_ctx.tune(k, v)
         batch = gpu_ctx.groupby_batch(jobs)
     finally:
-        for k, v in {**TUNE_DEFAULTS, "batch_graph": 0, "batch_tile": 131072, "batch_fill": 1,
+        for k, v in {**TUNE_DEFAULTS, "batch_graph": 0, "batch_tile": 131072, "batch_fill": 1, "batch_streams": 4,
                      "batch_part_rows": 1 << 22}.items():
             gpu_ctx.tune(k, v)
     for (plan, keys, cols), got in zip(jobs, batch):
