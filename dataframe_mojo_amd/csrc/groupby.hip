@@ -1326,13 +1326,13 @@ int run_query(dfr_context *ctx, Compiled &C, dfr_result **out, char **error) {
     // unordered second level: pass C writes the output rows itself, so
     // there are no state or key tables and no finalize pass
     bool fused = dense && !C.maintain_order;
-    // unordered one level: pass C and the seeded keys' merge put each group
-    // at the next slot of dense key and state tables sized by the most
-    // groups the query can have (its packed key domain, at most its rows):
-    // no table set-up, occupancy scan or compaction, and the finalize
-    // reads the tables in slot order
+    // one level: pass C and the seeded keys' merge put each group at the
+    // next slot of dense key and state tables sized by the most groups the
+    // query can have (its packed key domain, at most its rows): no table
+    // set-up, occupancy scan or compaction; the finalize reads the tables
+    // in slot order, or ordered, in the first-row order of those slots
     const uint64_t one_cap = std::min<uint64_t>(P.domain, (uint64_t)std::max<int64_t>(C.rows, 0));
-    const bool fused1 = !dense && !C.maintain_order && P.agg_mode == AGG_PARTITION && C.rows > 0 &&
+    const bool fused1 = !dense && P.agg_mode == AGG_PARTITION && C.rows > 0 &&
                         ctx->tune.fuse_one_level && !no_fuse && one_cap > 0 && one_cap <= ((uint64_t)1 << 28) &&
                         one_cap * (uint64_t)(P.words + 1) * 8 <=
                             std::min<uint64_t>((uint64_t)1 << 31, (uint64_t)ctx->pool.budget / 8);
