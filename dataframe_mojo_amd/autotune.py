@@ -256,9 +256,27 @@ def tune_class(ctx, cls: str, work: Workload, reps: int, min_gain: float, knobs=
         ctx.tune(key, chosen.get(key, cur))
         notes[key] = {"ms": {str(k): round(t * 1e3, 4) for k, t in results.items()},
                       "chosen": chosen.get(key, cur)}
+    confirm = None
+    if chosen:
+        # the chosen settings against the built-in ones, measured back to
+        # back: a descent step can win on noise, and steps can add up to
+        # nothing
+        t_new, _ = measure(ctx, work, reps)
+        for k in chosen:
+            ctx.tune(k, default_value(k))
+        t_old, _ = measure(ctx, work, reps)
+        confirm = {"chosen_ms": round(t_new * 1e3, 4), "builtin_ms": round(t_old * 1e3, 4)}
+        if t_new < t_old * (1 - min_gain):
+            for k, v in chosen.items():
+                ctx.tune(k, v)
+            best_t = t_new
+        else:
+            log(f"{cls}: {chosen} not confirmed ({t_new * 1e3:.3f} against {t_old * 1e3:.3f} ms): built-in kept")
+            chosen = {}
+            best_t = t_old
     log(f"{cls}: tuned {best_t * 1e3:.3f} ms (built-in {base_t * 1e3:.3f} ms): {chosen}")
     return chosen, {"workload": work.__class__.__name__, "builtin_ms": round(base_t * 1e3, 4),
-                    "tuned_ms": round(best_t * 1e3, 4), "knobs": notes}
+                    "tuned_ms": round(best_t * 1e3, 4), "knobs": notes, "confirm": confirm}
 
 
 _DEFAULTS: dict = {}

@@ -155,3 +155,48 @@ def test_result_digest_is_order_independent_and_row_coupled():
     m2 = valid.copy()
     m2[7] = not m2[7]
     assert autotune._digest_result(_FakeResult([(k, ones)], [(v, m2)]), 1, 1) != base
+
+
+class _FakeCtx:
+    """A context whose settings are only recorded."""
+
+    def __init__(self, defaults):
+        self.values = dict(defaults)
+
+    def tune(self, key, value):
+        self.values[key] = value
+
+
+def _scripted_measure(times):
+    """measure() replaced by times from a script: each call pops the next
+    time for the context's current settings (or the last one left)."""
+    def measure(ctx, work, reps):
+        key = tuple(sorted(ctx.values.items()))
+        seq = times[key]
+        t = seq.pop(0) if len(seq) > 1 else seq[0]
+        return t, "digest"
+    return measure
+
+
+@pytest.mark.parametrize("confirmed", [True, False])
+def test_tuned_settings_are_kept_only_when_confirmed(confirmed, monkeypatch):
+    # a knob that wins during the descent is kept only if it still beats
+    # the built-in settings, measured back to back at the end
+    defaults = {"a": 1, "b": 10}
+    monkeypatch.setattr(autotune, "_DEFAULTS", dict(defaults))
+    base, win = (("a", 1), ("b", 10)), (("a", 2), ("b", 10))
+    # built-in: the first measure, candidate a=1, then the confirmation
+    times = {
+        base: [10.0, 10.0, 10.0 if confirmed else 9.8],
+        win: [9.6, 9.6 if confirmed else 9.9],
+        (("a", 3), ("b", 10)): [10.1],
+        (("a", 2), ("b", 20)): [10.5],
+    }
+    monkeypatch.setattr(autotune, "measure", _scripted_measure(times))
+    ctx = _FakeCtx(defaults)
+    chosen, notes = autotune.tune_class(ctx, "x", object(), 1, 0.02, knobs=[("a", [1, 2, 3]), ("b", [10, 20])])
+    if confirmed:
+        assert chosen == {"a": 2} and ctx.values == {"a": 2, "b": 10}
+    else:
+        assert chosen == {} and ctx.values == defaults
+    assert notes["confirm"] is not None
